@@ -1,0 +1,1087 @@
+// fs2d_capi.hip -- C ABI (include/fs2d.h) over the gfx950 kernels.
+//
+// Host orchestration of FluidSim2D::runFrame (src/FluidSim2D.cpp:94-138): one
+// HIP stream per handle, all state resident in HBM, no per-step allocation
+// (the reference allocates temporaries in every stage, FluidSim2D.cpp:147,
+// 186, 253, 334, 366, 390, 399, 471).  The PCG runs its iterations without
+// host round trips; the host reads the convergence flag once per batch.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/fs2d.h"
+#include "fs2d_kernels.hpp"
+
+using namespace fs2d;
+
+namespace {
+
+enum KernelClass {
+    K_SL_ADVECT, K_GRAVITY, K_ASSEMBLE, K_FACTOR, K_PCG_FWD, K_PCG_BWD, K_PCG_SPMV, K_PCG_AXPY,
+    K_UPDATE_VEL, K_EXTRAPOLATE, K_PARTICLE_SORT, K_P2G, K_G2P, K_PARTICLE_ADVECT,
+    K_LS_SWEEP, K_LS_OTHER, K_DIAGNOSTICS, K_MISC, K_COUNT
+};
+const char* kKernelNames[K_COUNT] = {
+    "sl_advect", "gravity", "assemble", "mic0_factor", "pcg_forward_solve", "pcg_backward_solve",
+    "pcg_spmv_dot", "pcg_axpy_norm", "update_velocity", "extrapolate", "particle_sort", "p2g", "g2p",
+    "particle_advect", "levelset_sweep", "levelset_other", "diagnostics", "misc"};
+
+constexpr int kMaxBlocks = 2048;
+
+int blocks_for(long n) {
+    long b = (n + kBlock - 1) / kBlock;
+    return (int)std::max(1L, std::min(b, (long)kMaxBlocks));
+}
+
+struct TimedLaunch {
+    int cls;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct fs2d_sim {
+    int nx = 0, ny = 0, k = 0, mode = 0, flags = 0;
+    double dt = 0, dx = 0, dr = 0, rho = 0, gx = 0, gy = 0, alpha = 0;
+    double tau = 0.999, sigma = 0.25, tol = 1e-12;
+    int max_iters = 200;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    // grid state
+    float *u = nullptr, *v = nullptr, *u2 = nullptr, *v2 = nullptr, *nu = nullptr, *nv = nullptr;
+    float *dens = nullptr, *dens2 = nullptr;
+    uint8_t* cell = nullptr;
+    double *phi = nullptr, *phi2 = nullptr;
+    int *t0 = nullptr, *t1 = nullptr;
+    // projection
+    float *p = nullptr, *r = nullptr, *z = nullptr, *q = nullptr, *w = nullptr, *s[2] = {nullptr, nullptr};
+    float *Adiag = nullptr, *Ax = nullptr, *Ay = nullptr, *P = nullptr, *ex = nullptr, *ey = nullptr;
+    // extrapolation
+    uint32_t *mu = nullptr, *mv = nullptr;
+    int *fu0 = nullptr, *fu1 = nullptr, *fv0 = nullptr, *fv1 = nullptr, *fcount = nullptr;
+    // particles
+    float *px = nullptr, *py = nullptr, *pvx = nullptr, *pvy = nullptr;
+    long np = 0, np_cap = 0;
+    int *pkey = nullptr, *counts = nullptr, *starts = nullptr, *cursor = nullptr, *sorted = nullptr;
+    int* tile_sums = nullptr;
+    bool sorted_valid = false;
+    // scalars
+    PcgState* st = nullptr;
+    Diag* dg = nullptr;
+    double* partials = nullptr;
+    unsigned* counters = nullptr;
+    unsigned long long* handoff = nullptr;
+    unsigned* wctl = nullptr;
+    int* werr = nullptr;
+    PcgState* h_st = nullptr;
+    Diag* h_dg = nullptr;
+    int* h_err = nullptr;
+
+    std::vector<void*> allocs;
+    double current_time = 0, orig_volume = 0;
+    long long steps = 0;
+    int last_iters_hint = 8;
+    std::string err;
+
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> event_pool;
+    double ktime[K_COUNT] = {};
+    long long kcount[K_COUNT] = {};
+};
+
+namespace {
+
+fs2d_status fail(fs2d_sim* sim, fs2d_status st, const char* fmt, ...) {
+    if (sim) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        sim->err = buf;
+    }
+    return st;
+}
+
+#define HIPCHK(sim, x)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess)                                                               \
+            return fail(sim, e_ == hipErrorOutOfMemory ? FS2D_ERR_OOM : FS2D_ERR_DEVICE, "%s: %s", #x, \
+                        hipGetErrorString(e_));                                             \
+    } while (0)
+
+hipEvent_t get_event(fs2d_sim* s) {
+    if (!s->event_pool.empty()) {
+        hipEvent_t e = s->event_pool.back();
+        s->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Times one launch on the handle's stream when timing is enabled.
+struct Timed {
+    fs2d_sim* s;
+    int cls;
+    hipEvent_t a = nullptr;
+    Timed(fs2d_sim* s_, int c) : s(s_), cls(c) {
+        if (s->timing) {
+            a = get_event(s);
+            if (a) (void)hipEventRecord(a, s->stream);
+        }
+    }
+    ~Timed() {
+        if (a) {
+            hipEvent_t b = get_event(s);
+            if (b) {
+                (void)hipEventRecord(b, s->stream);
+                s->pending.push_back({cls, a, b});
+            } else {
+                s->event_pool.push_back(a);
+            }
+        }
+    }
+};
+
+void resolve_timing(fs2d_sim* s) {
+    for (auto& t : s->pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            s->ktime[t.cls] += ms;
+            s->kcount[t.cls] += 1;
+        }
+        s->event_pool.push_back(t.a);
+        s->event_pool.push_back(t.b);
+    }
+    s->pending.clear();
+}
+
+template <class T>
+fs2d_status dalloc(fs2d_sim* s, T** p, size_t n) {
+    void* ptr = nullptr;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&ptr, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(s, FS2D_ERR_OOM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    s->allocs.push_back(ptr);
+    *p = (T*)ptr;
+    e = hipMemsetAsync(ptr, 0, bytes, s->stream);
+    if (e != hipSuccess) return fail(s, FS2D_ERR_DEVICE, "hipMemsetAsync: %s", hipGetErrorString(e));
+    return FS2D_OK;
+}
+
+#define TRY(x)                           \
+    do {                                 \
+        fs2d_status st_ = (x);           \
+        if (st_ != FS2D_OK) return st_;  \
+    } while (0)
+
+template <class T>
+void fill(fs2d_sim* s, T* a, T v, size_t n) {
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(blocks_for((long)n)), dim3(kBlock), 0, s->stream, a, v, n);
+}
+template <class T>
+void dcopy(fs2d_sim* s, T* dst, const T* src, size_t n) {
+    hipLaunchKernelGGL(copy_kernel<T>, dim3(blocks_for((long)n)), dim3(kBlock), 0, s->stream, dst, src, n);
+}
+
+fs2d_status h2d(fs2d_sim* s, void* dst, const void* src, size_t bytes) {
+    HIPCHK(s, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return FS2D_OK;
+}
+fs2d_status d2h(fs2d_sim* s, void* dst, const void* src, size_t bytes) {
+    HIPCHK(s, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return FS2D_OK;
+}
+
+long n_u(const fs2d_sim* s) { return (long)(s->nx + 1) * s->ny; }
+long n_v(const fs2d_sim* s) { return (long)s->nx * (s->ny + 1); }
+long n_c(const fs2d_sim* s) { return (long)s->nx * s->ny; }
+
+MacView mac_view(const fs2d_sim* s) { return MacView{s->u, s->v, s->nx, s->ny, s->dx}; }
+
+WaveCtl wave_ctl(fs2d_sim* s) { return WaveCtl{s->handoff, s->wctl, s->werr}; }
+
+template <class Op>
+void launch_wavefront(fs2d_sim* s, const Op& op) {
+    const int nstrips = (s->ny + 63) / 64;
+    hipLaunchKernelGGL(wavefront_kernel<Op>, dim3(nstrips), dim3(64), 0, s->stream, op, s->nx, s->ny, wave_ctl(s));
+}
+
+fs2d_status ensure_particle_capacity(fs2d_sim* s, long n) {
+    if (n <= s->np_cap) return FS2D_OK;
+    float* a[4];
+    for (int k = 0; k < 4; k++) TRY(dalloc(s, &a[k], (size_t)n));
+    int *key, *sorted;
+    TRY(dalloc(s, &key, (size_t)n));
+    TRY(dalloc(s, &sorted, (size_t)n));
+    s->px = a[0]; s->py = a[1]; s->pvx = a[2]; s->pvy = a[3];
+    s->pkey = key; s->sorted = sorted;
+    s->np_cap = n;
+    return FS2D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Stages.
+// ---------------------------------------------------------------------------
+void sort_particles(fs2d_sim* s) {
+    if (s->sorted_valid) return;
+    Timed t(s, K_PARTICLE_SORT);
+    const long nc = n_c(s);
+    fill(s, s->counts, 0, (size_t)nc);
+    if (s->np > 0)
+        hipLaunchKernelGGL(particle_keys_kernel, dim3(blocks_for(s->np)), dim3(kBlock), 0, s->stream, s->px, s->py,
+                           s->np, s->dx, s->nx, s->ny, s->pkey, s->counts);
+    const int ntiles = (int)((nc + 1023) / 1024);
+    hipLaunchKernelGGL(scan_tiles_kernel, dim3(ntiles), dim3(kBlock), 0, s->stream, s->counts, s->starts,
+                       s->tile_sums, nc);
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(64), 0, s->stream, s->tile_sums, ntiles);
+    hipLaunchKernelGGL(scan_add_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->starts, s->tile_sums, nc);
+    if (s->np > 0) {
+        dcopy(s, s->cursor, s->starts, (size_t)nc);
+        hipLaunchKernelGGL(particle_scatter_kernel, dim3(blocks_for(s->np)), dim3(kBlock), 0, s->stream, s->pkey,
+                           s->np, s->cursor, s->sorted);
+        hipLaunchKernelGGL(segment_sort_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->starts,
+                           s->counts, s->sorted, nc);
+    }
+    s->sorted_valid = true;
+}
+
+void extrapolate_uv(fs2d_sim* s, float* uval, float* vval) {
+    Timed t(s, K_EXTRAPOLATE);
+    ExtraArr A{uval, s->mu, s->fu0, s->fu1, s->fcount + 0, s->nx + 1, s->ny};
+    ExtraArr B{vval, s->mv, s->fv0, s->fv1, s->fcount + 1, s->nx, s->ny + 1};
+    fill(s, s->fcount, 0, 2);
+    hipLaunchKernelGGL(extrap_layer1_kernel, dim3(blocks_for(n_u(s) + n_v(s))), dim3(kBlock), 0, s->stream, A, B);
+    hipLaunchKernelGGL(extrap_layers_kernel, dim3(2), dim3(1024), 0, s->stream, A, B);
+}
+
+void diagnostics(fs2d_sim* s) {
+    Timed t(s, K_DIAGNOSTICS);
+    const long n = n_c(s) + s->np;
+    const double pmass = s->k > 0 ? s->rho * s->dx * s->dx / (s->k * s->k) : 0.0;
+    hipLaunchKernelGGL(diagnostics_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s->stream, s->cell, mac_view(s),
+                       s->px, s->py, s->pvx, s->pvy, s->np, s->rho, s->gx, s->gy, pmass, s->dg, s->partials,
+                       s->counters + 3);
+}
+
+// FluidSim2D::createWaterLevelSet (src/FluidSim2D.cpp:653-732).
+void stage_level_set(fs2d_sim* s) {
+    if (!(s->flags & FS2D_FLAG_STATIC_CELLS)) {
+        const long nc = n_c(s);
+        sort_particles(s);
+        {
+            Timed t(s, K_LS_OTHER);
+            hipLaunchKernelGGL(ls_seed_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->px, s->py,
+                               s->starts, s->counts, s->sorted, s->t0, s->nx, nc, s->dx, s->dr);
+        }
+        int* tin = s->t0;
+        int* tout = s->t1;
+        for (int rep = 0; rep < 4; rep++)
+            for (int d = 0; d < 4; d++) {
+                Timed t(s, K_LS_SWEEP);
+                ClosestOp op{tin, tout, s->px, s->py, s->nx, s->ny, d & 1, d >> 1, s->dx, s->dr};
+                launch_wavefront(s, op);
+                std::swap(tin, tout);
+            }
+        double* a = s->phi;
+        double* b = s->phi2;
+        {
+            Timed t(s, K_LS_OTHER);
+            hipLaunchKernelGGL(ls_phi_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, tin, s->px, s->py,
+                               b, s->nx, nc, s->dx, s->dr);
+            hipLaunchKernelGGL(ls_surface_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, b, a, s->nx,
+                               s->ny);
+        }
+        for (int rep = 0; rep < 4; rep++)
+            for (int d = 0; d < 4; d++) {
+                Timed t(s, K_LS_SWEEP);
+                EikonalOp op{a, b, s->nx, s->ny, d & 1, d >> 1, s->dx};
+                launch_wavefront(s, op);
+                std::swap(a, b);
+            }
+        {
+            Timed t(s, K_LS_OTHER);
+            for (int rep = 0; rep < 2; rep++) {
+                hipLaunchKernelGGL(ls_smooth_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, a, b, s->nx,
+                                   s->ny);
+                std::swap(a, b);
+            }
+            // 16 sweeps and 2 smoothing passes: the result is back in a == s->phi
+            hipLaunchKernelGGL(classify_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->cell, a, nc);
+        }
+    }
+    diagnostics(s);
+}
+
+// applySemiLagrangianAdvection (src/FluidSim2D.cpp:206-235), optionally fused
+// with applyGravity.
+void stage_sl(fs2d_sim* s, bool with_gravity) {
+    Timed t(s, K_SL_ADVECT);
+    const bool dens = (s->flags & FS2D_FLAG_DENSITY) != 0;
+    const long n = n_u(s) + n_v(s) + (dens ? n_c(s) : 0);
+    const float gu = with_gravity ? (float)(s->dt * s->gx) : 0.0f;
+    const float gv = with_gravity ? (float)(s->dt * s->gy) : 0.0f;
+    hipLaunchKernelGGL(sl_advect_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s->stream, mac_view(s), s->u2, s->v2,
+                       dens ? s->dens : nullptr, s->dens2, s->dt, clamp_box(s->nx, s->ny, s->dx), gu, gv);
+    std::swap(s->u, s->u2);
+    std::swap(s->v, s->v2);
+    if (dens) std::swap(s->dens, s->dens2);
+}
+
+void stage_gravity(fs2d_sim* s) {
+    Timed t(s, K_GRAVITY);
+    hipLaunchKernelGGL(gravity_kernel, dim3(blocks_for(n_u(s) + n_v(s))), dim3(kBlock), 0, s->stream, s->u, s->v,
+                       n_u(s), n_v(s), (float)(s->dt * s->gx), (float)(s->dt * s->gy));
+}
+
+// transferVelocityToGrid (src/FluidSim2D.cpp:144-204).
+void stage_p2g(fs2d_sim* s) {
+    sort_particles(s);
+    {
+        Timed t(s, K_P2G);
+        hipLaunchKernelGGL(p2g_kernel, dim3(blocks_for(n_u(s) + n_v(s))), dim3(kBlock), 0, s->stream, s->px, s->py,
+                           s->pvx, s->pvy, s->starts, s->counts, s->sorted, s->u, s->v, s->mu, s->mv, s->nx, s->ny,
+                           s->dx);
+    }
+    extrapolate_uv(s, s->u, s->v);
+}
+
+// applyProjection (src/FluidSim2D.cpp:252-467).
+fs2d_status stage_projection(fs2d_sim* s) {
+    const long nc = n_c(s);
+    PcgState init{};
+    init.tol = s->tol;
+    init.max_iters = s->max_iters;
+    HIPCHK(s, hipMemcpyAsync(s->st, &init, sizeof init, hipMemcpyHostToDevice, s->stream));
+    {
+        Timed t(s, K_ASSEMBLE);
+        hipLaunchKernelGGL(assemble_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->cell, s->phi, s->u,
+                           s->v, s->Adiag, s->Ax, s->Ay, s->r, s->p, s->s[0], s->nx, s->ny,
+                           s->dt / (s->rho * s->dx * s->dx), 1.0 / s->dx, s->st, s->partials, s->counters + 0);
+    }
+    {
+        Timed t(s, K_FACTOR);
+        FactorOp op{s->cell, s->Adiag, s->Ax, s->Ay, s->P, s->ex, s->ey, s->tau, s->sigma, s->nx};
+        launch_wavefront(s, op);
+    }
+    FwdOp fwd{s->r, s->P, s->ex, s->ey, s->q, s->st, s->nx};
+    BwdOp bwd{s->q, s->P, s->ex, s->ey, s->r, s->z, s->st, s->partials, s->nx, s->ny, 1};
+    {
+        Timed t(s, K_PCG_FWD);
+        launch_wavefront(s, fwd);
+    }
+    {
+        Timed t(s, K_PCG_BWD);
+        launch_wavefront(s, bwd);
+    }
+    bwd.init_mode = 0;
+    int h = 0;
+    int batch = std::max(2, std::min(s->last_iters_hint, s->max_iters));
+    while (h < s->max_iters) {
+        const int nb = std::min(batch, s->max_iters - h);
+        for (int k = 0; k < nb; k++, h++) {
+            float* s_old = s->s[h & 1];
+            float* s_new = s->s[(h + 1) & 1];
+            {
+                Timed t(s, K_PCG_SPMV);
+                hipLaunchKernelGGL(pcg_spmv_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->cell,
+                                   s->Adiag, s->Ax, s->Ay, s->z, s_old, s_new, s->w, s->nx, s->ny, s->st,
+                                   s->partials, s->counters + 1);
+            }
+            {
+                Timed t(s, K_PCG_AXPY);
+                hipLaunchKernelGGL(pcg_axpy_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->p, s->r,
+                                   s_new, s->w, nc, s->st, s->partials, s->counters + 2);
+            }
+            {
+                Timed t(s, K_PCG_FWD);
+                launch_wavefront(s, fwd);
+            }
+            {
+                Timed t(s, K_PCG_BWD);
+                launch_wavefront(s, bwd);
+            }
+        }
+        HIPCHK(s, hipMemcpyAsync(s->h_st, s->st, sizeof(PcgState), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        if (s->h_st->done) break;
+        batch = 4;
+    }
+    TRY(d2h(s, s->h_st, s->st, sizeof(PcgState)));
+    s->last_iters_hint = std::max(2, s->h_st->iter + 1);
+    return s->h_st->hit_max ? FS2D_WARN_MAXITER : FS2D_OK;
+}
+
+// updateVelocity (src/FluidSim2D.cpp:469-550).
+void stage_update_velocity(fs2d_sim* s) {
+    {
+        Timed t(s, K_UPDATE_VEL);
+        hipLaunchKernelGGL(update_velocity_kernel, dim3(blocks_for(n_u(s) + n_v(s))), dim3(kBlock), 0, s->stream,
+                           s->cell, s->phi, s->p, s->u, s->v, s->nu, s->nv, s->mu, s->mv, s->nx, s->ny,
+                           s->dt / (s->rho * s->dx));
+    }
+    extrapolate_uv(s, s->nu, s->nv);
+    if (s->mode == FS2D_SEMILAGRANGIAN) {
+        Timed t(s, K_MISC);
+        dcopy(s, s->u, s->nu, (size_t)n_u(s));
+        dcopy(s, s->v, s->nv, (size_t)n_v(s));
+    }
+}
+
+// updateParticleVelocities (src/FluidSim2D.cpp:552-568).
+void stage_g2p(fs2d_sim* s) {
+    if (s->np > 0) {
+        Timed t(s, K_G2P);
+        hipLaunchKernelGGL(g2p_kernel, dim3(blocks_for(s->np)), dim3(kBlock), 0, s->stream, s->px, s->py, s->pvx,
+                           s->pvy, s->np, s->nu, s->nv, s->u, s->v, s->nx, s->ny, s->dx, (float)s->alpha);
+    }
+    Timed t(s, K_MISC);
+    dcopy(s, s->u, s->nu, (size_t)n_u(s));
+    dcopy(s, s->v, s->nv, (size_t)n_v(s));
+}
+
+// applyAdvection (src/FluidSim2D.cpp:570-604).
+void stage_particle_advect(fs2d_sim* s) {
+    if (s->np == 0) return;
+    Timed t(s, K_PARTICLE_ADVECT);
+    hipLaunchKernelGGL(particle_advect_kernel, dim3(blocks_for(s->np)), dim3(kBlock), 0, s->stream, s->px, s->py,
+                       s->pvx, s->pvy, s->np, mac_view(s), s->dt, clamp_box(s->nx, s->ny, s->dx), s->dg, s->partials,
+                       s->counters + 4);
+    s->sorted_valid = false;
+}
+
+// Status after a sequence of stages: one host round trip.
+fs2d_status finish(fs2d_sim* s, fs2d_status st) {
+    HIPCHK(s, hipMemcpyAsync(s->h_dg, s->dg, sizeof(Diag), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipMemcpyAsync(s->h_err, s->werr, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipGetLastError());
+    if (s->timing) resolve_timing(s);
+    if (*s->h_err) return fail(s, FS2D_ERR_HANG, "a wavefront hand-off wait expired");
+    if (s->h_dg->nan_flag) return fail(s, FS2D_ERR_NAN, "advected particle position is NaN");
+    if (st == FS2D_WARN_MAXITER) {
+        fail(s, st, "PCG: maximum iteration limit (%d) exceeded", s->max_iters);
+        return st;
+    }
+    if (s->np > 0 && s->h_dg->cfl_max > 5.0) {
+        fail(s, FS2D_WARN_CFL, "particle velocity violates CFL condition (C=%f)", s->h_dg->cfl_max);
+        return FS2D_WARN_CFL;
+    }
+    return st;
+}
+
+fs2d_status run_stage_impl(fs2d_sim* s, int stage, bool fuse_gravity, fs2d_status* worst) {
+    fs2d_status st = FS2D_OK;
+    switch (stage) {
+        case FS2D_STAGE_INIT: break;
+        case FS2D_STAGE_CREATE_WATER_LEVEL_SET: stage_level_set(s); break;
+        case FS2D_STAGE_TRANSFER_VELOCITY_TO_GRID: stage_p2g(s); break;
+        case FS2D_STAGE_SEMI_LAGRANGIAN_ADVECTION: stage_sl(s, fuse_gravity); break;
+        case FS2D_STAGE_APPLY_GRAVITY: stage_gravity(s); break;
+        case FS2D_STAGE_CREATE_SOLID_LEVEL_SET: break;  // no-op (src/FluidSim2D.cpp:734-736)
+        case FS2D_STAGE_APPLY_PROJECTION: st = stage_projection(s); break;
+        case FS2D_STAGE_UPDATE_VELOCITY: stage_update_velocity(s); break;
+        case FS2D_STAGE_UPDATE_PARTICLE_VELOCITIES: stage_g2p(s); break;
+        case FS2D_STAGE_APPLY_ADVECTION: stage_particle_advect(s); break;
+        default: return fail(s, FS2D_ERR_ARG, "unknown stage %d", stage);
+    }
+    if (st < 0) return st;
+    if (st > *worst) *worst = st;
+    return FS2D_OK;
+}
+
+bool valid_dtype(int field, int dtype) {
+    if (field == FS2D_FIELD_CELL) return dtype == FS2D_U8;
+    return dtype == FS2D_F32 || dtype == FS2D_F64;
+}
+
+size_t dtype_size(int dtype) { return dtype == FS2D_F64 ? 8 : dtype == FS2D_F32 ? 4 : 1; }
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+fs2d_status fs2d_config_init(fs2d_config* c) {
+    if (!c) return FS2D_ERR_ARG;
+    memset(c, 0, sizeof *c);
+    c->particles_per_cell_sqrt = 2;
+    c->dt = 0.0001;             // FluidSim2D.h:89
+    c->dx = 0.001;              // FluidSim2D.h:90
+    c->rho = 997.0;             // FluidSim2D.h:82
+    c->gravity_x = 0.0;         // FluidSim2D.h:80
+    c->gravity_y = -9.81;
+    c->mode = FS2D_PICFLIP;
+    c->pic_flip_alpha = 1.0;
+    c->tau = 0.999;             // FluidSim2D.cpp:369
+    c->sigma = 0.25;            // FluidSim2D.cpp:370
+    c->pcg_tol = 1e-12;         // FluidSim2D.cpp:453
+    c->pcg_max_iters = 200;     // FluidSim2D.cpp:429
+    c->seed = 1;                // glibc's default rand() seed
+    c->flags = 0;
+    return FS2D_OK;
+}
+
+static fs2d_status validate(const fs2d_config* c, char* why, size_t n) {
+    if (c->size_x < 3 || c->size_y < 3 || c->size_x > 16384 || c->size_y > 16384) {
+        snprintf(why, n, "size %dx%d outside [3, 16384]", c->size_x, c->size_y);
+        return FS2D_ERR_CONFIG;
+    }
+    if (!(c->dx > 0) || !(c->dt >= 0) || !(c->rho > 0) || !std::isfinite(c->dt)) {
+        snprintf(why, n, "dx, rho must be > 0 and dt >= 0");
+        return FS2D_ERR_CONFIG;
+    }
+    if (c->particles_per_cell_sqrt < 0 || c->particles_per_cell_sqrt > 16) {
+        snprintf(why, n, "particles_per_cell_sqrt %d outside [0, 16]", c->particles_per_cell_sqrt);
+        return FS2D_ERR_CONFIG;
+    }
+    if (c->mode != FS2D_SEMILAGRANGIAN && c->mode != FS2D_PICFLIP) {
+        snprintf(why, n, "mode %d", c->mode);
+        return FS2D_ERR_CONFIG;
+    }
+    if (c->pcg_max_iters < 0 || !(c->tau >= 0) || !(c->sigma >= 0) || !(c->pcg_tol >= 0)) {
+        snprintf(why, n, "solver parameters");
+        return FS2D_ERR_CONFIG;
+    }
+    if (!c->initial_values) {
+        snprintf(why, n, "initial_values is NULL");
+        return FS2D_ERR_CONFIG;
+    }
+    const int nx = c->size_x, ny = c->size_y;
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const uint8_t f = c->initial_values[(size_t)j * nx + i];
+            if (f > FS2D_SOLID) {
+                snprintf(why, n, "cell (%d,%d) has flag %u", i, j, f);
+                return FS2D_ERR_CONFIG;
+            }
+            // The reference reads cell(i-1,j) etc. of FLUID cells without bounds
+            // checks (FluidSim2D.cpp:267-299); a non-solid border is out of bounds there.
+            if ((i == 0 || j == 0 || i == nx - 1 || j == ny - 1) && f != FS2D_SOLID) {
+                snprintf(why, n, "border cell (%d,%d) is not SOLID", i, j);
+                return FS2D_ERR_CONFIG;
+            }
+        }
+    return FS2D_OK;
+}
+
+static thread_local std::string g_create_error;
+
+fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
+    if (!cfg || !out) return FS2D_ERR_ARG;
+    *out = nullptr;
+    char why[256] = {0};
+    fs2d_status vs = validate(cfg, why, sizeof why);
+    if (vs != FS2D_OK) {
+        g_create_error = why;
+        return vs;
+    }
+    fs2d_sim* s = new fs2d_sim();
+    s->nx = cfg->size_x;
+    s->ny = cfg->size_y;
+    s->k = cfg->particles_per_cell_sqrt;
+    s->dt = cfg->dt;
+    s->dx = cfg->dx;
+    s->dr = 0.9 * cfg->dx;  // FluidSim2D.cpp:32
+    s->rho = cfg->rho;
+    s->gx = cfg->gravity_x;
+    s->gy = cfg->gravity_y;
+    s->mode = cfg->mode;
+    s->alpha = cfg->pic_flip_alpha;
+    s->tau = cfg->tau;
+    s->sigma = cfg->sigma;
+    s->tol = cfg->pcg_tol;
+    s->max_iters = cfg->pcg_max_iters;
+    s->flags = cfg->flags;
+
+    auto bail = [&](fs2d_status st) {
+        g_create_error = s->err;
+        fs2d_destroy(s);
+        return st;
+    };
+    hipError_t e = hipGetDevice(&s->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        s->err = std::string("HIP device unavailable: ") + hipGetErrorString(e);
+        return bail(FS2D_ERR_DEVICE);
+    }
+    const size_t nU = n_u(s), nV = n_v(s), nC = n_c(s);
+    fs2d_status st = FS2D_OK;
+#define A(p, n) \
+    if ((st = dalloc(s, &s->p, (n))) != FS2D_OK) return bail(st)
+    A(u, nU); A(v, nV); A(u2, nU); A(v2, nV); A(nu, nU); A(nv, nV);
+    A(dens, nC); A(dens2, nC); A(cell, nC); A(phi, nC); A(phi2, nC); A(t0, nC); A(t1, nC);
+    A(p, nC); A(r, nC); A(z, nC); A(q, nC); A(w, nC); A(s[0], nC); A(s[1], nC);
+    A(Adiag, nC); A(Ax, nC); A(Ay, nC); A(P, nC); A(ex, nC); A(ey, nC);
+    A(mu, nU); A(mv, nV); A(fu0, nU); A(fu1, nU); A(fv0, nV); A(fv1, nV); A(fcount, 2);
+    A(counts, nC); A(starts, nC); A(cursor, nC); A(tile_sums, (nC + 1023) / 1024 + 1);
+    A(st, 1); A(dg, 1); A(partials, 3 * kMaxBlocks); A(counters, 8);
+    A(wctl, 4); A(werr, 1);
+    const int nstrips = (s->ny + 63) / 64;
+    A(handoff, (size_t)nstrips * s->nx * 4);
+#undef A
+    e = hipHostMalloc((void**)&s->h_st, sizeof(PcgState), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_dg, sizeof(Diag), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_err, sizeof(int), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        s->err = std::string("hipHostMalloc: ") + hipGetErrorString(e);
+        return bail(FS2D_ERR_OOM);
+    }
+    memset(s->h_st, 0, sizeof(PcgState));
+    memset(s->h_dg, 0, sizeof(Diag));
+    *s->h_err = 0;
+    const unsigned ctl0[4] = {0u, 0u, 1u, 0u};
+    if (hipMemcpyAsync(s->wctl, ctl0, sizeof ctl0, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+        hipMemcpyAsync(s->cell, cfg->initial_values, nC, hipMemcpyHostToDevice, s->stream) != hipSuccess) {
+        s->err = "initial upload failed";
+        return bail(FS2D_ERR_DEVICE);
+    }
+
+    // Particle seeding of FluidSim2D::create (src/FluidSim2D.cpp:49-69): k^2
+    // jittered samples per FLUID cell in row-major cell order, x jitter drawn
+    // before y jitter, with glibc's rand() stream (randf, :19-21).  random_r
+    // with a private 128-byte state is that stream without touching the
+    // process-wide generator.
+    std::vector<float> hx, hy;
+    size_t fluid = 0;
+    {
+        struct random_data rd;
+        char rstate[128];
+        memset(&rd, 0, sizeof rd);
+        memset(rstate, 0, sizeof rstate);
+        initstate_r(cfg->seed, rstate, sizeof rstate, &rd);
+        auto randf = [&](float mx) {
+            int32_t val;
+            random_r(&rd, &val);
+            return ((double)val / (double)RAND_MAX) * mx;
+        };
+        const int k = s->k, ppc = k * k;
+        const float dist = k > 0 ? 1.0f / k : 0.0f;
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++)
+                if (cfg->initial_values[(size_t)j * s->nx + i] == FS2D_FLUID) {
+                    fluid++;
+                    for (int e2 = 0; e2 < ppc; e2++) {
+                        const int x = e2 % k, y = e2 / k;
+                        const double jx = randf(dist);
+                        const double jy = randf(dist);
+                        const float fx = dist * x, fy = dist * y;
+                        hx.push_back((float)(((double)i + fx + jx) * s->dx));
+                        hy.push_back((float)(((double)j + fy + jy) * s->dx));
+                    }
+                }
+    }
+    s->orig_volume = fluid * s->dx * s->dx;
+    s->np = (long)hx.size();
+    if ((st = ensure_particle_capacity(s, std::max(s->np, 1L))) != FS2D_OK) return bail(st);
+    if (s->np > 0) {
+        if ((st = h2d(s, s->px, hx.data(), s->np * sizeof(float))) != FS2D_OK ||
+            (st = h2d(s, s->py, hy.data(), s->np * sizeof(float))) != FS2D_OK)
+            return bail(st);
+    }
+    if (s->flags & FS2D_FLAG_STATIC_CELLS) {
+        // Static cells: ghost-fluid faces (if any EMPTY cell exists) sit half-way
+        // between the centres.
+        std::vector<double> ph(nC);
+        for (size_t c = 0; c < nC; c++) ph[c] = cfg->initial_values[c] == FS2D_EMPTY ? 0.5 * s->dx : -0.5 * s->dx;
+        if ((st = h2d(s, s->phi, ph.data(), nC * sizeof(double))) != FS2D_OK) return bail(st);
+    }
+    if (hipStreamSynchronize(s->stream) != hipSuccess) {
+        s->err = "device synchronisation failed";
+        return bail(FS2D_ERR_DEVICE);
+    }
+    *out = s;
+    return FS2D_OK;
+}
+
+void fs2d_destroy(fs2d_sim* s) {
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (void* p : s->allocs) (void)hipFree(p);
+    for (auto& t : s->pending) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    for (auto e : s->event_pool) (void)hipEventDestroy(e);
+    if (s->h_st) (void)hipHostFree(s->h_st);
+    if (s->h_dg) (void)hipHostFree(s->h_dg);
+    if (s->h_err) (void)hipHostFree(s->h_err);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+fs2d_status fs2d_step(fs2d_sim* s) {
+    if (!s) return FS2D_ERR_ARG;
+    fs2d_status worst = FS2D_OK, st;
+    if (s->mode == FS2D_SEMILAGRANGIAN) {
+        // createWaterLevelSet, applySemiLagrangianAdvection + applyGravity (fused),
+        // createSolidLevelSet, applyProjection, updateVelocity, applyAdvection
+        const int stages[] = {FS2D_STAGE_CREATE_WATER_LEVEL_SET, FS2D_STAGE_SEMI_LAGRANGIAN_ADVECTION,
+                              FS2D_STAGE_CREATE_SOLID_LEVEL_SET, FS2D_STAGE_APPLY_PROJECTION,
+                              FS2D_STAGE_UPDATE_VELOCITY, FS2D_STAGE_APPLY_ADVECTION};
+        for (int stg : stages)
+            if ((st = run_stage_impl(s, stg, true, &worst)) != FS2D_OK) return st;
+    } else {
+        const int stages[] = {FS2D_STAGE_CREATE_WATER_LEVEL_SET, FS2D_STAGE_TRANSFER_VELOCITY_TO_GRID,
+                              FS2D_STAGE_APPLY_GRAVITY, FS2D_STAGE_CREATE_SOLID_LEVEL_SET,
+                              FS2D_STAGE_APPLY_PROJECTION, FS2D_STAGE_UPDATE_VELOCITY,
+                              FS2D_STAGE_UPDATE_PARTICLE_VELOCITIES, FS2D_STAGE_APPLY_ADVECTION};
+        for (int stg : stages)
+            if ((st = run_stage_impl(s, stg, false, &worst)) != FS2D_OK) return st;
+    }
+    s->current_time += s->dt;
+    s->steps++;
+    return finish(s, worst);
+}
+
+fs2d_status fs2d_run_stage(fs2d_sim* s, int stage) {
+    if (!s) return FS2D_ERR_ARG;
+    fs2d_status worst = FS2D_OK;
+    fs2d_status st = run_stage_impl(s, stage, false, &worst);
+    if (st != FS2D_OK) return st;
+    return finish(s, worst);
+}
+
+fs2d_status fs2d_synchronize(fs2d_sim* s) {
+    if (!s) return FS2D_ERR_ARG;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    if (s->timing) resolve_timing(s);
+    return FS2D_OK;
+}
+
+size_t fs2d_field_count(const fs2d_sim* s, int field) {
+    if (!s) return 0;
+    switch (field) {
+        case FS2D_FIELD_U: case FS2D_FIELD_NEWU: return (size_t)n_u(s);
+        case FS2D_FIELD_V: case FS2D_FIELD_NEWV: return (size_t)n_v(s);
+        case FS2D_FIELD_P: case FS2D_FIELD_PHI: case FS2D_FIELD_DENSITY: case FS2D_FIELD_CELL: return (size_t)n_c(s);
+        case FS2D_FIELD_PARTICLE_POS: case FS2D_FIELD_PARTICLE_VEL: return (size_t)s->np * 2;
+        default: return 0;
+    }
+}
+
+static float* fp32_field(fs2d_sim* s, int field) {
+    switch (field) {
+        case FS2D_FIELD_U: return s->u;
+        case FS2D_FIELD_V: return s->v;
+        case FS2D_FIELD_NEWU: return s->nu;
+        case FS2D_FIELD_NEWV: return s->nv;
+        case FS2D_FIELD_P: return s->p;
+        case FS2D_FIELD_DENSITY: return s->dens;
+        default: return nullptr;
+    }
+}
+
+fs2d_status fs2d_get_field(fs2d_sim* s, int field, int dtype, void* host, size_t count) {
+    if (!s || !host) return FS2D_ERR_ARG;
+    const size_t n = fs2d_field_count(s, field);
+    if (n == 0 && field != FS2D_FIELD_PARTICLE_POS && field != FS2D_FIELD_PARTICLE_VEL)
+        return fail(s, FS2D_ERR_ARG, "unknown field %d", field);
+    if (count != n) return fail(s, FS2D_ERR_ARG, "field %d has %zu elements, buffer has %zu", field, n, count);
+    if (!valid_dtype(field, dtype)) return fail(s, FS2D_ERR_ARG, "dtype %d invalid for field %d", dtype, field);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    if (n == 0) return FS2D_OK;
+    if (field == FS2D_FIELD_CELL) {
+        TRY(d2h(s, host, s->cell, n));
+        return FS2D_OK;
+    }
+    if (field == FS2D_FIELD_PHI) {
+        std::vector<double> tmp(n);
+        TRY(d2h(s, tmp.data(), s->phi, n * sizeof(double)));
+        if (dtype == FS2D_F64) memcpy(host, tmp.data(), n * 8);
+        else for (size_t i = 0; i < n; i++) ((float*)host)[i] = (float)tmp[i];
+        return FS2D_OK;
+    }
+    std::vector<float> tmp(n);
+    if (field == FS2D_FIELD_PARTICLE_POS || field == FS2D_FIELD_PARTICLE_VEL) {
+        const float* a = field == FS2D_FIELD_PARTICLE_POS ? s->px : s->pvx;
+        const float* b = field == FS2D_FIELD_PARTICLE_POS ? s->py : s->pvy;
+        std::vector<float> xs(s->np), ys(s->np);
+        TRY(d2h(s, xs.data(), a, s->np * sizeof(float)));
+        TRY(d2h(s, ys.data(), b, s->np * sizeof(float)));
+        for (long e = 0; e < s->np; e++) {
+            tmp[2 * e] = xs[e];
+            tmp[2 * e + 1] = ys[e];
+        }
+    } else {
+        TRY(d2h(s, tmp.data(), fp32_field(s, field), n * sizeof(float)));
+    }
+    if (dtype == FS2D_F32) memcpy(host, tmp.data(), n * 4);
+    else for (size_t i = 0; i < n; i++) ((double*)host)[i] = tmp[i];
+    return FS2D_OK;
+}
+
+static bool particle_in_box(const fs2d_sim* s, double x, double y) {
+    // Positions inside [dx, (n-1)dx) keep every transfer stencil inside the grid
+    // (the reference's particles never leave it: seeding in interior cells and
+    // clampPos after every move, FluidSim2D.cpp:52-64, 645-651).
+    return std::isfinite(x) && std::isfinite(y) && (float)x >= (float)s->dx && (float)y >= (float)s->dx &&
+           (double)(float)x < (s->nx - 1) * s->dx && (double)(float)y < (s->ny - 1) * s->dx;
+}
+
+fs2d_status fs2d_set_field(fs2d_sim* s, int field, int dtype, const void* host, size_t count) {
+    if (!s || !host) return FS2D_ERR_ARG;
+    const size_t n = fs2d_field_count(s, field);
+    if (n == 0 && field != FS2D_FIELD_PARTICLE_POS && field != FS2D_FIELD_PARTICLE_VEL)
+        return fail(s, FS2D_ERR_ARG, "unknown field %d", field);
+    if (count != n) return fail(s, FS2D_ERR_ARG, "field %d has %zu elements, buffer has %zu", field, n, count);
+    if (!valid_dtype(field, dtype)) return fail(s, FS2D_ERR_ARG, "dtype %d invalid for field %d", dtype, field);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    if (n == 0) return FS2D_OK;
+    if (field == FS2D_FIELD_CELL) {
+        const uint8_t* c = (const uint8_t*)host;
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) {
+                uint8_t f = c[(size_t)j * s->nx + i];
+                if (f > FS2D_SOLID || ((i == 0 || j == 0 || i == s->nx - 1 || j == s->ny - 1) && f != FS2D_SOLID))
+                    return fail(s, FS2D_ERR_ARG, "invalid cell flag %u at (%d,%d)", f, i, j);
+            }
+        TRY(h2d(s, s->cell, host, n));
+        return FS2D_OK;
+    }
+    if (field == FS2D_FIELD_PHI) {
+        std::vector<double> tmp(n);
+        for (size_t i = 0; i < n; i++)
+            tmp[i] = dtype == FS2D_F64 ? ((const double*)host)[i] : (double)((const float*)host)[i];
+        TRY(h2d(s, s->phi, tmp.data(), n * sizeof(double)));
+        return FS2D_OK;
+    }
+    std::vector<float> tmp(n);
+    for (size_t i = 0; i < n; i++)
+        tmp[i] = dtype == FS2D_F32 ? ((const float*)host)[i] : (float)((const double*)host)[i];
+    if (field == FS2D_FIELD_PARTICLE_POS || field == FS2D_FIELD_PARTICLE_VEL) {
+        std::vector<float> xs(s->np), ys(s->np);
+        for (long e = 0; e < s->np; e++) {
+            xs[e] = tmp[2 * e];
+            ys[e] = tmp[2 * e + 1];
+            if (field == FS2D_FIELD_PARTICLE_POS && !particle_in_box(s, xs[e], ys[e]))
+                return fail(s, FS2D_ERR_ARG, "particle %ld at (%g, %g) outside [dx, (n-1)dx)", e, xs[e], ys[e]);
+        }
+        float* a = field == FS2D_FIELD_PARTICLE_POS ? s->px : s->pvx;
+        float* b = field == FS2D_FIELD_PARTICLE_POS ? s->py : s->pvy;
+        TRY(h2d(s, a, xs.data(), s->np * sizeof(float)));
+        TRY(h2d(s, b, ys.data(), s->np * sizeof(float)));
+        if (field == FS2D_FIELD_PARTICLE_POS) s->sorted_valid = false;
+        return FS2D_OK;
+    }
+    TRY(h2d(s, fp32_field(s, field), tmp.data(), n * sizeof(float)));
+    return FS2D_OK;
+}
+
+fs2d_status fs2d_set_particles(fs2d_sim* s, int dtype, const void* pos_xy, size_t n) {
+    if (!s || (n > 0 && !pos_xy)) return FS2D_ERR_ARG;
+    if (dtype != FS2D_F32 && dtype != FS2D_F64) return fail(s, FS2D_ERR_ARG, "dtype %d", dtype);
+    if (n > (size_t)INT32_MAX / 2) return fail(s, FS2D_ERR_ARG, "too many particles");
+    std::vector<float> xs(n), ys(n);
+    for (size_t e = 0; e < n; e++) {
+        double x = dtype == FS2D_F32 ? ((const float*)pos_xy)[2 * e] : ((const double*)pos_xy)[2 * e];
+        double y = dtype == FS2D_F32 ? ((const float*)pos_xy)[2 * e + 1] : ((const double*)pos_xy)[2 * e + 1];
+        if (!particle_in_box(s, x, y))
+            return fail(s, FS2D_ERR_ARG, "particle %zu at (%g, %g) outside [dx, (n-1)dx)", e, x, y);
+        xs[e] = (float)x;
+        ys[e] = (float)y;
+    }
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    TRY(ensure_particle_capacity(s, std::max<long>((long)n, 1)));
+    s->np = (long)n;
+    if (n > 0) {
+        TRY(h2d(s, s->px, xs.data(), n * sizeof(float)));
+        TRY(h2d(s, s->py, ys.data(), n * sizeof(float)));
+        fill(s, s->pvx, 0.0f, n);
+        fill(s, s->pvy, 0.0f, n);
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+    }
+    s->sorted_valid = false;
+    return FS2D_OK;
+}
+
+fs2d_status fs2d_device_field(fs2d_sim* s, int field, void** dptr) {
+    if (!s || !dptr) return FS2D_ERR_ARG;
+    switch (field) {
+        case FS2D_FIELD_CELL: *dptr = s->cell; return FS2D_OK;
+        case FS2D_FIELD_PHI: *dptr = s->phi; return FS2D_OK;
+        case FS2D_FIELD_PARTICLE_POS: *dptr = s->px; return FS2D_OK;
+        case FS2D_FIELD_PARTICLE_VEL: *dptr = s->pvx; return FS2D_OK;
+        default: break;
+    }
+    float* f = fp32_field(s, field);
+    if (!f) return fail(s, FS2D_ERR_ARG, "unknown field %d", field);
+    *dptr = f;
+    return FS2D_OK;
+}
+
+void* fs2d_stream(fs2d_sim* s) { return s ? (void*)s->stream : nullptr; }
+
+fs2d_status fs2d_set_gravity(fs2d_sim* s, double gx, double gy) {
+    if (!s) return FS2D_ERR_ARG;
+    s->gx = gx;
+    s->gy = gy;
+    return FS2D_OK;
+}
+fs2d_status fs2d_set_pic_flip_alpha(fs2d_sim* s, double a) {
+    if (!s) return FS2D_ERR_ARG;
+    s->alpha = a;
+    return FS2D_OK;
+}
+fs2d_status fs2d_set_dt(fs2d_sim* s, double dt) {
+    if (!s || !(dt >= 0) || !std::isfinite(dt)) return FS2D_ERR_ARG;
+    s->dt = dt;
+    return FS2D_OK;
+}
+fs2d_status fs2d_set_solver(fs2d_sim* s, double tau, double sigma, double tol, int max_iters) {
+    if (!s || !(tau >= 0) || !(sigma >= 0) || !(tol >= 0) || max_iters < 0) return FS2D_ERR_ARG;
+    s->tau = tau;
+    s->sigma = sigma;
+    s->tol = tol;
+    s->max_iters = max_iters;
+    return FS2D_OK;
+}
+
+fs2d_status fs2d_get_stats(fs2d_sim* s, fs2d_stats* out) {
+    if (!s || !out) return FS2D_ERR_ARG;
+    HIPCHK(s, hipMemcpyAsync(s->h_dg, s->dg, sizeof(Diag), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipMemcpyAsync(s->h_st, s->st, sizeof(PcgState), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    memset(out, 0, sizeof *out);
+    out->water_volume = s->h_dg->volume;
+    out->total_energy = s->h_dg->grid_energy;
+    out->particle_total_energy = s->h_dg->particle_energy;
+    out->orig_water_volume = s->orig_volume;
+    out->current_time = s->current_time;
+    out->pcg_residual = s->h_st->rhs_norm > 0 ? s->h_st->r_norm / s->h_st->rhs_norm : 0.0;
+    out->cfl_max = s->h_dg->cfl_max;
+    out->pcg_iters = s->h_st->iter;
+    out->pcg_hit_max = s->h_st->hit_max;
+    out->n_particles = s->np;
+    out->steps = s->steps;
+    return FS2D_OK;
+}
+
+fs2d_status fs2d_avg_pressure(fs2d_sim* s, int fluid_only, double* out) {
+    if (!s || !out) return FS2D_ERR_ARG;
+    const size_t n = (size_t)n_c(s);
+    std::vector<float> p(n);
+    std::vector<uint8_t> c(n);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    TRY(d2h(s, p.data(), s->p, n * 4));
+    TRY(d2h(s, c.data(), s->cell, n));
+    double acc = 0;
+    size_t cnt = 0;
+    for (size_t i = 0; i < n; i++)
+        if (!fluid_only || c[i] == FS2D_FLUID) {
+            acc += p[i];
+            cnt++;
+        }
+    // avgPressure divides by all cells (:612), avgPressureInFluid by the fluid count (:625)
+    *out = acc / (double)(fluid_only ? cnt : n);
+    return FS2D_OK;
+}
+
+fs2d_status fs2d_max_velocity(fs2d_sim* s, double* out) {
+    if (!s || !out) return FS2D_ERR_ARG;
+    // FluidSim2D::maxVelocity (:629-638): |velInterp| at the corners of FLUID cells.
+    const size_t nU = (size_t)n_u(s), nV = (size_t)n_v(s), nC = (size_t)n_c(s);
+    std::vector<float> u(nU), v(nV);
+    std::vector<uint8_t> c(nC);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    TRY(d2h(s, u.data(), s->u, nU * 4));
+    TRY(d2h(s, v.data(), s->v, nV * 4));
+    TRY(d2h(s, c.data(), s->cell, nC));
+    auto cr = [](const std::vector<float>& a, int w, int h, double gx, double gy) -> double {
+        int x = (int)gx, y = (int)gy;
+        if (x < 0 || x >= w || y < 0 || y >= h) return 0;
+        double fx = gx - x, fy = gy - y;
+        double uu[4] = {-0.5 * fx * fx * fx + fx * fx - 0.5 * fx, 1.5 * fx * fx * fx - 2.5 * fx * fx + 1,
+                        -1.5 * fx * fx * fx + 2 * fx * fx + 0.5 * fx, 0.5 * fx * fx * fx - 0.5 * fx * fx};
+        double vv[4] = {-0.5 * fy * fy * fy + fy * fy - 0.5 * fy, 1.5 * fy * fy * fy - 2.5 * fy * fy + 1,
+                        -1.5 * fy * fy * fy + 2 * fy * fy + 0.5 * fy, 0.5 * fy * fy * fy - 0.5 * fy * fy};
+        double acc = 0;
+        for (int r = 0; r < 4; r++) {
+            int yy = std::min(std::max(y - 1 + r, 0), h - 1);
+            double row = 0;
+            for (int q = 0; q < 4; q++) row += uu[q] * a[(size_t)yy * w + std::min(std::max(x - 1 + q, 0), w - 1)];
+            acc += row * vv[r];
+        }
+        return acc;
+    };
+    double m = 0;
+    for (int j = 0; j < s->ny; j++)
+        for (int i = 0; i < s->nx; i++) {
+            if (c[(size_t)j * s->nx + i] != FS2D_FLUID) continue;
+            double X = i, Y = j;
+            double ux = std::min(std::max(X, 1e-6), s->nx - 1 - 1e-6);
+            double uy = std::min(std::max(Y - 0.5, 1e-6), s->ny - 1 - 1e-6);
+            double vx = std::min(std::max(X - 0.5, 1e-6), s->nx - 1 - 1e-6);
+            double vy = std::min(std::max(Y, 1e-6), s->ny - 1 - 1e-6);
+            double a = cr(u, s->nx + 1, s->ny, ux, uy), b = cr(v, s->nx, s->ny + 1, vx, vy);
+            m = std::max(m, std::sqrt(a * a + b * b));
+        }
+    *out = m;
+    return FS2D_OK;
+}
+
+fs2d_status fs2d_set_timing(fs2d_sim* s, int enable) {
+    if (!s) return FS2D_ERR_ARG;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    resolve_timing(s);
+    s->timing = enable != 0;
+    return FS2D_OK;
+}
+int fs2d_kernel_count(void) { return K_COUNT; }
+const char* fs2d_kernel_name(int k) { return (k >= 0 && k < K_COUNT) ? kKernelNames[k] : nullptr; }
+fs2d_status fs2d_get_kernel_time(fs2d_sim* s, int k, double* total_ms, long long* launches) {
+    if (!s || k < 0 || k >= K_COUNT || !total_ms || !launches) return FS2D_ERR_ARG;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    resolve_timing(s);
+    *total_ms = s->ktime[k];
+    *launches = s->kcount[k];
+    return FS2D_OK;
+}
+fs2d_status fs2d_reset_kernel_times(fs2d_sim* s) {
+    if (!s) return FS2D_ERR_ARG;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    resolve_timing(s);
+    for (int k = 0; k < K_COUNT; k++) {
+        s->ktime[k] = 0;
+        s->kcount[k] = 0;
+    }
+    return FS2D_OK;
+}
+
+const char* fs2d_last_error(const fs2d_sim* s) { return s ? s->err.c_str() : g_create_error.c_str(); }
+
+const char* fs2d_status_string(int st) {
+    switch (st) {
+        case FS2D_OK: return "FS2D_OK";
+        case FS2D_WARN_MAXITER: return "FS2D_WARN_MAXITER";
+        case FS2D_WARN_CFL: return "FS2D_WARN_CFL";
+        case FS2D_ERR_NAN: return "FS2D_ERR_NAN";
+        case FS2D_ERR_OOM: return "FS2D_ERR_OOM";
+        case FS2D_ERR_CONFIG: return "FS2D_ERR_CONFIG";
+        case FS2D_ERR_ARG: return "FS2D_ERR_ARG";
+        case FS2D_ERR_DEVICE: return "FS2D_ERR_DEVICE";
+        case FS2D_ERR_HANG: return "FS2D_ERR_HANG";
+        default: return "unknown";
+    }
+}
+
+}  // extern "C"

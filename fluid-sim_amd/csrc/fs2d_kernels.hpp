@@ -1,0 +1,998 @@
+// fs2d_kernels.hpp -- gfx950 kernels of the 2D step.
+//
+// Layout in HBM (all row-major, element (i, j) at [j * width + i]):
+//   u, u2, newu   FP32 (nx+1) x ny        v, v2, newv   FP32 nx x (ny+1)
+//   p, r, z, q, w, s[2], Adiag, Ax, Ay, P, ex, ey       FP32 nx x ny
+//   cell          uint8 nx x ny           phi[2]        FP64 nx x ny
+//   tidx[2]       int32 nx x ny (closest particle of the level-set sweeps)
+//   px, py, pvx, pvy  FP32 SoA, particle order as created
+// Reductions are deterministic: each block writes a partial, the last block to
+// finish combines the partials in index order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "fs2d_device.hpp"
+#include "fs2d_wavefront.hpp"
+
+namespace fs2d {
+
+constexpr int kBlock = 256;
+
+struct PcgState {
+    double sigma, alpha, beta, rhs_norm, r_norm, tol;
+    int iter, done, hit_max, max_iters;
+};
+
+struct Diag {  // createWaterLevelSet diagnostics (src/FluidSim2D.cpp:709-731)
+    double volume, grid_energy, particle_energy, cfl_max;
+    int nan_flag, pad;
+};
+
+// ---------------------------------------------------------------------------
+// Block reductions and the last-block protocol.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T x = __shfl_xor(v, o, 64);
+        v = x > v ? x : v;
+    }
+    return v;
+}
+// Sum over a 256-thread block in a fixed order; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double sh[kBlock / 64];
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kBlock / 64; k++) t += sh[k];
+    return t;
+}
+__device__ __forceinline__ double block_max(double v) {
+    __shared__ double sh[kBlock / 64];
+    v = wave_max(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kBlock / 64; k++) t = sh[k] > t ? sh[k] : t;
+    return t;
+}
+// Called by all threads after thread 0 wrote this block's partial.
+__device__ __forceinline__ bool last_block(unsigned* counter) {
+    __shared__ bool am_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) am_last = (atomicAdd(counter, 1u) == gridDim.x - 1);
+    __syncthreads();
+    if (am_last) __threadfence();
+    return am_last;
+}
+// Sum of partials[0..n) in a fixed order by the (last) block; result in thread 0.
+__device__ __forceinline__ double sum_partials(const double* partials, int n) {
+    double acc = 0;
+    for (int k = threadIdx.x; k < n; k += kBlock) acc += partials[k];
+    return block_sum(acc);
+}
+__device__ __forceinline__ double max_partials(const double* partials, int n) {
+    double acc = 0;
+    for (int k = threadIdx.x; k < n; k += kBlock) acc = partials[k] > acc ? partials[k] : acc;
+    return block_max(acc);
+}
+
+template <class T>
+__global__ void fill_kernel(T* a, T v, size_t n) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        a[e] = v;
+}
+template <class T>
+__global__ void copy_kernel(T* dst, const T* src, size_t n) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        dst[e] = src[e];
+}
+
+// ---------------------------------------------------------------------------
+// Semi-Lagrangian advection (src/FluidSim2D.cpp:206-235), out of place, with
+// applyGravity (:237-250) optionally fused into the store, and the passive
+// density (no reference counterpart) traced back from cell centres.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void sl_advect_kernel(MacView m, float* __restrict__ u_out,
+                                                           float* __restrict__ v_out,
+                                                           const float* __restrict__ d_in,
+                                                           float* __restrict__ d_out, double dt,
+                                                           ClampBox cb, float gu, float gv) {
+    const int nx = m.nx, ny = m.ny;
+    const long nu = (long)(nx + 1) * ny, nv = (long)nx * (ny + 1);
+    const long nd = d_in ? (long)nx * ny : 0;
+    const long n = nu + nv + nd;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        double ox, oy;
+        if (e < nu) {
+            const int i = (int)(e % (nx + 1)), j = (int)(e / (nx + 1));
+            backtrace(m, dt, cb, (double)i * m.dx, ((double)j + 0.5) * m.dx, ox, oy);
+            u_out[e] = vel_interp_u(m, ox, oy) + gu;
+        } else if (e < nu + nv) {
+            const long f = e - nu;
+            const int i = (int)(f % nx), j = (int)(f / nx);
+            backtrace(m, dt, cb, ((double)i + 0.5) * m.dx, (double)j * m.dx, ox, oy);
+            v_out[f] = vel_interp_v(m, ox, oy) + gv;
+        } else {
+            const long f = e - nu - nv;
+            const int i = (int)(f % nx), j = (int)(f / nx);
+            backtrace(m, dt, cb, ((double)i + 0.5) * m.dx, ((double)j + 0.5) * m.dx, ox, oy);
+            double gx = clampd(ox / m.dx - 0.5, 1e-6, (double)(nx - 1) - 1e-6);
+            double gy = clampd(oy / m.dx - 0.5, 1e-6, (double)(ny - 1) - 1e-6);
+            d_out[f] = catmull_rom(d_in, nx, ny, gx, gy);
+        }
+    }
+}
+
+// FluidSim2D::applyGravity (src/FluidSim2D.cpp:237-250): every face.
+__global__ void gravity_kernel(float* __restrict__ u, float* __restrict__ v, long nu, long nv, float gu,
+                               float gv) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nu + nv; e += (long)gridDim.x * blockDim.x) {
+        if (e < nu) u[e] += gu;
+        else v[e - nu] += gv;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Projection (src/FluidSim2D.cpp:252-467).
+// ---------------------------------------------------------------------------
+// Ghost-fluid matrix (:262-303) and rhs (:338-362); p = 0, s = 0; |rhs|_inf.
+__global__ __launch_bounds__(kBlock) void assemble_kernel(
+    const uint8_t* __restrict__ cell, const double* __restrict__ phi, const float* __restrict__ u,
+    const float* __restrict__ v, float* __restrict__ Adiag, float* __restrict__ Ax, float* __restrict__ Ay,
+    float* __restrict__ r, float* __restrict__ p, float* __restrict__ s0, int nx, int ny, double scaleA,
+    double scale, PcgState* st, double* partials, unsigned* counter) {
+    const long n = (long)nx * ny;
+    double m = 0;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(e % nx), j = (int)(e / nx);
+        double ad = 0, ax = 0, ay = 0, rhs = 0;
+        if (cell[e] == FLUID) {
+            const uint8_t cl = cell[e - 1], cr = cell[e + 1], cb = cell[e - nx], ct = cell[e + nx];
+            const double ph = phi[e];
+            if (cl == FLUID) ad += scaleA;
+            else if (cl == EMPTY) ad -= scaleA * fmax(phi[e - 1] / ph, -1e3);
+            if (cr == FLUID) { ad += scaleA; ax = -scaleA; }
+            else if (cr == EMPTY) ad += scaleA * (1 - fmax(phi[e + 1] / ph, -1e3));
+            if (cb == FLUID) ad += scaleA;
+            else if (cb == EMPTY) ad -= scaleA * fmax(phi[e - nx] / ph, -1e3);
+            if (ct == FLUID) { ad += scaleA; ay = -scaleA; }
+            else if (ct == EMPTY) ad += scaleA * (1 - fmax(phi[e + nx] / ph, -1e3));
+            const long iu = (long)j * (nx + 1) + i;
+            const double u0 = u[iu], u1 = u[iu + 1], v0 = v[e], v1 = v[e + nx];
+            rhs = -scale * (u1 - u0 + v1 - v0);
+            if (cl == SOLID) rhs -= scale * u0;
+            if (cr == SOLID) rhs += scale * u1;
+            if (cb == SOLID) rhs -= scale * v0;
+            if (ct == SOLID) rhs += scale * v1;
+        }
+        Adiag[e] = (float)ad;
+        Ax[e] = (float)ax;
+        Ay[e] = (float)ay;
+        r[e] = (float)rhs;
+        p[e] = 0.0f;
+        s0[e] = 0.0f;
+        const double a = fabs((double)(float)rhs);
+        m = a > m ? a : m;
+    }
+    double bm = block_max(m);
+    if (threadIdx.x == 0) partials[blockIdx.x] = bm;
+    if (last_block(counter)) {
+        double rn = max_partials(partials, gridDim.x);
+        if (threadIdx.x == 0) {
+            st->rhs_norm = rn;
+            st->r_norm = rn;
+            st->iter = 0;
+            st->sigma = 0;
+            st->alpha = 0;
+            st->beta = 0;
+            st->hit_max = st->max_iters <= 0;
+            st->done = (rn <= 1e-12) || (st->max_iters <= 0);
+            *counter = 0;
+        }
+    }
+}
+
+// MIC(0) factor (:364-388): precon carried in FP64 along the wavefront;
+// stores P = precon and the scaled couplings ex = Ax*P, ey = Ay*P used by the
+// triangular solves.
+struct FactorOp {
+    static constexpr int NW = 2;
+    const uint8_t* __restrict__ cell;
+    const float* __restrict__ Adiag;
+    const float* __restrict__ Ax;
+    const float* __restrict__ Ay;
+    float* __restrict__ P;
+    float* __restrict__ ex;
+    float* __restrict__ ey;
+    double tau, sigma;
+    int nx;
+    __device__ bool skip() const { return false; }
+    __device__ void boundary(uint32_t* s) const { s[0] = s[1] = 0; }
+    __device__ void compute(int i, int j, const uint32_t* left, const uint32_t* below, uint32_t* out) {
+        const long e = (long)j * nx + i;
+        double Pc = 0;
+        if (i >= 1 && j >= 1 && cell[e] == FLUID) {
+            const double PL = u2d(left), PB = u2d(below);
+            const double axl = Ax[e - 1], ayl = Ay[e - 1], ayb = Ay[e - nx], axb = Ax[e - nx];
+            const double A0 = Adiag[e];
+            const double t1 = axl * PL, t2 = ayb * PB;
+            double ee = A0 - t1 * t1 - t2 * t2 - tau * (axl * ayl * PL * PL + ayb * axb * PB * PB);
+            if (ee < sigma * A0) ee = A0;
+            Pc = 1.0 / sqrt(ee);
+        }
+        P[e] = (float)Pc;
+        ex[e] = (float)((double)Ax[e] * Pc);
+        ey[e] = (float)((double)Ay[e] * Pc);
+        d2u(Pc, out);
+    }
+    __device__ void strip_done(int) {}
+    __device__ void last_done(int) {}
+};
+
+// Forward solve of applyPreconditioner (:400-409): q = Lt^-1 r with
+// Lt = diag(1/P) + lower(A) diag(P).  State carried: (q, ex, ey) of the cell.
+struct FwdOp {
+    static constexpr int NW = 3;
+    const float* __restrict__ r;
+    const float* __restrict__ P;
+    const float* __restrict__ ex;
+    const float* __restrict__ ey;
+    float* __restrict__ q;
+    const PcgState* st;
+    int nx;
+    __device__ bool skip() const { return st->done != 0; }
+    __device__ void boundary(uint32_t* s) const { s[0] = s[1] = s[2] = 0; }
+    __device__ void compute(int i, int j, const uint32_t* left, const uint32_t* below, uint32_t* out) {
+        const long e = (long)j * nx + i;
+        const float Pc = P[e];
+        float qv = 0.0f;
+        if (Pc != 0.0f && i >= 1 && j >= 1)
+            qv = Pc * (r[e] - u2f(left[1]) * u2f(left[0]) - u2f(below[2]) * u2f(below[0]));
+        q[e] = qv;
+        out[0] = f2u(qv);
+        out[1] = f2u(ex[e]);
+        out[2] = f2u(ey[e]);
+    }
+    __device__ void strip_done(int) {}
+    __device__ void last_done(int) {}
+};
+
+// Backward solve (:410-419): z = Lt^-T q, mirrored onto the canonical sweep
+// (a = nx-1-i, b = ny-1-j); accumulates z.r for sigma (:428, :457) and updates
+// beta / sigma / iter in the last workgroup.
+struct BwdOp {
+    static constexpr int NW = 1;
+    const float* __restrict__ q;
+    const float* __restrict__ P;
+    const float* __restrict__ ex;
+    const float* __restrict__ ey;
+    const float* __restrict__ r;
+    float* __restrict__ z;
+    PcgState* st;
+    double* partials;
+    int nx, ny, init_mode;
+    double acc = 0;
+    __device__ bool skip() const { return st->done != 0; }
+    __device__ void boundary(uint32_t* s) const { s[0] = 0; }
+    __device__ void compute(int a, int b, const uint32_t* left, const uint32_t* below, uint32_t* out) {
+        const int i = nx - 1 - a, j = ny - 1 - b;
+        const long e = (long)j * nx + i;
+        const float Pc = P[e];
+        float zv = 0.0f;
+        if (Pc != 0.0f && i <= nx - 2 && j <= ny - 2)
+            zv = Pc * (q[e] - ex[e] * u2f(left[0]) - ey[e] * u2f(below[0]));
+        z[e] = zv;
+        acc += (double)zv * (double)r[e];
+        out[0] = f2u(zv);
+    }
+    __device__ void strip_done(int s) {
+        double t = wave_sum(acc);
+        if (threadIdx.x == 0) partials[s] = t;
+    }
+    __device__ void last_done(int nstrips) {
+        double t = 0;
+        for (int k = threadIdx.x; k < nstrips; k += 64) t += partials[k];
+        t = wave_sum(t);
+        if (threadIdx.x == 0) {
+            if (init_mode) {
+                st->sigma = t;
+                st->beta = 0;
+            } else {
+                st->beta = t / st->sigma;
+                st->sigma = t;
+                st->iter += 1;
+                if (st->iter >= st->max_iters) {
+                    st->done = 1;
+                    st->hit_max = 1;
+                }
+            }
+        }
+    }
+};
+
+// s' = z + beta*s (:459, FMA as Array2D.h:185), w = A s' (:433-444) and the
+// partial of w.s' for alpha (:450).  s is double-buffered because every cell
+// also reads s' of its four neighbours.
+__global__ __launch_bounds__(kBlock) void pcg_spmv_kernel(
+    const uint8_t* __restrict__ cell, const float* __restrict__ Adiag, const float* __restrict__ Ax,
+    const float* __restrict__ Ay, const float* __restrict__ z, const float* __restrict__ s_old,
+    float* __restrict__ s_new, float* __restrict__ w, int nx, int ny, PcgState* st, double* partials,
+    unsigned* counter) {
+    if (st->done) return;
+    const float beta = (float)st->beta;
+    const long n = (long)nx * ny;
+    double acc = 0;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const float sc = fmaf(beta, s_old[e], z[e]);
+        s_new[e] = sc;
+        float wv = 0.0f;
+        if (cell[e] == FLUID) {
+            const float sl = fmaf(beta, s_old[e - 1], z[e - 1]);
+            const float sr = fmaf(beta, s_old[e + 1], z[e + 1]);
+            const float sb = fmaf(beta, s_old[e - nx], z[e - nx]);
+            const float stp = fmaf(beta, s_old[e + nx], z[e + nx]);
+            wv = Adiag[e] * sc + Ax[e - 1] * sl + Ax[e] * sr + Ay[e - nx] * sb + Ay[e] * stp;
+        }
+        w[e] = wv;
+        acc += (double)wv * (double)sc;
+    }
+    double bs = block_sum(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+    if (last_block(counter)) {
+        double t = sum_partials(partials, gridDim.x);
+        if (threadIdx.x == 0) {
+            st->alpha = st->sigma / t;
+            *counter = 0;
+        }
+    }
+}
+
+// p += alpha s, r -= alpha w (:451-452), |r|_inf and the stopping test (:453).
+__global__ __launch_bounds__(kBlock) void pcg_axpy_kernel(float* __restrict__ p, float* __restrict__ r,
+                                                          const float* __restrict__ s,
+                                                          const float* __restrict__ w, long n, PcgState* st,
+                                                          double* partials, unsigned* counter) {
+    if (st->done) return;
+    const float a = (float)st->alpha;
+    float m = 0.0f;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        p[e] = fmaf(a, s[e], p[e]);
+        const float rv = fmaf(-a, w[e], r[e]);
+        r[e] = rv;
+        m = fmaxf(m, fabsf(rv));
+    }
+    double bm = block_max((double)m);
+    if (threadIdx.x == 0) partials[blockIdx.x] = bm;
+    if (last_block(counter)) {
+        double rn = max_partials(partials, gridDim.x);
+        if (threadIdx.x == 0) {
+            st->r_norm = rn;
+            if (rn <= st->tol * st->rhs_norm) st->done = 1;
+            *counter = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FluidSim2D::updateVelocity (src/FluidSim2D.cpp:469-550): newMac from mac and
+// the pressure gradient with ghost-fluid terms; faces not next to FLUID are
+// marked unknown (mask ~0) for extrapolation.  Faces u(nx, j) and v(i, ny)
+// lie outside the reference's loops and keep mac's value (mask 0).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void update_velocity_kernel(
+    const uint8_t* __restrict__ cell, const double* __restrict__ phi, const float* __restrict__ p,
+    const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ nu, float* __restrict__ nv,
+    uint32_t* __restrict__ mu, uint32_t* __restrict__ mv, int nx, int ny, double scale) {
+    const long nU = (long)(nx + 1) * ny, nV = (long)nx * (ny + 1);
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nU + nV; e += (long)gridDim.x * blockDim.x) {
+        if (e < nU) {
+            const int i = (int)(e % (nx + 1)), j = (int)(e / (nx + 1));
+            double val = u[e];
+            uint32_t mk = 0;
+            if (i < nx) {
+                const long c = (long)j * nx + i;
+                const uint8_t cc = cell[c];
+                const uint8_t cl = i > 0 ? cell[c - 1] : SOLID;
+                if ((i > 0 && cl == FLUID) || cc == FLUID) {
+                    if ((i == 0 || cl == SOLID) || cc == SOLID) val = 0;
+                    else if (i > 0 && cl == EMPTY)
+                        val -= scale * (1 - fmax(phi[c - 1] / phi[c], -1e3)) * (double)p[c];
+                    else if (cc == EMPTY)
+                        val -= scale * (fmax(phi[c] / phi[c - 1], -1e3) - 1) * (double)p[c - 1];
+                    else if (i > 0) val -= scale * ((double)p[c] - (double)p[c - 1]);
+                    else val = 0;
+                } else {
+                    mk = 0xFFFFFFFFu;
+                }
+            }
+            nu[e] = (float)val;
+            mu[e] = mk;
+        } else {
+            const long f = e - nU;
+            const int i = (int)(f % nx), j = (int)(f / nx);
+            double val = v[f];
+            uint32_t mk = 0;
+            if (j < ny) {
+                const long c = (long)j * nx + i;
+                const uint8_t cc = cell[c];
+                const uint8_t cb = j > 0 ? cell[c - nx] : SOLID;
+                if ((j > 0 && cb == FLUID) || cc == FLUID) {
+                    if ((j == 0 || cb == SOLID) || cc == SOLID) val = 0;
+                    else if (j > 0 && cb == EMPTY)
+                        val -= scale * (1 - fmax(phi[c - nx] / phi[c], -1e3)) * (double)p[c];
+                    else if (cc == EMPTY)
+                        val -= scale * (fmax(phi[c] / phi[c - nx], -1e3) - 1) * (double)p[c - nx];
+                    else if (j > 0) val -= scale * ((double)p[c] - (double)p[c - nx]);
+                    else val = 0;
+                } else {
+                    mk = 0xFFFFFFFFu;
+                }
+            }
+            nv[f] = (float)val;
+            mv[f] = mk;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Extrapolation (Array2D<double>::extrapolate, include/Array2D.h:552-591).
+// The reference's FIFO BFS is layer-synchronous: an entry of layer L takes the
+// mean of its neighbours of layer < L, all final before layer L starts.  Pass 1
+// marks layer 1 in parallel and compacts it; pass 2 runs the layers in one
+// workgroup per array, with a barrier between layers.
+// ---------------------------------------------------------------------------
+struct ExtraArr {
+    float* val;
+    uint32_t* mask;
+    int* front0;
+    int* front1;
+    int* count;  // [0] size of layer-1 frontier
+    int w, h;
+};
+
+__global__ __launch_bounds__(kBlock) void extrap_layer1_kernel(ExtraArr A, ExtraArr B) {
+    const long nA = (long)A.w * A.h, nB = (long)B.w * B.h;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nA + nB; e += (long)gridDim.x * blockDim.x) {
+        const ExtraArr& X = e < nA ? A : B;
+        const long f = e < nA ? e : e - nA;
+        const int i = (int)(f % X.w), j = (int)(f / X.w);
+        if (X.mask[f] == 0) continue;
+        // a neighbour flipping from ~0 to 1 concurrently reads as non-zero either way
+        bool k = (i > 0 && X.mask[f - 1] == 0) || (i < X.w - 1 && X.mask[f + 1] == 0) ||
+                 (j > 0 && X.mask[f - X.w] == 0) || (j < X.h - 1 && X.mask[f + X.w] == 0);
+        if (k) {
+            X.mask[f] = 1;
+            int slot = atomicAdd(X.count, 1);
+            X.front0[slot] = (int)f;
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void extrap_layers_kernel(ExtraArr A, ExtraArr B) {
+    ExtraArr X = blockIdx.x == 0 ? A : B;
+    __shared__ int n_cur, n_next;
+    if (threadIdx.x == 0) {
+        n_cur = *X.count;
+        n_next = 0;
+    }
+    __syncthreads();
+    int* cur = X.front0;
+    int* nxt = X.front1;
+    for (uint32_t L = 1; n_cur > 0; ++L) {
+        for (int k = threadIdx.x; k < n_cur; k += blockDim.x) {
+            const int f = cur[k];
+            const int i = f % X.w, j = f / X.w;
+            const int nb[4] = {i > 0 ? f - 1 : -1, i < X.w - 1 ? f + 1 : -1, j > 0 ? f - X.w : -1,
+                               j < X.h - 1 ? f + X.w : -1};
+            float sum = 0.0f;
+            int cnt = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (nb[q] < 0) continue;
+                const uint32_t mq = X.mask[nb[q]];
+                if (mq < L) {
+                    sum += X.val[nb[q]];
+                    cnt++;
+                }
+            }
+            X.val[f] = cnt == 0 ? 0.0f : sum / (float)cnt;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (nb[q] < 0) continue;
+                if (atomicCAS(&X.mask[nb[q]], 0xFFFFFFFFu, L + 1) == 0xFFFFFFFFu) {
+                    int slot = atomicAdd(&n_next, 1);
+                    nxt[slot] = nb[q];
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            n_cur = n_next;
+            n_next = 0;
+        }
+        __syncthreads();
+        int* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Particles.  Cell keys use the reference's truncation (int)(x/dx)
+// (src/FluidSim2D.cpp:767) in FP64.  Sorting is a counting sort by cell key
+// (histogram, exclusive scan, scatter) followed by an in-segment sort by
+// particle index, so the order is (key, index): the order a stable sort of the
+// particles by key produces, and the order in which the reference's serial
+// loops meet the particles of one cell.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void particle_keys_kernel(const float* __restrict__ px,
+                                                               const float* __restrict__ py, long np,
+                                                               double dx, int nx, int ny, int* __restrict__ key,
+                                                               int* __restrict__ counts) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < np; e += (long)gridDim.x * blockDim.x) {
+        int x = (int)((double)px[e] / dx), y = (int)((double)py[e] / dx);
+        x = clampi(x, 0, nx - 1);
+        y = clampi(y, 0, ny - 1);
+        const int k = y * nx + x;
+        key[e] = k;
+        atomicAdd(&counts[k], 1);
+    }
+}
+
+// Exclusive scan in three passes over 1024-element tiles.
+__global__ __launch_bounds__(kBlock) void scan_tiles_kernel(const int* __restrict__ in, int* __restrict__ out,
+                                                            int* __restrict__ tile_sums, long n) {
+    __shared__ int sh[1024];
+    const long base = (long)blockIdx.x * 1024;
+    for (int k = threadIdx.x; k < 1024; k += kBlock) sh[k] = base + k < n ? in[base + k] : 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {  // 1024 adds: small next to the memory traffic of the tile
+        int run = 0;
+        for (int k = 0; k < 1024; k++) {
+            int v = sh[k];
+            sh[k] = run;
+            run += v;
+        }
+        tile_sums[blockIdx.x] = run;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 1024; k += kBlock)
+        if (base + k < n) out[base + k] = sh[k];
+}
+__global__ void scan_sums_kernel(int* tile_sums, int ntiles) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        int run = 0;
+        for (int k = 0; k < ntiles; k++) {
+            int v = tile_sums[k];
+            tile_sums[k] = run;
+            run += v;
+        }
+    }
+}
+__global__ __launch_bounds__(kBlock) void scan_add_kernel(int* __restrict__ out, const int* __restrict__ tile_sums,
+                                                          long n) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+        out[e] += tile_sums[e / 1024];
+}
+
+__global__ __launch_bounds__(kBlock) void particle_scatter_kernel(const int* __restrict__ key, long np,
+                                                                  int* __restrict__ cursor,
+                                                                  int* __restrict__ sorted) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < np; e += (long)gridDim.x * blockDim.x) {
+        int slot = atomicAdd(&cursor[key[e]], 1);
+        sorted[slot] = (int)e;
+    }
+}
+__global__ __launch_bounds__(kBlock) void segment_sort_kernel(const int* __restrict__ start,
+                                                              const int* __restrict__ counts,
+                                                              int* __restrict__ sorted, long ncells) {
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < ncells; c += (long)gridDim.x * blockDim.x) {
+        const int s0 = start[c], n = counts[c];
+        for (int a = 1; a < n; a++) {
+            int v = sorted[s0 + a], b = a - 1;
+            while (b >= 0 && sorted[s0 + b] > v) {
+                sorted[s0 + b + 1] = sorted[s0 + b];
+                b--;
+            }
+            sorted[s0 + b + 1] = v;
+        }
+    }
+}
+
+// FluidSim2D::transferVelocityToGrid (src/FluidSim2D.cpp:144-204) as a gather:
+// every face sums the tent weights (Array2D::linearDistribute, Array2D.h:361-376)
+// of the particles of the cells that can reach it, divides by the weight sum,
+// and faces whose value is exactly 0 become unknown (:192-201).
+__global__ __launch_bounds__(kBlock) void p2g_kernel(const float* __restrict__ px, const float* __restrict__ py,
+                                                     const float* __restrict__ pvx, const float* __restrict__ pvy,
+                                                     const int* __restrict__ start, const int* __restrict__ counts,
+                                                     const int* __restrict__ sorted, float* __restrict__ u,
+                                                     float* __restrict__ v, uint32_t* __restrict__ mu,
+                                                     uint32_t* __restrict__ mv, int nx, int ny, double dx) {
+    const long nU = (long)(nx + 1) * ny, nV = (long)nx * (ny + 1);
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nU + nV; e += (long)gridDim.x * blockDim.x) {
+        const bool isU = e < nU;
+        const long f = isU ? e : e - nU;
+        const int W = isU ? nx + 1 : nx, H = isU ? ny : ny + 1;
+        const int i = (int)(f % W), j = (int)(f / W);
+        // cells whose particles reach this face: u: x in {i-1,i}, y in {j-1..j+1};
+        // v: x in {i-1..i+1}, y in {j-1,j}
+        const int cx0 = i - 1, cx1 = isU ? i : i + 1;
+        const int cy0 = j - 1, cy1 = isU ? j + 1 : j;
+        double sv = 0, sw = 0;
+        for (int cy = cy0; cy <= cy1; cy++) {
+            if (cy < 0 || cy >= ny) continue;
+            for (int cx = cx0; cx <= cx1; cx++) {
+                if (cx < 0 || cx >= nx) continue;
+                const int c = cy * nx + cx, s0 = start[c], n = counts[c];
+                for (int k = 0; k < n; k++) {
+                    const int pe = sorted[s0 + k];
+                    double gx = (double)px[pe] / dx, gy = (double)py[pe] / dx;
+                    if (isU) gy -= 0.5;
+                    else gx -= 0.5;
+                    const int ui = (int)gx, uj = (int)gy;
+                    const double ddx = gx - ui, ddy = gy - uj;
+                    const int x1 = clampi(ui, 0, W - 1), x2 = clampi(ui + 1, 0, W - 1);
+                    const int y1 = clampi(uj, 0, H - 1), y2 = clampi(uj + 1, 0, H - 1);
+                    double wt = 0;
+                    if (y1 == j) {
+                        if (x1 == i) wt += (1 - ddx) * (1 - ddy);
+                        if (x2 == i) wt += ddx * (1 - ddy);
+                    }
+                    if (y2 == j) {
+                        if (x1 == i) wt += (1 - ddx) * ddy;
+                        if (x2 == i) wt += ddx * ddy;
+                    }
+                    if (wt != 0) {
+                        sv += wt * (double)(isU ? pvx[pe] : pvy[pe]);
+                        sw += wt;
+                    }
+                }
+            }
+        }
+        const double val = sw > 0 ? sv / sw : sv;
+        const float fv = (float)val;
+        if (isU) {
+            u[f] = fv;
+            mu[f] = fv == 0.0f ? 0xFFFFFFFFu : 0u;
+        } else {
+            v[f] = fv;
+            mv[f] = fv == 0.0f ? 0xFFFFFFFFu : 0u;
+        }
+    }
+}
+
+// FluidSim2D::updateParticleVelocities (src/FluidSim2D.cpp:552-568): PIC/FLIP
+// blend of bilinear samples of newMac and of (newMac - mac).
+__device__ __forceinline__ void bilinear_pair(const float* __restrict__ a_new, const float* __restrict__ a_old,
+                                              int w, int h, double gx, double gy, float& pic, float& diff) {
+    int ui = (int)gx, uj = (int)gy;
+    float ddx = (float)(gx - ui), ddy = (float)(gy - uj);
+    int x1 = clampi(ui, 0, w - 1), x2 = clampi(ui + 1, 0, w - 1);
+    int y1 = clampi(uj, 0, h - 1), y2 = clampi(uj + 1, 0, h - 1);
+    const long i11 = (long)y1 * w + x1, i21 = (long)y1 * w + x2, i12 = (long)y2 * w + x1, i22 = (long)y2 * w + x2;
+    const float w11 = (1 - ddx) * (1 - ddy), w21 = ddx * (1 - ddy), w12 = (1 - ddx) * ddy, w22 = ddx * ddy;
+    pic = a_new[i11] * w11 + a_new[i21] * w21 + a_new[i12] * w12 + a_new[i22] * w22;
+    diff = (a_new[i11] - a_old[i11]) * w11 + (a_new[i21] - a_old[i21]) * w21 + (a_new[i12] - a_old[i12]) * w12 +
+           (a_new[i22] - a_old[i22]) * w22;
+}
+
+__global__ __launch_bounds__(kBlock) void g2p_kernel(const float* __restrict__ px, const float* __restrict__ py,
+                                                     float* __restrict__ pvx, float* __restrict__ pvy, long np,
+                                                     const float* __restrict__ nu, const float* __restrict__ nv,
+                                                     const float* __restrict__ u, const float* __restrict__ v,
+                                                     int nx, int ny, double dx, float alpha) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < np; e += (long)gridDim.x * blockDim.x) {
+        const double gx = (double)px[e] / dx, gy = (double)py[e] / dx;
+        float picu, du, picv, dv;
+        bilinear_pair(nu, u, nx + 1, ny, gx, gy - 0.5, picu, du);
+        bilinear_pair(nv, v, nx, ny + 1, gx - 0.5, gy, picv, dv);
+        const float fu = pvx[e] + du, fv = pvy[e] + dv;
+        pvx[e] = alpha * picu + (1 - alpha) * fu;
+        pvy[e] = alpha * picv + (1 - alpha) * fv;
+    }
+}
+
+// FluidSim2D::applyAdvection (src/FluidSim2D.cpp:570-604): CFL scan, forward
+// Ralston RK3 through the Catmull-Rom field, clampPos, NaN flag (on the
+// position before the move, as the reference checks it).
+__global__ __launch_bounds__(kBlock) void particle_advect_kernel(float* __restrict__ px, float* __restrict__ py,
+                                                                 const float* __restrict__ pvx,
+                                                                 const float* __restrict__ pvy, long np, MacView m,
+                                                                 double dt, ClampBox cb, Diag* dg, double* partials,
+                                                                 unsigned* counter) {
+    double cmax = 0;
+    int nan = 0;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < np; e += (long)gridDim.x * blockDim.x) {
+        const double c = ((double)pvx[e] + (double)pvy[e]) * dt / m.dx;
+        cmax = c > cmax ? c : cmax;
+        const double x = px[e], y = py[e];
+        const double k1x = vel_interp_u(m, x, y), k1y = vel_interp_v(m, x, y);
+        const double x2 = x + 0.5 * dt * k1x, y2 = y + 0.5 * dt * k1y;
+        const double k2x = vel_interp_u(m, x2, y2), k2y = vel_interp_v(m, x2, y2);
+        const double x3 = x + 0.75 * dt * k2x, y3 = y + 0.75 * dt * k2y;
+        const double k3x = vel_interp_u(m, x3, y3), k3y = vel_interp_v(m, x3, y3);
+        double nx_ = x + (2. / 9.) * dt * k1x, ny_ = y + (2. / 9.) * dt * k1y;
+        nx_ = nx_ + (3. / 9.) * dt * k2x;
+        ny_ = ny_ + (3. / 9.) * dt * k2y;
+        nx_ = nx_ + (4. / 9.) * dt * k3x;
+        ny_ = ny_ + (4. / 9.) * dt * k3y;
+        if (isnan(x) || isnan(y)) nan = 1;
+        px[e] = (float)clampd(nx_, cb.lox, cb.hix);
+        py[e] = (float)clampd(ny_, cb.loy, cb.hiy);
+    }
+    double bm = block_max(cmax);
+    __shared__ int sh_nan;
+    if (threadIdx.x == 0) sh_nan = 0;
+    __syncthreads();
+    if (nan) atomicOr(&sh_nan, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = bm;
+        if (sh_nan) atomicOr(&dg->nan_flag, 1);
+    }
+    if (last_block(counter)) {
+        double t = max_partials(partials, gridDim.x);
+        if (threadIdx.x == 0) {
+            dg->cfl_max = t;
+            *counter = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Level set (src/FluidSim2D.cpp:752-920), in FP64 and without contraction so
+// that every distance, comparison and sign equals the reference's double
+// arithmetic on the same positions.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double corner_dist(int i, int j, double x, double y, double dx, double dr) {
+#pragma clang fp contract(off)
+    const double ex = x - i * dx, ey = y - j * dx;
+    return sqrt(ex * ex + ey * ey) - dr;
+}
+
+// Scatter-min of constructFromParticles (:764-773): the first particle (in
+// index order) with the smallest distance to the cell's corner wins.
+__global__ __launch_bounds__(kBlock) void ls_seed_kernel(const float* __restrict__ px, const float* __restrict__ py,
+                                                         const int* __restrict__ start, const int* __restrict__ counts,
+                                                         const int* __restrict__ sorted, int* __restrict__ t,
+                                                         int nx, long ncells, double dx, double dr) {
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < ncells; c += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % nx), j = (int)(c / nx);
+        double best = HUGE_VAL;
+        int bt = -1;
+        const int s0 = start[c], n = counts[c];
+        for (int k = 0; k < n; k++) {
+            const int e = sorted[s0 + k];
+            const double d = corner_dist(i, j, px[e], py[e], dx, dr);
+            if (d < best) {
+                best = d;
+                bt = e;
+            }
+        }
+        t[c] = bt;
+    }
+}
+
+// One directional Gauss-Seidel sweep of the closest-particle propagation
+// (FluidSim2D.cpp:775-792).  Carried state: (index, x, y) of the adopted
+// particle, so the two "new" candidates need no memory access; the current
+// value and the two "old" neighbours come from the previous sweep's array.
+struct ClosestOp {
+    static constexpr int NW = 3;
+    const int* __restrict__ t_in;
+    int* __restrict__ t_out;
+    const float* __restrict__ px;
+    const float* __restrict__ py;
+    int nx, ny, flipx, flipy;
+    double dx, dr;
+    __device__ bool skip() const { return false; }
+    __device__ void boundary(uint32_t* s) const {
+        s[0] = 0xFFFFFFFFu;
+        s[1] = s[2] = 0;
+    }
+    __device__ void compute(int a, int b, const uint32_t* left, const uint32_t* below, uint32_t* out) {
+        const int i = flipx ? nx - 1 - a : a;
+        const int j = flipy ? ny - 1 - b : b;
+        const long e = (long)j * nx + i;
+        int tc = t_in[e];
+        double cphi = HUGE_VAL;
+        float cx = 0, cy = 0;
+        if (tc >= 0) {
+            cx = px[tc];
+            cy = py[tc];
+            cphi = corner_dist(i, j, cx, cy, dx, dr);
+        }
+        // candidates in the reference's order: (i-1,j), (i+1,j), (i,j-1), (i,j+1)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            int tn;
+            float qx = 0, qy = 0;
+            const bool xs = k < 2;
+            const bool minus = (k & 1) == 0;
+            // is this neighbour already updated in the current sweep?
+            const bool fresh = xs ? (minus != (bool)flipx) : (minus != (bool)flipy);
+            if (fresh) {
+                const uint32_t* st = xs ? left : below;
+                tn = (int)st[0];
+                qx = u2f(st[1]);
+                qy = u2f(st[2]);
+            } else {
+                const int ii = xs ? (minus ? i - 1 : i + 1) : i;
+                const int jj = xs ? j : (minus ? j - 1 : j + 1);
+                if (ii < 0 || ii >= nx || jj < 0 || jj >= ny) continue;
+                tn = t_in[(long)jj * nx + ii];
+                if (tn >= 0) {
+                    qx = px[tn];
+                    qy = py[tn];
+                }
+            }
+            if (tn < 0) continue;
+            const double d = corner_dist(i, j, qx, qy, dx, dr);
+            if (d < cphi) {
+                cphi = d;
+                tc = tn;
+                cx = qx;
+                cy = qy;
+            }
+        }
+        t_out[e] = tc;
+        out[0] = (uint32_t)tc;
+        out[1] = f2u(cx);
+        out[2] = f2u(cy);
+    }
+    __device__ void strip_done(int) {}
+    __device__ void last_done(int) {}
+};
+
+// phi from the adopted particle; then the surface marks (:803-834) and the
+// interior fill (:836-842) of redistance, reading phi of the finished sweeps.
+__global__ __launch_bounds__(kBlock) void ls_phi_kernel(const int* __restrict__ t, const float* __restrict__ px,
+                                                        const float* __restrict__ py, double* __restrict__ phi,
+                                                        int nx, long ncells, double dx, double dr) {
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < ncells; c += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % nx), j = (int)(c / nx);
+        const int e = t[c];
+        phi[c] = e >= 0 ? corner_dist(i, j, px[e], py[e], dx, dr) : HUGE_VAL;
+    }
+}
+__global__ __launch_bounds__(kBlock) void ls_surface_kernel(const double* __restrict__ phi, double* __restrict__ out,
+                                                            int nx, int ny) {
+    const long n = (long)nx * ny;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % nx), j = (int)(c / nx);
+        const double p0 = phi[c];
+        bool surf = false;
+        // pairs (i,j)-(i+1,j) and (i,j)-(i,j+1) are examined for i < nx-1, j < ny-1
+        if (i < nx - 1 && j < ny - 1) surf = surf || (p0 * phi[c + 1] < 0) || (p0 * phi[c + nx] < 0);
+        if (i >= 1 && j < ny - 1) surf = surf || (phi[c - 1] * p0 < 0);
+        if (j >= 1 && i < nx - 1) surf = surf || (phi[c - nx] * p0 < 0);
+        out[c] = (!surf && p0 < 0) ? -HUGE_VAL : p0;
+    }
+}
+
+// One directional sweep of the eikonal update (:845-902) on cells with phi < 0,
+// for logical a >= 1 and b >= 1.  Carried state: phi (FP64).
+struct EikonalOp {
+    static constexpr int NW = 2;
+    const double* __restrict__ in;
+    double* __restrict__ outp;
+    int nx, ny, flipx, flipy;
+    double dx;
+    __device__ bool skip() const { return false; }
+    __device__ void boundary(uint32_t* s) const { s[0] = s[1] = 0; }
+    __device__ void compute(int a, int b, const uint32_t* left, const uint32_t* below, uint32_t* out) {
+#pragma clang fp contract(off)
+        const int i = flipx ? nx - 1 - a : a;
+        const int j = flipy ? ny - 1 - b : b;
+        const long e = (long)j * nx + i;
+        double cur = in[e];
+        if (a >= 1 && b >= 1 && !(cur >= 0)) {
+            const double A = fabs(u2d(left)), B = fabs(u2d(below));
+            const double phi0 = A < B ? A : B;
+            const double phi1 = A > B ? A : B;
+            double d = phi0 + dx;
+            if (d > phi1) d = 0.5 * (phi0 + phi1 + sqrt(2 * dx * dx - (phi1 - phi0) * (phi1 - phi0)));
+            if (d < -cur) cur = -d;
+        }
+        outp[e] = cur;
+        d2u(cur, out);
+    }
+    __device__ void strip_done(int) {}
+    __device__ void last_done(int) {}
+};
+
+// Smoothing pass phi = min(phi, mean of 4 neighbours) on 1..n-2 (:904-919).
+__global__ __launch_bounds__(kBlock) void ls_smooth_kernel(const double* __restrict__ in, double* __restrict__ out,
+                                                           int nx, int ny) {
+#pragma clang fp contract(off)
+    const long n = (long)nx * ny;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % nx), j = (int)(c / nx);
+        double v = in[c];
+        if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2) {
+            const double avg = 0.25 * (in[c - 1] + in[c + 1] + in[c - nx] + in[c + nx]);
+            if (avg < v) v = avg;
+        }
+        out[c] = v;
+    }
+}
+
+// Classification (:657-664): non-solid cells are FLUID iff phi < 0.
+__global__ void classify_kernel(uint8_t* __restrict__ cell, const double* __restrict__ phi, long n) {
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
+        if (cell[c] != SOLID) cell[c] = phi[c] < 0.0 ? FLUID : EMPTY;
+}
+
+// Diagnostics (:709-731): volume, grid energy (Catmull-Rom velocity at cell
+// corners), particle energy; FP64 sums.
+__global__ __launch_bounds__(kBlock) void diagnostics_kernel(const uint8_t* __restrict__ cell, MacView m,
+                                                             const float* __restrict__ px,
+                                                             const float* __restrict__ py,
+                                                             const float* __restrict__ pvx,
+                                                             const float* __restrict__ pvy, long np, double rho,
+                                                             double gx, double gy, double pmass, Diag* dg,
+                                                             double* partials, unsigned* counter) {
+    const long nc = (long)m.nx * m.ny;
+    const double dx = m.dx, cm = rho * dx * dx;
+    double vol = 0, eg = 0, ep = 0;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < nc + np; c += (long)gridDim.x * blockDim.x) {
+        if (c < nc) {
+            if (cell[c] != FLUID) continue;
+            const int i = (int)(c % m.nx), j = (int)(c / m.nx);
+            const double x = i * dx, y = j * dx;
+            const double vx = vel_interp_u(m, x, y), vy = vel_interp_v(m, x, y);
+            vol += 1.0;
+            eg += 0.5 * cm * (vx * vx + vy * vy);
+            eg -= cm * (gx * x + gy * y);
+        } else {
+            const long e = c - nc;
+            const double vx = pvx[e], vy = pvy[e];
+            ep += 0.5 * pmass * (vx * vx + vy * vy);
+            ep -= pmass * (gx * (double)px[e] + gy * (double)py[e]);
+        }
+    }
+    double a = block_sum(vol), b = block_sum(eg), c2 = block_sum(ep);
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x + 0] = a;
+        partials[3 * blockIdx.x + 1] = b;
+        partials[3 * blockIdx.x + 2] = c2;
+    }
+    if (last_block(counter)) {
+        double s0 = 0, s1 = 0, s2 = 0;
+        for (int k = threadIdx.x; k < (int)gridDim.x; k += kBlock) {
+            s0 += partials[3 * k];
+            s1 += partials[3 * k + 1];
+            s2 += partials[3 * k + 2];
+        }
+        s0 = block_sum(s0);
+        s1 = block_sum(s1);
+        s2 = block_sum(s2);
+        if (threadIdx.x == 0) {
+            dg->volume = s0 * dx * dx;
+            dg->grid_energy = s1;
+            dg->particle_energy = s2;
+            *counter = 0;
+        }
+    }
+}
+
+}  // namespace fs2d

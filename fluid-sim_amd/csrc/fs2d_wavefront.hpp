@@ -1,0 +1,192 @@
+// fs2d_wavefront.hpp -- skewed-strip wavefront engine for 2D lexicographic
+// recurrences on gfx950.
+//
+// A recurrence of the form  X(a, b) = F(X(a-1, b), X(a, b-1), inputs(a, b))
+// over an NA x NB grid has anti-diagonal a+b as its only parallel axis.  Four
+// passes of the reference have this shape: the MIC(0) factor and the forward /
+// backward triangular solves of the preconditioner (src/FluidSim2D.cpp:364-421),
+// and the Gauss-Seidel sweeps of the level set (include/FluidSim2D.h:178-203,
+// src/FluidSim2D.cpp:775-792, 845-902).  Sweeps in the other three directions
+// are this canonical one with the axes mirrored by the operator.
+//
+// Mapping: one wave64 owns a strip of 64 consecutive rows b = 64*s + lane.  At
+// step t lane l processes column a = t - l, so each lane walks its own row
+// left to right one step behind the lane below it:
+//   * X(a-1, b) is the lane's own result of step t-1 (a register);
+//   * X(a, b-1) is lane l-1's result of step t-1, moved by one DPP
+//     `wave_shr:1` (no LDS, no barrier);
+//   * lane 0 gets X(a, 64*s-1) from the strip below through a hand-off row in
+//     global memory.  Each entry is one 64-bit word {payload, epoch} written
+//     and read with agent-scope relaxed atomics, so a consumer that sees the
+//     current epoch also sees the payload; no fences are needed.
+// Strips are taken in order from an atomic ticket, so strip s-1 is always
+// owned by a workgroup that started earlier and never waits on strip s: the
+// kernel cannot deadlock however many workgroups are resident.  Every wait is
+// bounded (kSpinLimit); on expiry the kernel raises an error flag and runs to
+// completion, so every wave exits.  The last workgroup to finish resets the
+// ticket and advances the epoch, which keeps the launch free of host-side
+// per-launch arguments (replayable from a graph).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fs2d {
+
+constexpr int kSpinLimit = 1 << 16;  // polls of one memory round trip each: well under 1 s
+
+// ctl[0] ticket, ctl[1] finished workgroups, ctl[2] epoch (starts at 1).
+struct WaveCtl {
+    unsigned long long* handoff;  // [strips][NA][NW]
+    unsigned* ctl;
+    int* err;
+};
+
+__device__ __forceinline__ unsigned long long atomic_ld64(const unsigned long long* p) {
+    return __hip_atomic_load(const_cast<unsigned long long*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_st64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned readlane_u32(unsigned v, int lane) {
+    return (unsigned)__builtin_amdgcn_readlane((int)v, lane);
+}
+
+// Op interface:
+//   static constexpr int NW;                       32-bit words of carried state
+//   bool skip() const;                             whole pass is a no-op
+//   void boundary(uint32_t s[NW]) const;           state of out-of-grid neighbours
+//   void compute(int a, int b, const uint32_t left[NW], const uint32_t below[NW],
+//                uint32_t out[NW]);                new state of (a, b); stores results
+//   void strip_done(int strip);                    once per strip, whole wave
+//   void last_done(int nstrips);                   by the last workgroup, whole wave
+template <class Op>
+__global__ __launch_bounds__(64) void wavefront_kernel(Op op, int NA, int NB, WaveCtl c) {
+    constexpr int NW = Op::NW;
+    const int lane = threadIdx.x;
+    const int nstrips = (NB + 63) >> 6;
+    __shared__ unsigned sh_strip, sh_epoch;
+    __shared__ int sh_err;
+    if (lane == 0) {
+        sh_strip = atomicAdd(&c.ctl[0], 1u);
+        sh_epoch = __hip_atomic_load(&c.ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sh_err = __hip_atomic_load(c.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    const int s = (int)sh_strip;
+    const unsigned epoch = sh_epoch;
+    // After an expired wait (sticky error) later launches do no work at all.
+    const bool skip = op.skip() || sh_err != 0;
+
+    if (!skip && s < nstrips) {
+        const int b = s * 64 + lane;
+        const int rows = min(64, NB - s * 64);
+        const bool row_ok = lane < rows;
+        uint32_t bnd[NW], left[NW], out[NW];
+        op.boundary(bnd);
+#pragma unroll
+        for (int w = 0; w < NW; w++) left[w] = out[w] = bnd[w];
+
+        const unsigned long long* src = c.handoff + (size_t)(s > 0 ? s - 1 : 0) * NA * NW;
+        unsigned long long* dst = c.handoff + (size_t)s * NA * NW;
+        const bool produce = (s + 1 < nstrips);
+        unsigned long long hv[NW];
+#pragma unroll
+        for (int w = 0; w < NW; w++) hv[w] = 0;
+        int c0 = -64;
+        bool failed = false;
+
+        const int T = NA + rows - 1;
+        for (int t = 0; t < T; ++t) {
+            const int a = t - lane;
+            uint32_t bl0[NW];
+            if (s == 0 || t >= NA) {
+#pragma unroll
+                for (int w = 0; w < NW; w++) bl0[w] = bnd[w];
+            } else {
+                if (t >= c0 + 64) {
+                    c0 += 64;
+                    const int col = c0 + lane;
+#pragma unroll
+                    for (int w = 0; w < NW; w++) hv[w] = col < NA ? atomic_ld64(src + (size_t)col * NW + w) : 0ull;
+                }
+                const int k = t - c0;
+                for (int spin = 0;; ++spin) {
+                    bool ok = true;
+#pragma unroll
+                    for (int w = 0; w < NW; w++)
+                        ok = ok && (readlane_u32((unsigned)(hv[w] >> 32), k) == epoch);
+                    if (ok || failed) break;
+                    if (spin >= kSpinLimit) {
+                        failed = true;
+                        if (lane == 0) atomicExch(c.err, 1);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    const int col = c0 + lane;
+#pragma unroll
+                    for (int w = 0; w < NW; w++)
+                        if (col < NA && (unsigned)(hv[w] >> 32) != epoch)
+                            hv[w] = atomic_ld64(src + (size_t)col * NW + w);
+                }
+#pragma unroll
+                for (int w = 0; w < NW; w++) bl0[w] = readlane_u32((unsigned)hv[w], k);
+            }
+            uint32_t below[NW];
+#pragma unroll
+            for (int w = 0; w < NW; w++)
+                below[w] = (uint32_t)__builtin_amdgcn_update_dpp((int)bl0[w], (int)out[w], 0x138 /* wave_shr:1 */,
+                                                                 0xf, 0xf, false);
+            if (a >= 0 && a < NA && row_ok) {
+                op.compute(a, b, left, below, out);
+#pragma unroll
+                for (int w = 0; w < NW; w++) left[w] = out[w];
+                if (produce && lane == 63) {
+#pragma unroll
+                    for (int w = 0; w < NW; w++)
+                        atomic_st64(dst + (size_t)a * NW + w,
+                                    (unsigned long long)out[w] | ((unsigned long long)epoch << 32));
+                }
+            }
+        }
+        op.strip_done(s);
+    }
+
+    // Completion protocol: the last workgroup resets the ticket, advances the
+    // epoch and runs the operator's final reduction.
+    __syncthreads();
+    __threadfence();
+    __shared__ unsigned sh_last;
+    if (lane == 0) {
+        unsigned done = atomicAdd(&c.ctl[1], 1u);
+        sh_last = (done == (unsigned)nstrips - 1u);
+    }
+    __syncthreads();
+    if (sh_last) {
+        __threadfence();
+        if (!skip) op.last_done(nstrips);
+        if (lane == 0) {
+            unsigned next = epoch + 1u;
+            if (next == 0u) next = 1u;
+            c.ctl[0] = 0u;
+            c.ctl[1] = 0u;
+            __hip_atomic_store(&c.ctl[2], next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+        }
+    }
+}
+
+// Bit casts for carried state.
+__device__ __forceinline__ uint32_t f2u(float x) { return __float_as_uint(x); }
+__device__ __forceinline__ float u2f(uint32_t x) { return __uint_as_float(x); }
+__device__ __forceinline__ void d2u(double x, uint32_t* w) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    w[0] = (uint32_t)b;
+    w[1] = (uint32_t)(b >> 32);
+}
+__device__ __forceinline__ double u2d(const uint32_t* w) {
+    return __longlong_as_double((long long)(((unsigned long long)w[1] << 32) | w[0]));
+}
+
+}  // namespace fs2d

@@ -1,0 +1,203 @@
+/*
+ * fs2d.h -- C ABI of the MI355X (gfx950) implementation of fluid-sim's 2D
+ * Eulerian step.
+ *
+ * The reference exposes this path as a C++ value type, FluidSim2D
+ * (include/FluidSim2D.h:65-176, src/FluidSim2D.cpp), with public fields and
+ * stage methods; there is no plugin or FFI layer.  Each entry point below
+ * replaces one member or method of that type; the comment above it cites it.
+ * fluid-sim_amd/cpp/FluidSim2D.hpp rebuilds the reference's method names on
+ * top of these calls.
+ *
+ * Conventions
+ *   - Arrays are row-major, element (i, j) at [j * width + i], with the
+ *     reference's shapes (Array2D.h:41-44, MACGrid2D.h:19-25):
+ *       u, newu : (size_x + 1) x size_y        v, newv : size_x x (size_y + 1)
+ *       p, phi, cell, density : size_x x size_y
+ *     particle arrays are (x, y) pairs, particle order as created.
+ *   - Every call returns an fs2d_status; fs2d_last_error() explains the last
+ *     non-OK status of a handle.  No call exits the process (the reference
+ *     calls exit() on allocation failure and on NaN particles).
+ *   - One HIP stream per handle; calls on different handles may run on
+ *     different threads.  A handle is not thread-safe.
+ *   - Device state is FP32 (velocities, pressure, particles, density) except
+ *     the level set, which is FP64 as in the reference; dot products and
+ *     residual accumulations of the PCG are FP64.
+ */
+#ifndef FS2D_H
+#define FS2D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fs2d_sim fs2d_sim;
+
+typedef enum {
+    FS2D_OK = 0,
+    FS2D_WARN_MAXITER = 1,   /* PCG hit max_iters (reference: log_error, FluidSim2D.cpp:464-466) */
+    FS2D_WARN_CFL = 2,       /* particle CFL > 5 (reference: log_warn, FluidSim2D.cpp:582-585) */
+    FS2D_ERR_NAN = -1,       /* NaN particle position (reference: exit(1), FluidSim2D.cpp:598-601) */
+    FS2D_ERR_OOM = -2,       /* allocation failed (reference: exit, Array2D.h:69-71) */
+    FS2D_ERR_CONFIG = -3,    /* invalid configuration */
+    FS2D_ERR_ARG = -4,       /* invalid argument (field id, size, stage) */
+    FS2D_ERR_DEVICE = -5,    /* HIP runtime error */
+    FS2D_ERR_HANG = -6       /* a wavefront kernel's bounded wait expired */
+} fs2d_status;
+
+/* FluidSimMode (include/FluidSim2D.h:40-42). */
+typedef enum { FS2D_SEMILAGRANGIAN = 0, FS2D_PICFLIP = 1 } fs2d_mode;
+
+/* FluidCellType (include/FluidSim2D.h:44-46). */
+typedef enum { FS2D_EMPTY = 0, FS2D_FLUID = 1, FS2D_SOLID = 2 } fs2d_cell;
+
+/* FluidSim2D::StageType (include/FluidSim2D.h:93-97). */
+typedef enum {
+    FS2D_STAGE_INIT = 0,
+    FS2D_STAGE_CREATE_WATER_LEVEL_SET = 1,
+    FS2D_STAGE_TRANSFER_VELOCITY_TO_GRID = 2,
+    FS2D_STAGE_SEMI_LAGRANGIAN_ADVECTION = 3,
+    FS2D_STAGE_APPLY_GRAVITY = 4,
+    FS2D_STAGE_CREATE_SOLID_LEVEL_SET = 5,
+    FS2D_STAGE_APPLY_PROJECTION = 6,
+    FS2D_STAGE_UPDATE_VELOCITY = 7,
+    FS2D_STAGE_UPDATE_PARTICLE_VELOCITIES = 8,
+    FS2D_STAGE_APPLY_ADVECTION = 9
+} fs2d_stage;
+
+/* Field ids for fs2d_get_field / fs2d_set_field. */
+typedef enum {
+    FS2D_FIELD_U = 0,        /* mac.u    (FluidSim2D.h:71) */
+    FS2D_FIELD_V = 1,        /* mac.v */
+    FS2D_FIELD_NEWU = 2,     /* newMac.u (FluidSim2D.h:72) */
+    FS2D_FIELD_NEWV = 3,     /* newMac.v */
+    FS2D_FIELD_P = 4,        /* p        (FluidSim2D.h:73) */
+    FS2D_FIELD_PHI = 5,      /* waterLevelSet.phi (FluidSim2D.h:78) */
+    FS2D_FIELD_DENSITY = 6,  /* passive density (no reference counterpart) */
+    FS2D_FIELD_CELL = 7,     /* cell     (FluidSim2D.h:74), uint8 only */
+    FS2D_FIELD_PARTICLE_POS = 8, /* particles    (FluidSim2D.h:75), 2 per particle */
+    FS2D_FIELD_PARTICLE_VEL = 9  /* particleVels (FluidSim2D.h:76), 2 per particle */
+} fs2d_field;
+
+typedef enum { FS2D_F32 = 0, FS2D_F64 = 1, FS2D_U8 = 2 } fs2d_dtype;
+
+/* Build-only switches (no reference counterpart). */
+enum {
+    FS2D_FLAG_STATIC_CELLS = 1,  /* keep the created cell flags; no level set ("all-fluid" mode) */
+    FS2D_FLAG_DENSITY = 2        /* advect a passive cell-centred density in the SL stage */
+};
+
+/*
+ * FluidSim2DConfig (include/FluidSim2D.h:48-63), plus the solver constants the
+ * reference hard-codes (FluidSim2D.cpp:369-370, 429, 453) as parameters.
+ * fs2d_config_init() fills the reference's values: tau 0.999, sigma 0.25,
+ * pcg_tol 1e-12 (relative infinity norm), pcg_max_iters 200, seed 1 (glibc's
+ * default rand() seed, which create() uses through randf, FluidSim2D.cpp:19-21).
+ */
+typedef struct {
+    int size_x;
+    int size_y;
+    int particles_per_cell_sqrt;
+    double dt;
+    double dx;
+    double rho;
+    double gravity_x;
+    double gravity_y;
+    int mode;                       /* fs2d_mode */
+    double pic_flip_alpha;          /* PIC weight: 1 = pure PIC (FluidSim2D.cpp:564) */
+    const uint8_t* initial_values;  /* size_x*size_y flags, row-major; border must be SOLID */
+    double tau;
+    double sigma;
+    double pcg_tol;
+    int pcg_max_iters;
+    unsigned seed;
+    int flags;                      /* FS2D_FLAG_* */
+} fs2d_config;
+
+/* Per-step observables: FluidSim2D::waterVolume/totalEnergy/particleTotalEnergy
+ * (FluidSim2D.h:106-114, computed at FluidSim2D.cpp:709-731), currentTime
+ * (FluidSim2D.h:87) and the solver's state. */
+typedef struct {
+    double water_volume;
+    double total_energy;
+    double particle_total_energy;
+    double orig_water_volume;
+    double current_time;
+    double pcg_residual;      /* |r|_inf / |rhs|_inf at exit of the last solve */
+    double cfl_max;           /* max (vx+vy)*dt/dx over particles (FluidSim2D.cpp:572-581) */
+    int pcg_iters;            /* iterations of the last solve */
+    int pcg_hit_max;
+    long long n_particles;
+    long long steps;
+} fs2d_stats;
+
+/* Fill *cfg with the reference's defaults (FluidSim2D.h:80-91 and the
+ * constants above); size, dt, dx, mode and flags still have to be set. */
+fs2d_status fs2d_config_init(fs2d_config* cfg);
+
+/* FluidSim2D::create (src/FluidSim2D.cpp:23-76): allocate device state, copy
+ * the flags, seed k^2 jittered particles per FLUID cell. */
+fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out);
+
+/* FluidSim2D::free (src/FluidSim2D.cpp:78-92). */
+void fs2d_destroy(fs2d_sim* sim);
+
+/* FluidSim2D::runFrame / update (src/FluidSim2D.cpp:94-142): one step of the
+ * member dt; stage order by mode.  Returns the worst status of the stages. */
+fs2d_status fs2d_step(fs2d_sim* sim);
+
+/* One stage method (FluidSim2D.h:124-140), e.g. applyProjection(). */
+fs2d_status fs2d_run_stage(fs2d_sim* sim, int stage);
+
+/* Wait for all queued work of the handle. */
+fs2d_status fs2d_synchronize(fs2d_sim* sim);
+
+/* Public field access (FluidSim2D.h:71-78): copy between device state and a
+ * host buffer of `count` elements of `dtype` (conversion on the host side). */
+size_t fs2d_field_count(const fs2d_sim* sim, int field);
+fs2d_status fs2d_get_field(fs2d_sim* sim, int field, int dtype, void* host, size_t count);
+fs2d_status fs2d_set_field(fs2d_sim* sim, int field, int dtype, const void* host, size_t count);
+/* Replacing the particle set changes its size; velocities are zeroed. */
+fs2d_status fs2d_set_particles(fs2d_sim* sim, int dtype, const void* pos_xy, size_t n);
+
+/* Device pointer of a field (FP32 arrays, uint8 cell flags; particle fields
+ * are SoA: x then y, each n long).  For zero-copy interop on the handle's stream. */
+fs2d_status fs2d_device_field(fs2d_sim* sim, int field, void** dptr);
+void* fs2d_stream(fs2d_sim* sim);
+
+/* Runtime-mutable fields: gravity, picFlipAlpha, dt (FluidSim2D.h:80, 84, 89),
+ * written by demo/FluidRenderer2D.cpp:322, 382-388. */
+fs2d_status fs2d_set_gravity(fs2d_sim* sim, double gx, double gy);
+fs2d_status fs2d_set_pic_flip_alpha(fs2d_sim* sim, double alpha);
+fs2d_status fs2d_set_dt(fs2d_sim* sim, double dt);
+/* Solver constants of applyProjection (FluidSim2D.cpp:369-370, 429, 453). */
+fs2d_status fs2d_set_solver(fs2d_sim* sim, double tau, double sigma, double tol, int max_iters);
+
+/* waterVolume, totalEnergy, particleTotalEnergy, currentTime and solver stats. */
+fs2d_status fs2d_get_stats(fs2d_sim* sim, fs2d_stats* out);
+
+/* FluidSim2D::avgPressure / avgPressureInFluid / maxVelocity
+ * (src/FluidSim2D.cpp:607-638). */
+fs2d_status fs2d_avg_pressure(fs2d_sim* sim, int fluid_only, double* out);
+fs2d_status fs2d_max_velocity(fs2d_sim* sim, double* out);
+
+/* Per-kernel timing with HIP events on the handle's stream (no reference
+ * counterpart; PerformanceCounter, src/PerformanceCounter.cpp:9-74, timed
+ * stages with a host clock).  Kernel classes are listed by fs2d_kernel_name. */
+fs2d_status fs2d_set_timing(fs2d_sim* sim, int enable);
+int fs2d_kernel_count(void);
+const char* fs2d_kernel_name(int kernel);
+fs2d_status fs2d_get_kernel_time(fs2d_sim* sim, int kernel, double* total_ms, long long* launches);
+fs2d_status fs2d_reset_kernel_times(fs2d_sim* sim);
+
+const char* fs2d_last_error(const fs2d_sim* sim);
+const char* fs2d_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FS2D_H */

@@ -1,0 +1,341 @@
+"""GPU parity: every stage of the HIP path against the CPU oracle from an
+identical state, plus teacher-forced whole steps.
+
+Bars (north star): cell flags, particle cell indices and the level set's
+discrete decisions bit-exact; velocities, pressures and particle positions
+within 1e-5 relative L2 of the FP64 oracle.  The GPU stores FP32; the oracle
+is given the GPU's FP32 values (widened to FP64) so that both start from the
+same numbers.  The semi-Lagrangian comparison uses the oracle's out-of-place
+mode: the reference's in-place update (FluidSim2D.cpp:209-233) has no parallel
+equivalent; its gap is measured in test_sl_inplace_gap.
+"""
+import numpy as np
+import pytest
+
+import fs2d
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def pair(cells, **kw):
+    """A GPU sim and an oracle with the same configuration."""
+    okw = dict(kw)
+    static = okw.pop("static_cells", False)
+    dens = okw.pop("density", False)
+    g = fs2d.Sim(cells, static_cells=static, density=dens, **okw)
+    o = O.Oracle(cells, level_set=not static, density=dens,
+                 sl_inplace=False, **{k: v for k, v in okw.items() if k != "seed"})
+    return g, o
+
+
+def push(g, o, fields):
+    """Copy GPU state into the oracle (FP32 values widened to FP64)."""
+    for f in fields:
+        if f == fs2d.F_CELL:
+            o.set(O.F_CELL, g.get(fs2d.F_CELL))
+        elif f == fs2d.F_PHI:
+            o.set(O.F_PHI, g.get(fs2d.F_PHI, np.float64))
+        elif f == fs2d.F_PARTICLE_POS:
+            o.set(O.F_POS, g.get(f).astype(np.float64))
+        elif f == fs2d.F_PARTICLE_VEL:
+            o.set(O.F_VEL, g.get(f).astype(np.float64))
+        else:
+            o.set(f, g.get(f).astype(np.float64))
+
+
+def smooth_field(shape, rng, scale=1.0):
+    from scipy.ndimage import gaussian_filter
+    return (scale * gaussian_filter(rng.standard_normal(shape), 2.0, mode="nearest") * 4).astype(np.float32)
+
+
+def settle(g, o, steps=3, mode_stages=None):
+    """Run a few GPU steps so that velocities, levels and particles are generic."""
+    for _ in range(steps):
+        g.step()
+
+
+SCENE64 = dict(k=2, dt=0.005, dx=1.0 / 64, mode=fs2d.SEMILAGRANGIAN)
+
+
+# ---------------------------------------------------------------------------
+def test_create_seeding_matches_oracle():
+    cells = O.dam_break(64, 64)
+    g, o = pair(cells, **SCENE64)
+    pg = g.get(fs2d.F_PARTICLE_POS)
+    po = o.get(O.F_POS)
+    assert pg.shape == po.shape == (4 * 888, 2)
+    np.testing.assert_array_equal(pg, po.astype(np.float32))
+
+
+@pytest.mark.parametrize("n,mode", [(64, fs2d.SEMILAGRANGIAN), (96, fs2d.PICFLIP)])
+def test_level_set_bit_exact(n, mode):
+    cells = O.dam_break(n, n)
+    g, o = pair(cells, k=2, dt=0.005, dx=1.0 / n, mode=mode)
+    for it in range(3):
+        push(g, o, [fs2d.F_PARTICLE_POS, fs2d.F_PARTICLE_VEL, fs2d.F_U, fs2d.F_V, fs2d.F_CELL])
+        g.run_stage(fs2d.STAGE_LEVEL_SET)
+        o.run_stage(O.ST_LEVELSET)
+        np.testing.assert_array_equal(g.get(fs2d.F_CELL), o.get(O.F_CELL))
+        np.testing.assert_array_equal(g.get(fs2d.F_PHI, np.float64), o.get(O.F_PHI))
+        sg, so = g.stats(), o.stats()
+        assert sg["water_volume"] == pytest.approx(so["water_volume"], rel=1e-12)
+        assert sg["particle_total_energy"] == pytest.approx(so["particle_total_energy"], rel=1e-9)
+        assert sg["total_energy"] == pytest.approx(so["total_energy"], rel=1e-5)
+        g.step()  # move on to a generic state
+
+
+def test_sl_advection_stage():
+    n = 64
+    cells = O.dam_break(n, n)
+    g, o = pair(cells, **SCENE64)
+    rng = np.random.default_rng(0)
+    g.set(fs2d.F_U, smooth_field((n, n + 1), rng))
+    g.set(fs2d.F_V, smooth_field((n + 1, n), rng))
+    push(g, o, [fs2d.F_U, fs2d.F_V])
+    g.run_stage(fs2d.STAGE_SL)
+    o.run_stage(O.ST_SL)
+    eu, ev = rel_l2(g.get(fs2d.F_U), o.get(O.F_U)), rel_l2(g.get(fs2d.F_V), o.get(O.F_V))
+    print("SL rel-L2 u %.3e v %.3e" % (eu, ev))
+    assert eu < REL and ev < REL
+
+
+def test_density_advection_stage():
+    n = 48
+    cells = O.all_fluid(n, n)
+    g, o = pair(cells, k=0, dt=0.01, dx=1.0 / n, mode=fs2d.SEMILAGRANGIAN, static_cells=True, density=True)
+    rng = np.random.default_rng(1)
+    g.set(fs2d.F_U, smooth_field((n, n + 1), rng))
+    g.set(fs2d.F_V, smooth_field((n + 1, n), rng))
+    g.set(fs2d.F_DENSITY, np.abs(smooth_field((n, n), rng)))
+    push(g, o, [fs2d.F_U, fs2d.F_V, fs2d.F_DENSITY])
+    g.run_stage(fs2d.STAGE_SL)
+    o.run_stage(O.ST_SL)
+    e = rel_l2(g.get(fs2d.F_DENSITY), o.get(O.F_DENS))
+    print("density rel-L2 %.3e" % e)
+    assert e < REL
+
+
+def test_gravity_stage():
+    n = 32
+    g, o = pair(O.dam_break(n, n), k=2, dt=0.01, dx=1.0 / n)
+    g.set_gravity(0.3, -9.81)
+    o.set_params(0.01, (0.3, -9.81), 1.0)
+    push(g, o, [fs2d.F_U, fs2d.F_V])
+    g.run_stage(fs2d.STAGE_GRAVITY)
+    o.run_stage(O.ST_GRAVITY)
+    assert rel_l2(g.get(fs2d.F_U), o.get(O.F_U)) < 1e-7
+    assert rel_l2(g.get(fs2d.F_V), o.get(O.F_V)) < 1e-7
+
+
+def projection_case(n, mode, steps):
+    cells = O.dam_break(n, n)
+    g, o = pair(cells, k=2, dt=0.005, dx=1.0 / n, mode=mode, max_iters=1000)
+    for _ in range(steps):
+        g.step()
+    g.run_stage(fs2d.STAGE_LEVEL_SET)
+    if mode == fs2d.PICFLIP:
+        g.run_stage(fs2d.STAGE_P2G)
+    else:
+        g.run_stage(fs2d.STAGE_SL)
+    g.run_stage(fs2d.STAGE_GRAVITY)
+    push(g, o, [fs2d.F_CELL, fs2d.F_PHI, fs2d.F_U, fs2d.F_V])
+    return g, o
+
+
+@pytest.mark.parametrize("mode", [fs2d.SEMILAGRANGIAN, fs2d.PICFLIP])
+def test_projection_matched_iterations(mode):
+    """Same algorithm, same iteration count: FP32 GPU vs FP64 oracle."""
+    g, o = projection_case(64, mode, 4)
+    for iters in (1, 5, 20):
+        g.set_solver(tau=0.999, tol=0.0, max_iters=iters)
+        o.set_options(tau=0.999, tol=0.0, max_iters=iters, sl_inplace=False)
+        g.run_stage(fs2d.STAGE_PROJECT, allow=(fs2d.OK, fs2d.WARN_MAXITER))
+        o.run_stage(O.ST_PROJECT)
+        assert g.stats()["pcg_iters"] == o.stats()["last_iters"] == iters
+        e = rel_l2(g.get(fs2d.F_P), o.get(O.F_P))
+        print("iters %d p rel-L2 %.3e" % (iters, e))
+        assert e < REL
+
+
+@pytest.mark.parametrize("mode", [fs2d.SEMILAGRANGIAN, fs2d.PICFLIP])
+def test_projection_converged_and_velocity_update(mode):
+    """GPU to tol 1e-6 vs the oracle to 1e-12; then updateVelocity."""
+    g, o = projection_case(64, mode, 4)
+    g.set_solver(tau=0.97, tol=1e-6, max_iters=500)
+    o.set_options(tau=0.97, tol=1e-12, max_iters=2000, sl_inplace=False)
+    g.run_stage(fs2d.STAGE_PROJECT)
+    o.run_stage(O.ST_PROJECT)
+    sg = g.stats()
+    ep = rel_l2(g.get(fs2d.F_P), o.get(O.F_P))
+    print("converged: gpu iters %d (res %.2e), oracle iters %d, p rel-L2 %.3e"
+          % (sg["pcg_iters"], sg["pcg_residual"], o.stats()["last_iters"], ep))
+    assert sg["pcg_residual"] <= 1e-6
+    assert ep < 2e-4
+    # velocity update from the same pressure
+    push(g, o, [fs2d.F_P])
+    g.run_stage(fs2d.STAGE_UPDATE_VELOCITY)
+    o.run_stage(O.ST_UPDATEVEL)
+    eu = rel_l2(g.get(fs2d.F_NEWU), o.get(O.F_NEWU))
+    ev = rel_l2(g.get(fs2d.F_NEWV), o.get(O.F_NEWV))
+    print("update velocity rel-L2 u %.3e v %.3e" % (eu, ev))
+    assert eu < REL and ev < REL
+
+
+def test_mic0_preconditioner_matches():
+    """One preconditioner application (0 CG updates) equals the oracle's."""
+    g, o = projection_case(48, fs2d.SEMILAGRANGIAN, 3)
+    # max_iters=1, tol=0: p = alpha * M^-1 rhs
+    g.set_solver(tau=0.999, tol=0.0, max_iters=1)
+    o.set_options(tau=0.999, tol=0.0, max_iters=1, sl_inplace=False)
+    g.run_stage(fs2d.STAGE_PROJECT, allow=(fs2d.OK, fs2d.WARN_MAXITER))
+    o.run_stage(O.ST_PROJECT)
+    e = rel_l2(g.get(fs2d.F_P), o.get(O.F_P))
+    print("1-iteration p rel-L2 %.3e" % e)
+    assert e < REL
+
+
+def test_p2g_stage():
+    n = 64
+    cells = O.dam_break(n, n)
+    g, o = pair(cells, k=2, dt=0.005, dx=1.0 / n, mode=fs2d.PICFLIP)
+    for _ in range(3):
+        g.step()
+    rng = np.random.default_rng(3)
+    m = g.stats()["n_particles"]
+    g.set(fs2d.F_PARTICLE_VEL, rng.standard_normal((m, 2)).astype(np.float32))
+    push(g, o, [fs2d.F_PARTICLE_POS, fs2d.F_PARTICLE_VEL])
+    g.run_stage(fs2d.STAGE_P2G)
+    o.run_stage(O.ST_P2G)
+    eu, ev = rel_l2(g.get(fs2d.F_U), o.get(O.F_U)), rel_l2(g.get(fs2d.F_V), o.get(O.F_V))
+    print("P2G rel-L2 u %.3e v %.3e" % (eu, ev))
+    assert eu < REL and ev < REL
+
+
+def test_g2p_and_particle_advection():
+    n = 64
+    cells = O.dam_break(n, n)
+    g, o = pair(cells, k=2, dt=0.005, dx=1.0 / n, mode=fs2d.PICFLIP, alpha=0.05)
+    for _ in range(3):
+        g.step()
+    g.run_stage(fs2d.STAGE_LEVEL_SET)
+    g.run_stage(fs2d.STAGE_P2G)
+    g.run_stage(fs2d.STAGE_GRAVITY)
+    g.run_stage(fs2d.STAGE_PROJECT)
+    g.run_stage(fs2d.STAGE_UPDATE_VELOCITY)
+    push(g, o, [fs2d.F_PARTICLE_POS, fs2d.F_PARTICLE_VEL, fs2d.F_U, fs2d.F_V, fs2d.F_NEWU, fs2d.F_NEWV])
+    g.run_stage(fs2d.STAGE_G2P)
+    o.run_stage(O.ST_G2P)
+    e = rel_l2(g.get(fs2d.F_PARTICLE_VEL), o.get(O.F_VEL))
+    print("G2P rel-L2 %.3e" % e)
+    assert e < REL
+    push(g, o, [fs2d.F_PARTICLE_VEL, fs2d.F_U, fs2d.F_V])
+    g.run_stage(fs2d.STAGE_ADVECT)
+    o.run_stage(O.ST_ADVECT)
+    e = rel_l2(g.get(fs2d.F_PARTICLE_POS), o.get(O.F_POS))
+    print("advect rel-L2 %.3e" % e)
+    assert e < REL
+    # particle cell indices after the move are those of the oracle's positions
+    pg = g.get(fs2d.F_PARTICLE_POS).astype(np.float64)
+    kg = (pg / (1.0 / n)).astype(int)
+    ko = (o.get(O.F_POS) / (1.0 / n)).astype(int)
+    assert np.mean(np.all(kg == ko, axis=1)) > 0.999
+
+
+@pytest.mark.parametrize("mode", [fs2d.SEMILAGRANGIAN, fs2d.PICFLIP])
+def test_teacher_forced_steps(mode):
+    """Whole steps: the oracle restarts from the GPU's state every step."""
+    n = 64
+    cells = O.dam_break(n, n)
+    g, o = pair(cells, k=2, dt=0.005, dx=1.0 / n, mode=mode, alpha=0.05, tau=0.97,
+                tol=1e-6, max_iters=500)
+    o.set_options(tau=0.97, tol=1e-12, max_iters=2000, sl_inplace=False)
+    for step in range(6):
+        push(g, o, [fs2d.F_PARTICLE_POS, fs2d.F_PARTICLE_VEL, fs2d.F_U, fs2d.F_V, fs2d.F_CELL])
+        g.step()
+        o.step()
+        np.testing.assert_array_equal(g.get(fs2d.F_CELL), o.get(O.F_CELL))
+        eu, ev = rel_l2(g.get(fs2d.F_U), o.get(O.F_U)), rel_l2(g.get(fs2d.F_V), o.get(O.F_V))
+        ex = rel_l2(g.get(fs2d.F_PARTICLE_POS), o.get(O.F_POS))
+        ep = rel_l2(g.get(fs2d.F_P), o.get(O.F_P))
+        print("step %d: u %.2e v %.2e pos %.2e p %.2e" % (step, eu, ev, ex, ep))
+        assert ex < REL
+        assert eu < 1e-4 and ev < 1e-4 and ep < 1e-4
+
+
+def test_golden_semilag_rows_on_gpu():
+    """The GPU step reproduces the reference's conservation rows 1-2
+    (paper/data/conservation-semilag-state1.csv) within FP32 accuracy."""
+    import json, os
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "paper_conservation.json")))
+    ref = gold["files"]["conservation-semilag-state1.csv"]
+    g = fs2d.Sim(O.demo_triangle(128), k=2, dt=0.033, dx=0.02, mode=fs2d.SEMILAGRANGIAN)
+    got = []
+    for r in range(2):
+        g.run_stage(fs2d.STAGE_LEVEL_SET)
+        s = g.stats()
+        got.append([s["water_volume"], s["total_energy"], s["particle_total_energy"]])
+        if r == 0:
+            for st in (fs2d.STAGE_SL, fs2d.STAGE_GRAVITY, fs2d.STAGE_PROJECT,
+                       fs2d.STAGE_UPDATE_VELOCITY, fs2d.STAGE_ADVECT):
+                g.run_stage(st, allow=(fs2d.OK, fs2d.WARN_MAXITER))
+    print("gpu rows", got, "ref", ref[:2])
+    np.testing.assert_allclose(np.array(got)[:, 0], np.array(ref[:2])[:, 0], rtol=1e-9)
+    np.testing.assert_allclose(np.array(got)[:, 1:], np.array(ref[:2])[:, 1:], rtol=1e-6)
+
+
+def test_all_fluid_box_divergence_free_2048():
+    """Size-independent property at the benchmark size: after the projection
+    the discrete divergence of every FLUID cell is at the solver tolerance."""
+    n = 2048
+    cells = O.all_fluid(n, n)
+    g = fs2d.Sim(cells, k=0, dt=1e-3, dx=1.0 / n, mode=fs2d.SEMILAGRANGIAN, static_cells=True,
+                 gravity=(0.0, 0.0), tau=0.97, tol=1e-6, max_iters=2000)
+    rng = np.random.default_rng(7)
+    u = rng.standard_normal((n, n + 1)).astype(np.float32)
+    v = rng.standard_normal((n + 1, n)).astype(np.float32)
+    g.set(fs2d.F_U, u)
+    g.set(fs2d.F_V, v)
+    g.run_stage(fs2d.STAGE_PROJECT)
+    s = g.stats()
+    g.run_stage(fs2d.STAGE_UPDATE_VELOCITY)
+    u2, v2 = g.get(fs2d.F_U).astype(np.float64), g.get(fs2d.F_V).astype(np.float64)
+    div = (u2[:, 1:] - u2[:, :-1]) + (v2[1:, :] - v2[:-1, :])
+    div0 = (u[:, 1:].astype(np.float64) - u[:, :-1]) + (v[1:, :].astype(np.float64) - v[:-1, :])
+    fl = cells == O.FLUID
+    print("2048^2: iters %d residual %.2e  max|div| %.3e (before %.3e)"
+          % (s["pcg_iters"], s["pcg_residual"], np.abs(div[fl]).max(), np.abs(div0[fl]).max()))
+    assert s["pcg_residual"] <= 1e-6
+    assert np.abs(div[fl]).max() < 1e-4 * np.abs(div0[fl]).max()
+
+
+def test_error_paths():
+    n = 32
+    g = fs2d.Sim(O.dam_break(n, n), k=2, dt=0.005, dx=1.0 / n, mode=fs2d.SEMILAGRANGIAN)
+    # PCG cap: a warning status, not an abort
+    g.set_solver(tol=0.0, max_iters=1)
+    g.run_stage(fs2d.STAGE_LEVEL_SET)
+    g.run_stage(fs2d.STAGE_GRAVITY)
+    assert g.run_stage(fs2d.STAGE_PROJECT, allow=(fs2d.WARN_MAXITER,)) == fs2d.WARN_MAXITER
+    # particles outside the interior box are rejected
+    with pytest.raises(fs2d.FS2DError) as e:
+        g.set_particles(np.array([[0.5 / n, 0.5]]))
+    assert e.value.status == fs2d.ERR_ARG
+    with pytest.raises(fs2d.FS2DError):
+        g.set(fs2d.F_U, np.zeros(5, np.float32))
+    # empty particle set: the level set marks nothing FLUID and rhs = 0 (p = 0)
+    g.set_particles(np.zeros((0, 2)))
+    g.set_solver()
+    g.step()
+    cell = g.get(fs2d.F_CELL)
+    assert not (cell == fs2d.FLUID).any()
+    assert not g.get(fs2d.F_P).any()
