@@ -61,9 +61,17 @@ struct fs2d_sim {
     uint8_t* cell = nullptr;
     double *phi = nullptr, *phi2 = nullptr;
     int *t0 = nullptr, *t1 = nullptr;
-    // projection
-    float *p = nullptr, *r = nullptr, *z = nullptr, *q = nullptr, *w = nullptr, *s[2] = {nullptr, nullptr};
-    float *Adiag = nullptr, *Ax = nullptr, *Ay = nullptr, *P = nullptr, *ex = nullptr, *ey = nullptr;
+    // projection: row-major pressure + SK4 solver arrays (fs2d_skew.hpp)
+    float* p = nullptr;
+    SkGeom geom{};
+    float *r = nullptr, *z = nullptr, *q = nullptr, *w = nullptr, *s[2] = {nullptr, nullptr}, *pk = nullptr;
+    float *Adiag = nullptr, *P = nullptr, *fx = nullptr, *fy = nullptr, *gx_ = nullptr, *gy_ = nullptr;
+    uint8_t* cpl = nullptr;  // coupling bits (CPL_*)
+    PcgScalars* sc = nullptr;
+    PcgScalars* h_sc = nullptr;
+    double *qq = nullptr, *ws_part = nullptr, *rmax_part = nullptr;
+    unsigned long long* sk_ticket = nullptr;
+    unsigned long long* sk_handoff = nullptr;  // separate from the level-set engine's: epochs differ
     // extrapolation
     uint32_t *mu = nullptr, *mv = nullptr;
     int *fu0 = nullptr, *fu1 = nullptr, *fv0 = nullptr, *fv1 = nullptr, *fcount = nullptr;
@@ -74,14 +82,11 @@ struct fs2d_sim {
     int* tile_sums = nullptr;
     bool sorted_valid = false;
     // scalars
-    PcgState* st = nullptr;
     Diag* dg = nullptr;
     double* partials = nullptr;
-    unsigned* counters = nullptr;
     unsigned long long* handoff = nullptr;
     unsigned* wctl = nullptr;
     int* werr = nullptr;
-    PcgState* h_st = nullptr;
     Diag* h_dg = nullptr;
     int* h_err = nullptr;
 
@@ -276,9 +281,10 @@ void diagnostics(fs2d_sim* s) {
     Timed t(s, K_DIAGNOSTICS);
     const long n = n_c(s) + s->np;
     const double pmass = s->k > 0 ? s->rho * s->dx * s->dx / (s->k * s->k) : 0.0;
-    hipLaunchKernelGGL(diagnostics_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s->stream, s->cell, mac_view(s),
-                       s->px, s->py, s->pvx, s->pvy, s->np, s->rho, s->gx, s->gy, pmass, s->dg, s->partials,
-                       s->counters + 3);
+    const int nb = blocks_for(n);
+    hipLaunchKernelGGL(diagnostics_kernel, dim3(nb), dim3(kBlock), 0, s->stream, s->cell, mac_view(s), s->px,
+                       s->py, s->pvx, s->pvy, s->np, s->rho, s->gx, s->gy, pmass, s->partials);
+    hipLaunchKernelGGL(diag_finalize_kernel, dim3(1), dim3(kBlock), 0, s->stream, s->partials, nb, s->dx, s->dg);
 }
 
 // FluidSim2D::createWaterLevelSet (src/FluidSim2D.cpp:653-732).
@@ -363,70 +369,92 @@ void stage_p2g(fs2d_sim* s) {
     extrapolate_uv(s, s->u, s->v);
 }
 
-// applyProjection (src/FluidSim2D.cpp:252-467).
+// applyProjection (src/FluidSim2D.cpp:252-467).  Per iteration: SpMV with the
+// s update, the p/r update, forward and backward solves; scalars travel in
+// device memory, and the host reads the stopping flag once per batch.
+SkCtl sk_ctl(fs2d_sim* s) { return SkCtl{s->sk_ticket, s->sk_handoff, s->werr}; }
+
+template <class Op, bool B>
+void launch_sweep(fs2d_sim* s, const Op& op) {
+    hipLaunchKernelGGL((sweep_kernel<Op, B>), dim3(s->geom.S), dim3(kSweepThreads), 0, s->stream, op, s->geom,
+                       sk_ctl(s));
+}
+
 fs2d_status stage_projection(fs2d_sim* s) {
-    const long nc = n_c(s);
-    PcgState init{};
+    const SkGeom& g = s->geom;
+    const long ns = g.size();
+    const int nb = blocks_for(ns);
+    const double sA = s->dt / (s->rho * s->dx * s->dx);
+    PcgScalars init{};
     init.tol = s->tol;
-    init.max_iters = s->max_iters;
-    HIPCHK(s, hipMemcpyAsync(s->st, &init, sizeof init, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(s, hipMemcpyAsync(s->sc, &init, sizeof init, hipMemcpyHostToDevice, s->stream));
     {
         Timed t(s, K_ASSEMBLE);
-        hipLaunchKernelGGL(assemble_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->cell, s->phi, s->u,
-                           s->v, s->Adiag, s->Ax, s->Ay, s->r, s->p, s->s[0], s->nx, s->ny,
-                           s->dt / (s->rho * s->dx * s->dx), 1.0 / s->dx, s->st, s->partials, s->counters + 0);
+        hipLaunchKernelGGL(assemble_sk_kernel, dim3(nb), dim3(kBlock), 0, s->stream, g, s->cell, s->phi, s->u,
+                           s->v, s->Adiag, s->cpl, s->r, s->pk, s->s[0], sA, 1.0 / s->dx, s->rmax_part);
     }
     {
         Timed t(s, K_FACTOR);
-        FactorOp op{s->cell, s->Adiag, s->Ax, s->Ay, s->P, s->ex, s->ey, s->tau, s->sigma, s->nx};
-        launch_wavefront(s, op);
+        FactorOp op{{s->Adiag}, s->cpl, {s->P, s->fx, s->fy, s->gx_, s->gy_}, s->tau, s->sigma, sA, s->rmax_part, nb,
+                    s->max_iters, s->sc};
+        launch_sweep<FactorOp, false>(s, op);
     }
-    FwdOp fwd{s->r, s->P, s->ex, s->ey, s->q, s->st, s->nx};
-    BwdOp bwd{s->q, s->P, s->ex, s->ey, s->r, s->z, s->st, s->partials, s->nx, s->ny, 1};
+    double* qq0 = s->qq;
+    double* qq1 = s->qq + g.S;
     {
         Timed t(s, K_PCG_FWD);
-        launch_wavefront(s, fwd);
+        FwdOp op{{s->r, s->P, s->fx, s->fy}, nullptr, {s->q}, s->sc, s->rmax_part, qq0, nb, 0};
+        launch_sweep<FwdOp, false>(s, op);
     }
     {
         Timed t(s, K_PCG_BWD);
-        launch_wavefront(s, bwd);
+        BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, 0};
+        launch_sweep<BwdOp, true>(s, op);
     }
-    bwd.init_mode = 0;
     int h = 0;
     int batch = std::max(2, std::min(s->last_iters_hint, s->max_iters));
     while (h < s->max_iters) {
-        const int nb = std::min(batch, s->max_iters - h);
-        for (int k = 0; k < nb; k++, h++) {
+        const int nbatch = std::min(batch, s->max_iters - h);
+        for (int k = 0; k < nbatch; k++, h++) {
             float* s_old = s->s[h & 1];
             float* s_new = s->s[(h + 1) & 1];
+            double* sig_cur = (h & 1) ? qq1 : qq0;   // sigma_h
+            double* sig_prev = (h & 1) ? qq0 : qq1;  // sigma_{h-1}; receives sigma_{h+1}
             {
                 Timed t(s, K_PCG_SPMV);
-                hipLaunchKernelGGL(pcg_spmv_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->cell,
-                                   s->Adiag, s->Ax, s->Ay, s->z, s_old, s_new, s->w, s->nx, s->ny, s->st,
-                                   s->partials, s->counters + 1);
+                hipLaunchKernelGGL(pcg_spmv_sk_kernel, dim3(nb), dim3(kBlock), 0, s->stream, g, s->Adiag, s->cpl,
+                                   s->z, s_old, s_new, s->w, (float)sA, s->sc, sig_cur, sig_prev, h == 0 ? 1 : 0,
+                                   s->ws_part);
             }
             {
                 Timed t(s, K_PCG_AXPY);
-                hipLaunchKernelGGL(pcg_axpy_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->p, s->r,
-                                   s_new, s->w, nc, s->st, s->partials, s->counters + 2);
+                hipLaunchKernelGGL(pcg_axpy_sk_kernel, dim3(nb), dim3(kBlock), 0, s->stream, g, s->pk, s->r,
+                                   s_new, s->w, s->sc, sig_cur, s->ws_part, nb, s->rmax_part);
             }
             {
                 Timed t(s, K_PCG_FWD);
-                launch_wavefront(s, fwd);
+                FwdOp op{{s->r, s->P, s->fx, s->fy}, nullptr, {s->q}, s->sc, s->rmax_part, sig_prev, nb, 1};
+                launch_sweep<FwdOp, false>(s, op);
             }
             {
                 Timed t(s, K_PCG_BWD);
-                launch_wavefront(s, bwd);
+                BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, h + 1};
+                launch_sweep<BwdOp, true>(s, op);
             }
         }
-        HIPCHK(s, hipMemcpyAsync(s->h_st, s->st, sizeof(PcgState), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipMemcpyAsync(s->h_sc, s->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s->stream));
         HIPCHK(s, hipStreamSynchronize(s->stream));
-        if (s->h_st->done) break;
+        if (s->h_sc->done) break;
         batch = 4;
     }
-    TRY(d2h(s, s->h_st, s->st, sizeof(PcgState)));
-    s->last_iters_hint = std::max(2, s->h_st->iter + 1);
-    return s->h_st->hit_max ? FS2D_WARN_MAXITER : FS2D_OK;
+    {
+        Timed t(s, K_MISC);
+        hipLaunchKernelGGL(sk_to_rowmajor_kernel, dim3(blocks_for(n_c(s))), dim3(kBlock), 0, s->stream, g, s->pk,
+                           s->p);
+    }
+    TRY(d2h(s, s->h_sc, s->sc, sizeof(PcgScalars)));
+    s->last_iters_hint = std::max(2, s->h_sc->iter + 1);
+    return s->h_sc->iter >= s->max_iters ? FS2D_WARN_MAXITER : FS2D_OK;
 }
 
 // updateVelocity (src/FluidSim2D.cpp:469-550).
@@ -461,9 +489,10 @@ void stage_g2p(fs2d_sim* s) {
 void stage_particle_advect(fs2d_sim* s) {
     if (s->np == 0) return;
     Timed t(s, K_PARTICLE_ADVECT);
-    hipLaunchKernelGGL(particle_advect_kernel, dim3(blocks_for(s->np)), dim3(kBlock), 0, s->stream, s->px, s->py,
-                       s->pvx, s->pvy, s->np, mac_view(s), s->dt, clamp_box(s->nx, s->ny, s->dx), s->dg, s->partials,
-                       s->counters + 4);
+    const int nb = blocks_for(s->np);
+    hipLaunchKernelGGL(particle_advect_kernel, dim3(nb), dim3(kBlock), 0, s->stream, s->px, s->py, s->pvx, s->pvy,
+                       s->np, mac_view(s), s->dt, clamp_box(s->nx, s->ny, s->dx), s->dg, s->partials);
+    hipLaunchKernelGGL(cfl_finalize_kernel, dim3(1), dim3(kBlock), 0, s->stream, s->partials, nb, s->dg);
     s->sorted_valid = false;
 }
 
@@ -630,23 +659,28 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
     if ((st = dalloc(s, &s->p, (n))) != FS2D_OK) return bail(st)
     A(u, nU); A(v, nV); A(u2, nU); A(v2, nV); A(nu, nU); A(nv, nV);
     A(dens, nC); A(dens2, nC); A(cell, nC); A(phi, nC); A(phi2, nC); A(t0, nC); A(t1, nC);
-    A(p, nC); A(r, nC); A(z, nC); A(q, nC); A(w, nC); A(s[0], nC); A(s[1], nC);
-    A(Adiag, nC); A(Ax, nC); A(Ay, nC); A(P, nC); A(ex, nC); A(ey, nC);
+    s->geom = make_geom(s->nx, s->ny);
+    const size_t nS = (size_t)s->geom.size();
+    A(p, nC);
+    A(r, nS); A(z, nS); A(q, nS); A(w, nS); A(s[0], nS); A(s[1], nS); A(pk, nS);
+    A(Adiag, nS); A(P, nS); A(fx, nS); A(fy, nS); A(gx_, nS); A(gy_, nS); A(cpl, nS);
+    A(sc, 1); A(qq, 2 * (size_t)s->geom.S); A(ws_part, kMaxBlocks); A(rmax_part, kMaxBlocks); A(sk_ticket, 1);
     A(mu, nU); A(mv, nV); A(fu0, nU); A(fu1, nU); A(fv0, nV); A(fv1, nV); A(fcount, 2);
     A(counts, nC); A(starts, nC); A(cursor, nC); A(tile_sums, (nC + 1023) / 1024 + 1);
-    A(st, 1); A(dg, 1); A(partials, 3 * kMaxBlocks); A(counters, 8);
+    A(dg, 1); A(partials, 3 * kMaxBlocks);
     A(wctl, 4); A(werr, 1);
     const int nstrips = (s->ny + 63) / 64;
     A(handoff, (size_t)nstrips * s->nx * 4);
+    A(sk_handoff, (size_t)nstrips * s->nx * 3);
 #undef A
-    e = hipHostMalloc((void**)&s->h_st, sizeof(PcgState), hipHostMallocDefault);
+    e = hipHostMalloc((void**)&s->h_sc, sizeof(PcgScalars), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_dg, sizeof(Diag), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_err, sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) {
         s->err = std::string("hipHostMalloc: ") + hipGetErrorString(e);
         return bail(FS2D_ERR_OOM);
     }
-    memset(s->h_st, 0, sizeof(PcgState));
+    memset(s->h_sc, 0, sizeof(PcgScalars));
     memset(s->h_dg, 0, sizeof(Diag));
     *s->h_err = 0;
     const unsigned ctl0[4] = {0u, 0u, 1u, 0u};
@@ -722,7 +756,7 @@ void fs2d_destroy(fs2d_sim* s) {
         (void)hipEventDestroy(t.b);
     }
     for (auto e : s->event_pool) (void)hipEventDestroy(e);
-    if (s->h_st) (void)hipHostFree(s->h_st);
+    if (s->h_sc) (void)hipHostFree(s->h_sc);
     if (s->h_dg) (void)hipHostFree(s->h_dg);
     if (s->h_err) (void)hipHostFree(s->h_err);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -959,7 +993,7 @@ fs2d_status fs2d_set_solver(fs2d_sim* s, double tau, double sigma, double tol, i
 fs2d_status fs2d_get_stats(fs2d_sim* s, fs2d_stats* out) {
     if (!s || !out) return FS2D_ERR_ARG;
     HIPCHK(s, hipMemcpyAsync(s->h_dg, s->dg, sizeof(Diag), hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(s, hipMemcpyAsync(s->h_st, s->st, sizeof(PcgState), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipMemcpyAsync(s->h_sc, s->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
     memset(out, 0, sizeof *out);
     out->water_volume = s->h_dg->volume;
@@ -967,10 +1001,10 @@ fs2d_status fs2d_get_stats(fs2d_sim* s, fs2d_stats* out) {
     out->particle_total_energy = s->h_dg->particle_energy;
     out->orig_water_volume = s->orig_volume;
     out->current_time = s->current_time;
-    out->pcg_residual = s->h_st->rhs_norm > 0 ? s->h_st->r_norm / s->h_st->rhs_norm : 0.0;
+    out->pcg_residual = s->h_sc->rhs_norm > 0 ? s->h_sc->r_norm / s->h_sc->rhs_norm : 0.0;
     out->cfl_max = s->h_dg->cfl_max;
-    out->pcg_iters = s->h_st->iter;
-    out->pcg_hit_max = s->h_st->hit_max;
+    out->pcg_iters = s->h_sc->iter;
+    out->pcg_hit_max = s->h_sc->iter >= s->max_iters;
     out->n_particles = s->np;
     out->steps = s->steps;
     return FS2D_OK;

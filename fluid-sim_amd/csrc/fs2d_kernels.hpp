@@ -2,12 +2,15 @@
 //
 // Layout in HBM (all row-major, element (i, j) at [j * width + i]):
 //   u, u2, newu   FP32 (nx+1) x ny        v, v2, newv   FP32 nx x (ny+1)
-//   p, r, z, q, w, s[2], Adiag, Ax, Ay, P, ex, ey       FP32 nx x ny
+//   p             FP32 nx x ny (row-major copy of the solver's pressure)
+//   PCG vectors and coefficients (r, z, q, w, s[2], p, Adiag, Ax, Ay, P, fx,
+//   fy, gx, gy) FP32 in the strip-skewed SK4 layout of fs2d_skew.hpp
 //   cell          uint8 nx x ny           phi[2]        FP64 nx x ny
 //   tidx[2]       int32 nx x ny (closest particle of the level-set sweeps)
 //   px, py, pvx, pvy  FP32 SoA, particle order as created
-// Reductions are deterministic: each block writes a partial, the last block to
-// finish combines the partials in index order.
+// Reductions are deterministic and fence-free: each block writes a partial and
+// a later kernel (kernel boundaries order the writes) combines the partials in
+// index order.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,15 +19,11 @@
 
 #include "fs2d_device.hpp"
 #include "fs2d_wavefront.hpp"
+#include "fs2d_sweep.hpp"
 
 namespace fs2d {
 
 constexpr int kBlock = 256;
-
-struct PcgState {
-    double sigma, alpha, beta, rhs_norm, r_norm, tol;
-    int iter, done, hit_max, max_iters;
-};
 
 struct Diag {  // createWaterLevelSet diagnostics (src/FluidSim2D.cpp:709-731)
     double volume, grid_energy, particle_energy, cfl_max;
@@ -72,17 +71,7 @@ __device__ __forceinline__ double block_max(double v) {
         for (int k = 0; k < kBlock / 64; k++) t = sh[k] > t ? sh[k] : t;
     return t;
 }
-// Called by all threads after thread 0 wrote this block's partial.
-__device__ __forceinline__ bool last_block(unsigned* counter) {
-    __shared__ bool am_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) am_last = (atomicAdd(counter, 1u) == gridDim.x - 1);
-    __syncthreads();
-    if (am_last) __threadfence();
-    return am_last;
-}
-// Sum of partials[0..n) in a fixed order by the (last) block; result in thread 0.
+// Sum / max of partials[0..n) by a whole block in a fixed order; result in thread 0.
 __device__ __forceinline__ double sum_partials(const double* partials, int n) {
     double acc = 0;
     for (int k = threadIdx.x; k < n; k += kBlock) acc += partials[k];
@@ -151,41 +140,57 @@ __global__ void gravity_kernel(float* __restrict__ u, float* __restrict__ v, lon
 }
 
 // ---------------------------------------------------------------------------
-// Projection (src/FluidSim2D.cpp:252-467).
+// Projection (src/FluidSim2D.cpp:252-467) on the SK4 layout (fs2d_skew.hpp).
 // ---------------------------------------------------------------------------
-// Ghost-fluid matrix (:262-303) and rhs (:338-362); p = 0, s = 0; |rhs|_inf.
-__global__ __launch_bounds__(kBlock) void assemble_kernel(
-    const uint8_t* __restrict__ cell, const double* __restrict__ phi, const float* __restrict__ u,
-    const float* __restrict__ v, float* __restrict__ Adiag, float* __restrict__ Ax, float* __restrict__ Ay,
-    float* __restrict__ r, float* __restrict__ p, float* __restrict__ s0, int nx, int ny, double scaleA,
-    double scale, PcgState* st, double* partials, unsigned* counter) {
-    const long n = (long)nx * ny;
+// Decode an SK4 element index into its cell; false for padding entries.
+__device__ __forceinline__ bool sk_cell(const SkGeom& g, long e, int& i, int& j) {
+    const int e4 = (int)(e & 3), l = (int)((e >> 2) & 63);
+    const long r = e >> 8;
+    const int grp = (int)(r % g.T4), s = (int)(r / g.T4);
+    const int t = grp * 4 + e4;
+    i = t - l;
+    j = s * 64 + l;
+    return i >= 0 && i < g.nx && j < g.ny;
+}
+
+// Ghost-fluid matrix (:262-303) and rhs (:338-362) written in SK4 order: the
+// diagonal, and the coupling bits of each FLUID cell (its off-diagonals are
+// -scaleA towards FLUID neighbours).  The PCG vectors p and s start at zero
+// (:424).  Partial |rhs|_inf per block.
+__global__ __launch_bounds__(kBlock) void assemble_sk_kernel(
+    SkGeom g, const uint8_t* __restrict__ cell, const double* __restrict__ phi, const float* __restrict__ u,
+    const float* __restrict__ v, float* __restrict__ Adiag, uint8_t* __restrict__ flags, float* __restrict__ r,
+    float* __restrict__ p, float* __restrict__ s0, double scaleA, double scale, double* partials) {
+    const int nx = g.nx;
     double m = 0;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(e % nx), j = (int)(e / nx);
-        double ad = 0, ax = 0, ay = 0, rhs = 0;
-        if (cell[e] == FLUID) {
-            const uint8_t cl = cell[e - 1], cr = cell[e + 1], cb = cell[e - nx], ct = cell[e + nx];
-            const double ph = phi[e];
-            if (cl == FLUID) ad += scaleA;
-            else if (cl == EMPTY) ad -= scaleA * fmax(phi[e - 1] / ph, -1e3);
-            if (cr == FLUID) { ad += scaleA; ax = -scaleA; }
-            else if (cr == EMPTY) ad += scaleA * (1 - fmax(phi[e + 1] / ph, -1e3));
-            if (cb == FLUID) ad += scaleA;
-            else if (cb == EMPTY) ad -= scaleA * fmax(phi[e - nx] / ph, -1e3);
-            if (ct == FLUID) { ad += scaleA; ay = -scaleA; }
-            else if (ct == EMPTY) ad += scaleA * (1 - fmax(phi[e + nx] / ph, -1e3));
-            const long iu = (long)j * (nx + 1) + i;
-            const double u0 = u[iu], u1 = u[iu + 1], v0 = v[e], v1 = v[e + nx];
-            rhs = -scale * (u1 - u0 + v1 - v0);
-            if (cl == SOLID) rhs -= scale * u0;
-            if (cr == SOLID) rhs += scale * u1;
-            if (cb == SOLID) rhs -= scale * v0;
-            if (ct == SOLID) rhs += scale * v1;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < g.size(); e += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        double ad = 0, rhs = 0;
+        uint32_t f = 0;
+        if (sk_cell(g, e, i, j)) {
+            const long c = (long)j * nx + i;
+            if (cell[c] == FLUID) {
+                const uint8_t cl = cell[c - 1], cr = cell[c + 1], cb = cell[c - nx], ct = cell[c + nx];
+                const double ph = phi[c];
+                if (cl == FLUID) { ad += scaleA; f |= CPL_L; }
+                else if (cl == EMPTY) ad -= scaleA * fmax(phi[c - 1] / ph, -1e3);
+                if (cr == FLUID) { ad += scaleA; f |= CPL_R; }
+                else if (cr == EMPTY) ad += scaleA * (1 - fmax(phi[c + 1] / ph, -1e3));
+                if (cb == FLUID) { ad += scaleA; f |= CPL_D; }
+                else if (cb == EMPTY) ad -= scaleA * fmax(phi[c - nx] / ph, -1e3);
+                if (ct == FLUID) { ad += scaleA; f |= CPL_U; }
+                else if (ct == EMPTY) ad += scaleA * (1 - fmax(phi[c + nx] / ph, -1e3));
+                const long iu = (long)j * (nx + 1) + i;
+                const double u0 = u[iu], u1 = u[iu + 1], v0 = v[c], v1 = v[c + nx];
+                rhs = -scale * (u1 - u0 + v1 - v0);
+                if (cl == SOLID) rhs -= scale * u0;
+                if (cr == SOLID) rhs += scale * u1;
+                if (cb == SOLID) rhs -= scale * v0;
+                if (ct == SOLID) rhs += scale * v1;
+            }
         }
         Adiag[e] = (float)ad;
-        Ax[e] = (float)ax;
-        Ay[e] = (float)ay;
+        flags[e] = (uint8_t)f;
         r[e] = (float)rhs;
         p[e] = 0.0f;
         s0[e] = 0.0f;
@@ -194,186 +199,71 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(
     }
     double bm = block_max(m);
     if (threadIdx.x == 0) partials[blockIdx.x] = bm;
-    if (last_block(counter)) {
-        double rn = max_partials(partials, gridDim.x);
-        if (threadIdx.x == 0) {
-            st->rhs_norm = rn;
-            st->r_norm = rn;
-            st->iter = 0;
-            st->sigma = 0;
-            st->alpha = 0;
-            st->beta = 0;
-            st->hit_max = st->max_iters <= 0;
-            st->done = (rn <= 1e-12) || (st->max_iters <= 0);
-            *counter = 0;
-        }
-    }
 }
 
-// MIC(0) factor (:364-388): precon carried in FP64 along the wavefront;
-// stores P = precon and the scaled couplings ex = Ax*P, ey = Ay*P used by the
-// triangular solves.
-struct FactorOp {
-    static constexpr int NW = 2;
-    const uint8_t* __restrict__ cell;
-    const float* __restrict__ Adiag;
-    const float* __restrict__ Ax;
-    const float* __restrict__ Ay;
-    float* __restrict__ P;
-    float* __restrict__ ex;
-    float* __restrict__ ey;
-    double tau, sigma;
-    int nx;
-    __device__ bool skip() const { return false; }
-    __device__ void boundary(uint32_t* s) const { s[0] = s[1] = 0; }
-    __device__ void compute(int i, int j, const uint32_t* left, const uint32_t* below, uint32_t* out) {
-        const long e = (long)j * nx + i;
-        double Pc = 0;
-        if (i >= 1 && j >= 1 && cell[e] == FLUID) {
-            const double PL = u2d(left), PB = u2d(below);
-            const double axl = Ax[e - 1], ayl = Ay[e - 1], ayb = Ay[e - nx], axb = Ax[e - nx];
-            const double A0 = Adiag[e];
-            const double t1 = axl * PL, t2 = ayb * PB;
-            double ee = A0 - t1 * t1 - t2 * t2 - tau * (axl * ayl * PL * PL + ayb * axb * PB * PB);
-            if (ee < sigma * A0) ee = A0;
-            Pc = 1.0 / sqrt(ee);
-        }
-        P[e] = (float)Pc;
-        ex[e] = (float)((double)Ax[e] * Pc);
-        ey[e] = (float)((double)Ay[e] * Pc);
-        d2u(Pc, out);
+// Sum of a small partial array by one wave, broadcast to every thread of the
+// block (fixed order, identical in every block).
+__device__ __forceinline__ double block_sum_bcast(const double* a, int n) {
+    __shared__ double sh;
+    __syncthreads();  // earlier readers of sh are done
+    if (threadIdx.x < 64) {
+        double t = wave_sum_of(a, n);
+        if (threadIdx.x == 0) sh = t;
     }
-    __device__ void strip_done(int) {}
-    __device__ void last_done(int) {}
-};
+    __syncthreads();
+    return sh;
+}
 
-// Forward solve of applyPreconditioner (:400-409): q = Lt^-1 r with
-// Lt = diag(1/P) + lower(A) diag(P).  State carried: (q, ex, ey) of the cell.
-struct FwdOp {
-    static constexpr int NW = 3;
-    const float* __restrict__ r;
-    const float* __restrict__ P;
-    const float* __restrict__ ex;
-    const float* __restrict__ ey;
-    float* __restrict__ q;
-    const PcgState* st;
-    int nx;
-    __device__ bool skip() const { return st->done != 0; }
-    __device__ void boundary(uint32_t* s) const { s[0] = s[1] = s[2] = 0; }
-    __device__ void compute(int i, int j, const uint32_t* left, const uint32_t* below, uint32_t* out) {
-        const long e = (long)j * nx + i;
-        const float Pc = P[e];
-        float qv = 0.0f;
-        if (Pc != 0.0f && i >= 1 && j >= 1)
-            qv = Pc * (r[e] - u2f(left[1]) * u2f(left[0]) - u2f(below[2]) * u2f(below[0]));
-        q[e] = qv;
-        out[0] = f2u(qv);
-        out[1] = f2u(ex[e]);
-        out[2] = f2u(ey[e]);
-    }
-    __device__ void strip_done(int) {}
-    __device__ void last_done(int) {}
-};
-
-// Backward solve (:410-419): z = Lt^-T q, mirrored onto the canonical sweep
-// (a = nx-1-i, b = ny-1-j); accumulates z.r for sigma (:428, :457) and updates
-// beta / sigma / iter in the last workgroup.
-struct BwdOp {
-    static constexpr int NW = 1;
-    const float* __restrict__ q;
-    const float* __restrict__ P;
-    const float* __restrict__ ex;
-    const float* __restrict__ ey;
-    const float* __restrict__ r;
-    float* __restrict__ z;
-    PcgState* st;
-    double* partials;
-    int nx, ny, init_mode;
+// s' = z + beta s (:459, FMA as Array2D.h:185), w = A s' (:433-444), partial
+// w.s' (:450).  sigma_k = z.r comes from the forward solves' per-strip q.q
+// partials; beta = sigma_k / sigma_{k-1} (:457-458).  s is double-buffered
+// because every cell also needs s' of its four neighbours.
+__global__ __launch_bounds__(kBlock) void pcg_spmv_sk_kernel(
+    SkGeom g, const float* __restrict__ Adiag, const uint8_t* __restrict__ flags, const float* __restrict__ z,
+    const float* __restrict__ s_old, float* __restrict__ s_new, float* __restrict__ w, float sA,
+    PcgScalars* sc, const double* sig_cur, const double* sig_prev, int first, double* partials) {
+    if (sc->done) return;
+    const double sig = block_sum_bcast(sig_cur, g.S);
+    double beta_d = 0;
+    if (!first) beta_d = sig / block_sum_bcast(sig_prev, g.S);
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->sigma = sig;
+    const float beta = (float)beta_d;
     double acc = 0;
-    __device__ bool skip() const { return st->done != 0; }
-    __device__ void boundary(uint32_t* s) const { s[0] = 0; }
-    __device__ void compute(int a, int b, const uint32_t* left, const uint32_t* below, uint32_t* out) {
-        const int i = nx - 1 - a, j = ny - 1 - b;
-        const long e = (long)j * nx + i;
-        const float Pc = P[e];
-        float zv = 0.0f;
-        if (Pc != 0.0f && i <= nx - 2 && j <= ny - 2)
-            zv = Pc * (q[e] - ex[e] * u2f(left[0]) - ey[e] * u2f(below[0]));
-        z[e] = zv;
-        acc += (double)zv * (double)r[e];
-        out[0] = f2u(zv);
-    }
-    __device__ void strip_done(int s) {
-        double t = wave_sum(acc);
-        if (threadIdx.x == 0) partials[s] = t;
-    }
-    __device__ void last_done(int nstrips) {
-        double t = 0;
-        for (int k = threadIdx.x; k < nstrips; k += 64) t += partials[k];
-        t = wave_sum(t);
-        if (threadIdx.x == 0) {
-            if (init_mode) {
-                st->sigma = t;
-                st->beta = 0;
-            } else {
-                st->beta = t / st->sigma;
-                st->sigma = t;
-                st->iter += 1;
-                if (st->iter >= st->max_iters) {
-                    st->done = 1;
-                    st->hit_max = 1;
-                }
-            }
-        }
-    }
-};
-
-// s' = z + beta*s (:459, FMA as Array2D.h:185), w = A s' (:433-444) and the
-// partial of w.s' for alpha (:450).  s is double-buffered because every cell
-// also reads s' of its four neighbours.
-__global__ __launch_bounds__(kBlock) void pcg_spmv_kernel(
-    const uint8_t* __restrict__ cell, const float* __restrict__ Adiag, const float* __restrict__ Ax,
-    const float* __restrict__ Ay, const float* __restrict__ z, const float* __restrict__ s_old,
-    float* __restrict__ s_new, float* __restrict__ w, int nx, int ny, PcgState* st, double* partials,
-    unsigned* counter) {
-    if (st->done) return;
-    const float beta = (float)st->beta;
-    const long n = (long)nx * ny;
-    double acc = 0;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-        const float sc = fmaf(beta, s_old[e], z[e]);
-        s_new[e] = sc;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < g.size(); e += (long)gridDim.x * blockDim.x) {
+        const float sc_ = fmaf(beta, s_old[e], z[e]);
+        s_new[e] = sc_;
+        const float ad = Adiag[e];
         float wv = 0.0f;
-        if (cell[e] == FLUID) {
-            const float sl = fmaf(beta, s_old[e - 1], z[e - 1]);
-            const float sr = fmaf(beta, s_old[e + 1], z[e + 1]);
-            const float sb = fmaf(beta, s_old[e - nx], z[e - nx]);
-            const float stp = fmaf(beta, s_old[e + nx], z[e + nx]);
-            wv = Adiag[e] * sc + Ax[e - 1] * sl + Ax[e] * sr + Ay[e - nx] * sb + Ay[e] * stp;
+        if (ad != 0.0f) {  // FLUID: interior, all four neighbours exist
+            int i, j;
+            sk_cell(g, e, i, j);
+            const uint32_t f = flags[e];
+            float nsum = 0.0f;
+            if (f & CPL_L) { const long n = g.idx(i - 1, j); nsum += fmaf(beta, s_old[n], z[n]); }
+            if (f & CPL_R) { const long n = g.idx(i + 1, j); nsum += fmaf(beta, s_old[n], z[n]); }
+            if (f & CPL_D) { const long n = g.idx(i, j - 1); nsum += fmaf(beta, s_old[n], z[n]); }
+            if (f & CPL_U) { const long n = g.idx(i, j + 1); nsum += fmaf(beta, s_old[n], z[n]); }
+            wv = fmaf(-sA, nsum, ad * sc_);
         }
         w[e] = wv;
-        acc += (double)wv * (double)sc;
+        acc += (double)wv * (double)sc_;
     }
     double bs = block_sum(acc);
     if (threadIdx.x == 0) partials[blockIdx.x] = bs;
-    if (last_block(counter)) {
-        double t = sum_partials(partials, gridDim.x);
-        if (threadIdx.x == 0) {
-            st->alpha = st->sigma / t;
-            *counter = 0;
-        }
-    }
 }
 
-// p += alpha s, r -= alpha w (:451-452), |r|_inf and the stopping test (:453).
-__global__ __launch_bounds__(kBlock) void pcg_axpy_kernel(float* __restrict__ p, float* __restrict__ r,
-                                                          const float* __restrict__ s,
-                                                          const float* __restrict__ w, long n, PcgState* st,
-                                                          double* partials, unsigned* counter) {
-    if (st->done) return;
-    const float a = (float)st->alpha;
+// alpha = sigma / (w.s') (:450); p += alpha s', r -= alpha w (:451-452);
+// partial |r|_inf for the stopping test (:453), finished by the forward solve.
+__global__ __launch_bounds__(kBlock) void pcg_axpy_sk_kernel(
+    SkGeom g, float* __restrict__ p, float* __restrict__ r, const float* __restrict__ s,
+    const float* __restrict__ w, const PcgScalars* sc, const double* sig_cur, const double* ws_partials,
+    int n_ws, double* partials) {
+    if (sc->done) return;
+    const double sig = block_sum_bcast(sig_cur, g.S);
+    const double ws = block_sum_bcast(ws_partials, n_ws);
+    const float a = (float)(sig / ws);
     float m = 0.0f;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < g.size(); e += (long)gridDim.x * blockDim.x) {
         p[e] = fmaf(a, s[e], p[e]);
         const float rv = fmaf(-a, w[e], r[e]);
         r[e] = rv;
@@ -381,13 +271,15 @@ __global__ __launch_bounds__(kBlock) void pcg_axpy_kernel(float* __restrict__ p,
     }
     double bm = block_max((double)m);
     if (threadIdx.x == 0) partials[blockIdx.x] = bm;
-    if (last_block(counter)) {
-        double rn = max_partials(partials, gridDim.x);
-        if (threadIdx.x == 0) {
-            st->r_norm = rn;
-            if (rn <= st->tol * st->rhs_norm) st->done = 1;
-            *counter = 0;
-        }
+}
+
+// SK4 -> row-major copy of the pressure.
+__global__ __launch_bounds__(kBlock) void sk_to_rowmajor_kernel(SkGeom g, const float* __restrict__ sk,
+                                                               float* __restrict__ rm) {
+    const long n = (long)g.nx * g.ny;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % g.nx), j = (int)(c / g.nx);
+        rm[c] = sk[g.idx(i, j)];
     }
 }
 
@@ -717,8 +609,7 @@ __global__ __launch_bounds__(kBlock) void g2p_kernel(const float* __restrict__ p
 __global__ __launch_bounds__(kBlock) void particle_advect_kernel(float* __restrict__ px, float* __restrict__ py,
                                                                  const float* __restrict__ pvx,
                                                                  const float* __restrict__ pvy, long np, MacView m,
-                                                                 double dt, ClampBox cb, Diag* dg, double* partials,
-                                                                 unsigned* counter) {
+                                                                 double dt, ClampBox cb, Diag* dg, double* partials) {
     double cmax = 0;
     int nan = 0;
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < np; e += (long)gridDim.x * blockDim.x) {
@@ -749,13 +640,12 @@ __global__ __launch_bounds__(kBlock) void particle_advect_kernel(float* __restri
         partials[blockIdx.x] = bm;
         if (sh_nan) atomicOr(&dg->nan_flag, 1);
     }
-    if (last_block(counter)) {
-        double t = max_partials(partials, gridDim.x);
-        if (threadIdx.x == 0) {
-            dg->cfl_max = t;
-            *counter = 0;
-        }
-    }
+}
+
+// Max of the per-block CFL partials (one block).
+__global__ __launch_bounds__(kBlock) void cfl_finalize_kernel(const double* partials, int n, Diag* dg) {
+    double t = max_partials(partials, n);
+    if (threadIdx.x == 0) dg->cfl_max = t;
 }
 
 // ---------------------------------------------------------------------------
@@ -949,8 +839,8 @@ __global__ __launch_bounds__(kBlock) void diagnostics_kernel(const uint8_t* __re
                                                              const float* __restrict__ py,
                                                              const float* __restrict__ pvx,
                                                              const float* __restrict__ pvy, long np, double rho,
-                                                             double gx, double gy, double pmass, Diag* dg,
-                                                             double* partials, unsigned* counter) {
+                                                             double gx, double gy, double pmass,
+                                                             double* partials) {
     const long nc = (long)m.nx * m.ny;
     const double dx = m.dx, cm = rho * dx * dx;
     double vol = 0, eg = 0, ep = 0;
@@ -976,22 +866,23 @@ __global__ __launch_bounds__(kBlock) void diagnostics_kernel(const uint8_t* __re
         partials[3 * blockIdx.x + 1] = b;
         partials[3 * blockIdx.x + 2] = c2;
     }
-    if (last_block(counter)) {
-        double s0 = 0, s1 = 0, s2 = 0;
-        for (int k = threadIdx.x; k < (int)gridDim.x; k += kBlock) {
-            s0 += partials[3 * k];
-            s1 += partials[3 * k + 1];
-            s2 += partials[3 * k + 2];
-        }
-        s0 = block_sum(s0);
-        s1 = block_sum(s1);
-        s2 = block_sum(s2);
-        if (threadIdx.x == 0) {
-            dg->volume = s0 * dx * dx;
-            dg->grid_energy = s1;
-            dg->particle_energy = s2;
-            *counter = 0;
-        }
+}
+
+// Sums of the per-block diagnostics partials in index order (one block).
+__global__ __launch_bounds__(kBlock) void diag_finalize_kernel(const double* partials, int n, double dx, Diag* dg) {
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int k = threadIdx.x; k < n; k += kBlock) {
+        s0 += partials[3 * k];
+        s1 += partials[3 * k + 1];
+        s2 += partials[3 * k + 2];
+    }
+    s0 = block_sum(s0);
+    s1 = block_sum(s1);
+    s2 = block_sum(s2);
+    if (threadIdx.x == 0) {
+        dg->volume = s0 * dx * dx;
+        dg->grid_energy = s1;
+        dg->particle_energy = s2;
     }
 }
 
