@@ -400,15 +400,15 @@ fs2d_status stage_projection(fs2d_sim* s) {
         launch_sweep<FactorOp, false>(s, op);
     }
     double* qq0 = s->qq;
-    double* qq1 = s->qq + g.S;
+    double* qq1 = s->qq + (size_t)g.S * kLoaders;
     {
         Timed t(s, K_PCG_FWD);
-        FwdOp op{{s->r, s->P, s->fx, s->fy}, nullptr, {s->q}, s->sc, s->rmax_part, qq0, nb, 0};
+        FwdOp op{{s->r, s->P, s->fx, s->fy}, nullptr, {s->q}, s->sc, s->rmax_part, nb, 0};
         launch_sweep<FwdOp, false>(s, op);
     }
     {
         Timed t(s, K_PCG_BWD);
-        BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, 0};
+        BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, qq0, 0};
         launch_sweep<BwdOp, true>(s, op);
     }
     int h = 0;
@@ -433,12 +433,12 @@ fs2d_status stage_projection(fs2d_sim* s) {
             }
             {
                 Timed t(s, K_PCG_FWD);
-                FwdOp op{{s->r, s->P, s->fx, s->fy}, nullptr, {s->q}, s->sc, s->rmax_part, sig_prev, nb, 1};
+                FwdOp op{{s->r, s->P, s->fx, s->fy}, nullptr, {s->q}, s->sc, s->rmax_part, nb, 1};
                 launch_sweep<FwdOp, false>(s, op);
             }
             {
                 Timed t(s, K_PCG_BWD);
-                BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, h + 1};
+                BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, sig_prev, h + 1};
                 launch_sweep<BwdOp, true>(s, op);
             }
         }
@@ -664,7 +664,7 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
     A(p, nC);
     A(r, nS); A(z, nS); A(q, nS); A(w, nS); A(s[0], nS); A(s[1], nS); A(pk, nS);
     A(Adiag, nS); A(P, nS); A(fx, nS); A(fy, nS); A(gx_, nS); A(gy_, nS); A(cpl, nS);
-    A(sc, 1); A(qq, 2 * (size_t)s->geom.S); A(ws_part, kMaxBlocks); A(rmax_part, kMaxBlocks); A(sk_ticket, 1);
+    A(sc, 1); A(qq, 2 * (size_t)s->geom.S * kLoaders); A(ws_part, kMaxBlocks); A(rmax_part, kMaxBlocks); A(sk_ticket, 1);
     A(mu, nU); A(mv, nV); A(fu0, nU); A(fu1, nU); A(fv0, nV); A(fv1, nV); A(fcount, 2);
     A(counts, nC); A(starts, nC); A(cursor, nC); A(tile_sums, (nC + 1023) / 1024 + 1);
     A(dg, 1); A(partials, 3 * kMaxBlocks);

@@ -223,9 +223,9 @@ __global__ __launch_bounds__(kBlock) void pcg_spmv_sk_kernel(
     const float* __restrict__ s_old, float* __restrict__ s_new, float* __restrict__ w, float sA,
     PcgScalars* sc, const double* sig_cur, const double* sig_prev, int first, double* partials) {
     if (sc->done) return;
-    const double sig = block_sum_bcast(sig_cur, g.S);
+    const double sig = block_sum_bcast(sig_cur, g.S * kLoaders);
     double beta_d = 0;
-    if (!first) beta_d = sig / block_sum_bcast(sig_prev, g.S);
+    if (!first) beta_d = sig / block_sum_bcast(sig_prev, g.S * kLoaders);
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->sigma = sig;
     const float beta = (float)beta_d;
     double acc = 0;
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kBlock) void pcg_axpy_sk_kernel(
     const float* __restrict__ w, const PcgScalars* sc, const double* sig_cur, const double* ws_partials,
     int n_ws, double* partials) {
     if (sc->done) return;
-    const double sig = block_sum_bcast(sig_cur, g.S);
+    const double sig = block_sum_bcast(sig_cur, g.S * kLoaders);
     const double ws = block_sum_bcast(ws_partials, n_ws);
     const float a = (float)(sig / ws);
     float m = 0.0f;

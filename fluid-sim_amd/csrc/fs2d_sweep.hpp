@@ -8,33 +8,43 @@
 // row l in lane l and at step t processes column i = t - l: the row neighbour
 // is the lane's own previous result, the column neighbour is lane l-1's
 // previous result moved by one DPP wave_shr:1 (wave_shl:1 backward), and the
-// strip's edge lane receives the neighbouring strip's boundary row.
+// strip's edge lane receives the neighbouring strip's boundary row through the
+// DPP `old` operand.
 //
-// A step of a solve is one DPP and three FMAs; a lone wave issues them at one
-// VALU instruction per 4 clocks, so the design keeps every other cost off the
-// compute wave:
-//   wave 0          compute: per group of 4 steps, LDS reads for the next
-//                   group issued before the current one is computed, DPP/FMA
-//                   chain, one dwordx4 store per output array.  It never waits
-//                   on a global load.
-//   waves 1..3      loaders: stream the strip's inputs (SK4 layout, 1 KB per
-//                   wave-wide dwordx4 load) into an LDS ring, kBatch groups in
-//                   flight per loader.
-//   wave 4          poller: reads the neighbouring strip's boundary row from
+// A solve step is one DPP and two FMAs.  A lone wave issues about one
+// instruction per 4 clocks whatever its type, so the compute wave does nothing
+// else it can hand to another wave:
+//   wave 0          compute: supergroups of 16 steps; the LDS reads of the
+//                   next supergroup are issued while the current one computes
+//                   (two register buffers, no copies); readiness flags are read
+//                   one supergroup ahead; one dwordx4 store per output array
+//                   and 4 steps; no global loads and no waits on them.
+//   waves 1..L      loaders: stream the strip's inputs (SK4 layout, 1 KB per
+//                   wave-wide dwordx4 load), apply the operator's per-cell
+//                   transform (e.g. P*r) and fill an LDS ring of supergroups.
+//   wave L+1        poller: reads the neighbouring strip's boundary row from
 //                   tagged 64-bit words ({payload, epoch}, agent-scope relaxed
 //                   atomics, see fs2d_wavefront.hpp) into an LDS ring indexed
-//                   by step, so a group's four values are one aligned read.
-// The producing lane's boundary values go through a 2-entry LDS outbox and are
-// stored by 4*NW lanes one group later.  Waves signal each other through LDS
-// counters.  Strips are taken from a 64-bit ticket (strip = ticket mod S,
-// launch = ticket / S, epoch = launch + 1): a strip only ever waits on a strip
-// owned by a workgroup that started earlier, and no reset is needed between
-// launches.  Every wait is bounded; on expiry the workgroup raises the error
-// flag and all its waves run to completion.
+//                   by step; steps whose edge column is outside the grid get
+//                   the boundary value.
+// The producing lane's 16 boundary values per supergroup go through a 2-entry
+// LDS outbox and are stored as tagged words by 16 lanes one supergroup later.
+//
+// Waves signal each other through LDS counters without s_waitcnt: LDS executes
+// one wave's operations in order, so a flag written after data (or after the
+// reads that free a slot) is seen only after them; compiler barriers keep the
+// accesses on their side of the flag.
+//
+// Strips are taken from a 64-bit ticket (strip = ticket mod S, launch =
+// ticket / S, epoch = launch + 1): a strip only ever waits on a strip owned by
+// a workgroup that started earlier, and no reset is needed between launches.
+// Every wait is bounded; on expiry the workgroup raises the error flag and all
+// its waves run to completion.
 //
 // SK4 layout: element (s, t, l) at ((s * T4 + t/4) * 64 + l) * 4 + t%4 (float
-// arrays; byte arrays use the same element index), T4 = ceil((nx + 64) / 4).
-// Padding entries hold zero coefficients, so every lane computes every step.
+// arrays; byte arrays use the same element index), T4 a multiple of 8 (an even
+// number of supergroups) with 4 T4 >= nx + 64.  Padding entries hold zero
+// coefficients, so every lane computes every step.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -58,7 +68,7 @@ __host__ inline SkGeom make_geom(int nx, int ny) {
     g.nx = nx;
     g.ny = ny;
     g.S = (ny + 63) / 64;
-    g.T4 = (nx + 64 + 3) / 4;
+    g.T4 = (nx + 64 + 31) / 32 * 8;
     return g;
 }
 
@@ -72,77 +82,203 @@ struct SkCtl {
 // -scaleA towards each coupled neighbour (FluidSim2D.cpp:266-301).
 enum : uint32_t { CPL_L = 1, CPL_R = 2, CPL_D = 4, CPL_U = 8 };
 
+constexpr int kSub = 4;         // SK4 groups per supergroup (16 steps)
 constexpr int kLoaders = 3;
-constexpr int kBatch = 8;                  // groups in flight per loader
-constexpr int kRing = 32;                  // groups in the LDS ring
-constexpr int kHandRing = 256;             // steps in the hand-off ring
-constexpr int kFlagEvery = 4;              // groups between consumption flags
+constexpr int kBatch = 2;       // supergroups per loader batch
+constexpr int kRing = 8;        // supergroups in the LDS ring (a power of two)
+constexpr int kHandRing = 256;  // steps in the hand-off ring
 constexpr int kSweepThreads = 64 * (kLoaders + 2);
-constexpr int kLdsSpin = 1 << 26;          // bounded LDS waits (~seconds)
-static_assert(kRing >= kLoaders * kBatch + 2 * kFlagEvery, "ring too small for the loaders");
+constexpr int kLdsSpin = 1 << 26;  // bounded LDS waits (~seconds)
+static_assert(kRing >= kLoaders * kBatch + 2, "ring too small for the loaders");
+static_assert((kRing & (kRing - 1)) == 0, "ring slots are taken modulo kRing");
 
 __device__ __forceinline__ float f4get(const float4& v, int q) {
     return q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w));
 }
+__device__ __forceinline__ uint32_t u4get(const uint4& v, int q) {
+    return q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w));
+}
 
-// LDS flags between the waves of one workgroup.  Only LDS data is published
-// through them, so a workgroup-scope release/acquire (which also waits for
-// every outstanding global store) is more than needed: the writer waits for
-// its own LDS operations (lgkmcnt) and LDS processes each wave's operations in
-// order; the compiler barriers keep LDS accesses on their side of the flag.
-__device__ __forceinline__ unsigned lds_ld_acq(unsigned* p) {
+__device__ __forceinline__ unsigned lds_ld(unsigned* p) {
     unsigned v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     asm volatile("" ::: "memory");
     return v;
 }
-__device__ __forceinline__ void lds_st_rel(unsigned* p, unsigned v) {
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+__device__ __forceinline__ void lds_st(unsigned* p, unsigned v) {
+    asm volatile("" ::: "memory");
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+__device__ __forceinline__ unsigned uniform(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
 
-// Neighbour-row value: lane l-1's (forward) or lane l+1's (backward) register,
-// `old` on the edge lane.
-template <bool B>
-__device__ __forceinline__ uint32_t nbr(uint32_t old, uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, B ? 0x130 /* wave_shl:1 */ : 0x138 /* wave_shr:1 */,
-                                                 0xf, 0xf, false);
+// Neighbour-row value: lane l-1's (forward) or lane l+1's (backward) register.
+// The edge lane gets `edge` (C: a strip upstream) or zero (no strip upstream;
+// every operator's boundary state is zero).
+template <bool B, bool C>
+__device__ __forceinline__ uint32_t nbr(uint32_t edge, uint32_t v) {
+    constexpr int ctl = B ? 0x130 /* wave_shl:1 */ : 0x138 /* wave_shr:1 */;
+    if (C) return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)v, ctl, 0xf, 0xf, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, ctl, 0xf, 0xf, true);
 }
 
+template <class Op>
+struct SweepLds {
+    float4 ring_f[kRing][kSub][Op::NF][64];
+    uint32_t ring_b[Op::NB ? kRing : 1][kSub][64];
+    uint32_t hand[Op::NW][kHandRing];
+    uint32_t outbox[2][Op::NW][16];
+    unsigned slot_tag[kRing];
+    unsigned consumed_sg, hand_ready, abort_flag;
+};
+
 // Op interface:
-//   NW    32-bit words of carried state, NF float input arrays, NB (0/1) byte
-//         input array `bytes`, NOUT float output arrays
-//   const float* in[NF]; const uint8_t* bytes; float* out[NOUT]
-//   bool skip(bool first)   wave 0 only; first: strip order 0
-//   void boundary(uint32_t*) state of out-of-grid neighbours
-//   template <bool B> void group(const float4* x, uint32_t b4, uint32_t* st,
-//        const uint32_t (*hb)[4], float (*o)[4], uint32_t (*prod)[4])
-//        4 steps; step q uses component q of x[a] and byte q of b4; st is the
-//        lane's state after the previous step, updated; hb[w][q] the edge
-//        lane's neighbour word w at step q; o[n][q] output n; prod[w][q] state
-//        word w after step q
-//   void strip_done(int s)  wave 0, after the last step
+//   NW    32-bit words of carried state; NIN float input arrays in[]; NF float
+//         arrays in the ring; NB (0/1) byte input array `bytes`; NOUT float
+//         output arrays out[]
+//   bool skip(bool first)      wave 0 only; first: strip order 0
+//   void xform(const float4 (&in)[NIN], float4 (&x)[NF], double& acc)
+//         loader: ring values of one SK4 group from its inputs
+//   void loader_done(int s, int loader, double acc)
+//   template <bool B, bool C> void group(const float4 (&x)[NF], uint32_t b4,
+//        uint32_t (&st)[NW], const uint4 (&hb)[NW], float (&o)[NOUT][4],
+//        uint32_t (&prod)[NW][4])
+//         4 steps; step q uses component q of x[a], byte q of b4 and, on the
+//         edge lane of a consuming strip, component q of hb[w]; st is the
+//         lane's state after the previous step, updated; o[n][q] output n;
+//         prod[w][q] state word w after step q
+//   void strip_done(int s)     wave 0, after the last step
+template <class Op, bool B, bool C, bool P>
+__device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<Op>& L, int s, unsigned epoch,
+                                              unsigned long long* dst, int* err) {
+    constexpr int NW = Op::NW, NF = Op::NF, NB = Op::NB, NOUT = Op::NOUT;
+    const int lane = threadIdx.x & 63;
+    const int NSG = g.T4 / kSub, nx = g.nx;
+    const int T = g.T4 * 4;
+    const int prod_lane = B ? 0 : 63;
+    const long strip_base = (long)s * g.T4 * 256;
+
+    struct Buf {
+        float4 x[kSub][NF];
+        uint32_t b[kSub];
+        uint4 h[kSub][NW];
+    };
+    // first step of supergroup sg (processing order)
+    auto sg_t0 = [&](int sg) { return B ? T - 16 * (sg + 1) : 16 * sg; };
+    auto ready = [&](int sg, unsigned tag, unsigned hr) {
+        return tag == (unsigned)sg + 1u && (!C || hr >= 16u * (unsigned)(sg + 1));
+    };
+    auto fetch = [&](int sg, Buf& f) {
+        const int slot = sg & (kRing - 1);
+        const int t0 = sg_t0(sg);
+#pragma unroll
+        for (int j = 0; j < kSub; j++) {
+#pragma unroll
+            for (int a = 0; a < NF; a++) f.x[j][a] = L.ring_f[slot][j][a][lane];
+            f.b[j] = NB ? L.ring_b[NB ? slot : 0][j][lane] : 0u;
+#pragma unroll
+            for (int w = 0; w < NW; w++)
+                f.h[j][w] = C ? *(const uint4*)&L.hand[w][(t0 + 4 * j) & (kHandRing - 1)] : make_uint4(0, 0, 0, 0);
+        }
+    };
+    bool bad = false;
+    auto wait = [&](int sg) {
+        for (int spin = 0; !bad; spin++) {
+            const unsigned tag = uniform(lds_ld(&L.slot_tag[sg & (kRing - 1)]));
+            const unsigned hr = C ? uniform(lds_ld(&L.hand_ready)) : 0u;
+            if (ready(sg, tag, hr)) break;
+            if (spin >= kLdsSpin || lds_ld(&L.abort_flag)) {
+                bad = true;
+                atomicExch(err, 1);
+                atomicExch(&L.abort_flag, 1u);
+            }
+            __builtin_amdgcn_s_sleep(0);
+        }
+    };
+
+    uint32_t st[NW];
+#pragma unroll
+    for (int w = 0; w < NW; w++) st[w] = 0u;
+    unsigned tag_n = 0, hr_n = 0;  // flags of the supergroup after the one being prefetched
+
+    // One supergroup: prefetch sg+1 into `nxt` while computing `cur`.  The
+    // prefetch index is clamped, so the last supergroup re-reads its own slot
+    // and every iteration has the same shape (no partial waits on the LDS
+    // counter in the steady state).
+    auto step = [&](int sg, Buf& cur, Buf& nxt) {
+        const int t0 = sg_t0(sg);
+        const int sn = min(sg + 1, NSG - 1);
+        if (!ready(sn, uniform(tag_n), uniform(hr_n))) wait(sn);
+        // flags of the following supergroup, read before the prefetch so that
+        // checking them next time waits for these two reads only
+        tag_n = lds_ld(&L.slot_tag[min(sg + 2, NSG - 1) & (kRing - 1)]);
+        if (C) hr_n = lds_ld(&L.hand_ready);
+        fetch(sn, nxt);
+        uint32_t ob = 0;
+        if (P && lane < 16 * NW) ob = L.outbox[(sg - 1) & 1][lane >> 4][lane & 15];
+
+        uint32_t prod[kSub][NW][4];
+#pragma unroll
+        for (int k = 0; k < kSub; k++) {
+            const int j = B ? kSub - 1 - k : k;
+            float o[NOUT][4];
+            op.template group<B, C>(cur.x[j], cur.b[j], st, cur.h[j], o, prod[j]);
+#pragma unroll
+            for (int n = 0; n < NOUT; n++) {
+                float* base = op.out[n] + strip_base + (long)t0 * 64;  // uniform
+                *(float4*)(base + j * 256 + lane * 4) = make_float4(o[n][0], o[n][1], o[n][2], o[n][3]);
+            }
+        }
+        if (P) {
+            if (sg > 0 && lane < 16 * NW) {
+                const int col = sg_t0(sg - 1) + (lane & 15) - prod_lane;
+                if (col >= 0 && col < nx)
+                    atomic_st64(dst + (size_t)col * NW + (lane >> 4),
+                                (unsigned long long)ob | ((unsigned long long)epoch << 32));
+            }
+            if (lane == prod_lane)
+#pragma unroll
+                for (int w = 0; w < NW; w++)
+#pragma unroll
+                    for (int j = 0; j < kSub; j++)
+                        *(uint4*)&L.outbox[sg & 1][w][4 * j] =
+                            make_uint4(prod[j][w][0], prod[j][w][1], prod[j][w][2], prod[j][w][3]);
+        }
+        if (lane == 0) lds_st(&L.consumed_sg, (unsigned)sg + 1u);
+    };
+
+    Buf b0, b1;
+    wait(0);
+    fetch(0, b0);
+    tag_n = lds_ld(&L.slot_tag[1 & (kRing - 1)]);
+    if (C) hr_n = lds_ld(&L.hand_ready);
+    for (int sg = 0; sg < NSG; sg += 2) {  // NSG is even
+        step(sg, b0, b1);
+        step(sg + 1, b1, b0);
+    }
+    if (P && lane < 16 * NW) {
+        const uint32_t ob = L.outbox[(NSG - 1) & 1][lane >> 4][lane & 15];
+        const int col = sg_t0(NSG - 1) + (lane & 15) - prod_lane;
+        if (col >= 0 && col < nx)
+            atomic_st64(dst + (size_t)col * NW + (lane >> 4),
+                        (unsigned long long)ob | ((unsigned long long)epoch << 32));
+    }
+    op.strip_done(s);
+}
+
 template <class Op, bool BACKWARD>
 __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(Op op, SkGeom g, SkCtl c) {
-    constexpr int NW = Op::NW, NF = Op::NF, NB = Op::NB, NOUT = Op::NOUT;
+    constexpr int NW = Op::NW, NIN = Op::NIN, NF = Op::NF, NB = Op::NB;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-
-    __shared__ __align__(16) float4 ring_f[kRing + 1][NF][64];  // + a spare slot for tail groups
-    __shared__ uint32_t ring_b[NB ? kRing + 1 : 1][64];
-    __shared__ unsigned slot_tag[kRing];
-    __shared__ __align__(16) uint32_t hand[NW][kHandRing];
-    __shared__ __align__(16) uint32_t outbox[2][NW][4];
-    __shared__ unsigned consumed_groups, hand_ready, hand_consumed, abort_flag;
+    __shared__ __align__(16) SweepLds<Op> L;
     __shared__ unsigned long long sh_ticket;
     __shared__ int sh_skip;
 
     if (threadIdx.x == 0) {
         sh_ticket = atomicAdd(c.ticket, 1ull);
-        consumed_groups = 0;
-        hand_ready = 0;
-        hand_consumed = 0;
-        abort_flag = 0;
+        L.consumed_sg = 0;
+        L.hand_ready = 0;
+        L.abort_flag = 0;
     }
-    if (threadIdx.x < kRing) slot_tag[threadIdx.x] = 0;
+    if (threadIdx.x < kRing) L.slot_tag[threadIdx.x] = 0;
     __syncthreads();
     const int order = (int)(sh_ticket % (unsigned long long)g.S);
     const unsigned epoch = (unsigned)(sh_ticket / (unsigned long long)g.S) + 1u;
@@ -155,209 +291,104 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(Op op, SkGeom g, S
     __syncthreads();
     if (sh_skip) return;
 
-    const int T4 = g.T4, nx = g.nx;
-    const long strip_base = (long)s * T4 * 256;
+    const int NSG = g.T4 / kSub, T = g.T4 * 4, nx = g.nx;
     const bool consume = BACKWARD ? (s + 1 < g.S) : (s > 0);
     const bool produce = BACKWARD ? (s > 0) : (s + 1 < g.S);
     const int edge_lane = BACKWARD ? 63 : 0;
-    const int prod_lane = BACKWARD ? 0 : 63;
     auto raise = [&]() {
         atomicExch(c.err, 1);
-        atomicExch(&abort_flag, 1u);
-    };
-    // Hand-off values are consumed in order k = 0, 1, ...: column k forward,
-    // nx-1-k backward.  Largest k among a group's four steps (-1: none).
-    auto group_kmax = [&](int t0) {
-        if (BACKWARD) {
-            const int lo = max(t0 - 63, 0), hi = min(t0 - 60, nx - 1);
-            return lo <= hi ? nx - 1 - lo : -1;
-        }
-        return t0 < nx ? min(t0 + 3, nx - 1) : -1;
+        atomicExch(&L.abort_flag, 1u);
     };
 
     if (wave == 0) {
-        // ------------------------------------------------------------- compute
-        uint32_t st[NW], bnd[NW];
-        op.boundary(bnd);
-#pragma unroll
-        for (int w = 0; w < NW; w++) st[w] = bnd[w];
         unsigned long long* dst = c.handoff + (size_t)s * nx * NW;
-        bool bad = false;
-
-        auto fetch = [&](int gi, float4* x, uint32_t& b4, uint32_t (*hb)[4], unsigned& tag, unsigned& hr) {
-            const int slot = gi % kRing;
-            const int t0 = (BACKWARD ? T4 - 1 - gi : gi) * 4;
-            tag = lds_ld_acq(&slot_tag[slot]);
-            hr = consume ? lds_ld_acq(&hand_ready) : 0u;
-#pragma unroll
-            for (int a = 0; a < NF; a++) x[a] = ring_f[slot][a][lane];
-            b4 = NB ? ring_b[NB ? slot : 0][lane] : 0u;
-            if (consume) {
-#pragma unroll
-                for (int w = 0; w < NW; w++) {
-                    const uint4 h = *(const uint4*)&hand[w][t0 & (kHandRing - 1)];
-                    hb[w][0] = h.x;
-                    hb[w][1] = h.y;
-                    hb[w][2] = h.z;
-                    hb[w][3] = h.w;
-                }
-            }
-        };
-        auto ready = [&](int gi, unsigned tag, unsigned hr) {
-            const int km = consume ? group_kmax((BACKWARD ? T4 - 1 - gi : gi) * 4) : -1;
-            return tag == (unsigned)gi + 1u && (km < 0 || (int)hr > km);
-        };
-        auto wait_ready = [&](int gi) {
-            for (int spin = 0; !bad; spin++) {
-                const unsigned tag = lds_ld_acq(&slot_tag[gi % kRing]);
-                const unsigned hr = consume ? lds_ld_acq(&hand_ready) : 0u;
-                if (ready(gi, tag, hr)) break;
-                if (spin >= kLdsSpin || lds_ld_acq(&abort_flag)) {
-                    bad = true;
-                    raise();
-                }
-                __builtin_amdgcn_s_sleep(0);
-            }
-        };
-
-        float4 xc[NF], xn[NF];
-        uint32_t bc = 0, bn = 0;
-        uint32_t hbc[NW][4], hbn[NW][4];
-        unsigned tg, hr;
-#pragma unroll
-        for (int w = 0; w < NW; w++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) hbc[w][q] = hbn[w][q] = bnd[w];
-        wait_ready(0);
-        fetch(0, xc, bc, hbc, tg, hr);
-        int pend_t0 = -1;  // step of the group whose boundary words sit in the outbox
-        for (int gi = 0; gi < T4; gi++) {
-            const int t0 = (BACKWARD ? T4 - 1 - gi : gi) * 4;
-            const bool more = gi + 1 < T4;
-            if (more) fetch(gi + 1, xn, bn, hbn, tg, hr);
-            uint32_t ob = 0;
-            if (produce && pend_t0 >= 0 && lane < 4 * NW) ob = outbox[(gi - 1) & 1][lane >> 2][lane & 3];
-
-            // steps whose edge column lies outside the grid see the boundary
-            if (consume) {
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int col = t0 + q - edge_lane;
-                    if (col < 0 || col >= nx)
-#pragma unroll
-                        for (int w = 0; w < NW; w++) hbc[w][q] = bnd[w];
-                }
-            }
-            float o[NOUT > 0 ? NOUT : 1][4];
-            uint32_t prod[NW][4];
-            op.template group<BACKWARD>(xc, bc, st, hbc, o, prod);
-#pragma unroll
-            for (int n = 0; n < NOUT; n++)
-                *(float4*)(op.out[n] + strip_base + (long)t0 * 64 + lane * 4) =
-                    make_float4(o[n][0], o[n][1], o[n][2], o[n][3]);
-            if (produce) {
-                // word (w, q) of the previous group: lane 4 w + q
-                if (pend_t0 >= 0 && lane < 4 * NW) {
-                    const int col = pend_t0 + (lane & 3) - prod_lane;
-                    if (col >= 0 && col < nx)
-                        atomic_st64(dst + (size_t)col * NW + (lane >> 2),
-                                    (unsigned long long)ob | ((unsigned long long)epoch << 32));
-                }
-                if (lane == prod_lane)
-#pragma unroll
-                    for (int w = 0; w < NW; w++)
-                        *(uint4*)&outbox[gi & 1][w][0] = make_uint4(prod[w][0], prod[w][1], prod[w][2], prod[w][3]);
-                pend_t0 = t0;
-            }
-            if (lane == 0 && ((gi % kFlagEvery) == kFlagEvery - 1 || !more)) {
-                lds_st_rel(&consumed_groups, (unsigned)gi + 1u);
-                const int km = group_kmax(t0);
-                if (consume && km >= 0) lds_st_rel(&hand_consumed, (unsigned)km + 1u);
-            }
-            if (more) {
-                if (!ready(gi + 1, tg, hr)) {
-                    wait_ready(gi + 1);
-                    fetch(gi + 1, xn, bn, hbn, tg, hr);
-                }
-#pragma unroll
-                for (int a = 0; a < NF; a++) xc[a] = xn[a];
-                bc = bn;
-#pragma unroll
-                for (int w = 0; w < NW; w++)
-#pragma unroll
-                    for (int q = 0; q < 4; q++) hbc[w][q] = hbn[w][q];
-            }
-        }
-        if (produce && pend_t0 >= 0) {
-            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-            if (lane < 4 * NW) {
-                const uint32_t ob = outbox[(T4 - 1) & 1][lane >> 2][lane & 3];
-                const int col = pend_t0 + (lane & 3) - prod_lane;
-                if (col >= 0 && col < nx)
-                    atomic_st64(dst + (size_t)col * NW + (lane >> 2),
-                                (unsigned long long)ob | ((unsigned long long)epoch << 32));
-            }
-        }
-        op.strip_done(s);
+        if (consume && produce)
+            sweep_compute<Op, BACKWARD, true, true>(op, g, L, s, epoch, dst, c.err);
+        else if (consume)
+            sweep_compute<Op, BACKWARD, true, false>(op, g, L, s, epoch, dst, c.err);
+        else if (produce)
+            sweep_compute<Op, BACKWARD, false, true>(op, g, L, s, epoch, dst, c.err);
+        else
+            sweep_compute<Op, BACKWARD, false, false>(op, g, L, s, epoch, dst, c.err);
     } else if (wave <= kLoaders) {
         // ------------------------------------------------------------- loaders
-        // A batch of kBatch groups is loaded into registers with all loads in
-        // flight, then written to the ring.  Tail groups beyond the strip go to
-        // a spare slot nobody reads, so every register index is a constant.
+        // Loader m takes batches of kBatch supergroups, m-th of every kLoaders.
+        // All loads of a batch are in flight together; addresses past the strip
+        // are clamped and their ring writes skipped (a uniform branch), so
+        // every register index stays a constant.
         const int me = wave - 1;
+        const long strip_base = (long)s * g.T4 * 256 + lane * 4;
         bool bad = false;
-        for (int gb = me * kBatch; gb < T4; gb += kLoaders * kBatch) {
-            const int last = min(gb + kBatch, T4) - 1;
-            // slot of group gi is free once group gi - kRing has been consumed
-            for (int spin = 0; !bad && (int)lds_ld_acq(&consumed_groups) < last - kRing + 1; spin++) {
-                if (spin >= kLdsSpin || lds_ld_acq(&abort_flag)) {
+        double acc = 0.0;
+        for (int sb = me * kBatch; sb < NSG; sb += kLoaders * kBatch) {
+            const int last = min(sb + kBatch, NSG) - 1;
+            // slot of supergroup sg is free once sg - kRing has been consumed
+            for (int spin = 0; !bad && (int)lds_ld(&L.consumed_sg) < last - kRing + 1; spin++) {
+                if (spin >= kLdsSpin || lds_ld(&L.abort_flag)) {
                     bad = true;
                     raise();
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
-            float4 v[kBatch * NF];
-            uint32_t b[kBatch];
+            float4 v[kBatch][kSub][NIN];
+            uint32_t b[kBatch][kSub];
 #pragma unroll
             for (int k = 0; k < kBatch; k++) {
-                const int gi = min(gb + k, T4 - 1);
-                const long base = strip_base + (long)(BACKWARD ? T4 - 1 - gi : gi) * 256 + lane * 4;
+                const int sg = min(sb + k, NSG - 1);
+                const int t0 = BACKWARD ? T - 16 * (sg + 1) : 16 * sg;
 #pragma unroll
-                for (int a = 0; a < NF; a++) v[k * NF + a] = *(const float4*)(op.in[a] + base);
-                b[k] = NB ? *(const uint32_t*)(op.bytes + base) : 0u;
+                for (int j = 0; j < kSub; j++) {
+                    const long e = strip_base + (long)(t0 + 4 * j) * 64;
+#pragma unroll
+                    for (int a = 0; a < NIN; a++) v[k][j][a] = *(const float4*)(op.in[a] + e);
+                    b[k][j] = NB ? *(const uint32_t*)(op.bytes + e) : 0u;
+                }
             }
 #pragma unroll
             for (int k = 0; k < kBatch; k++) {
-                const int slot = gb + k < T4 ? (gb + k) % kRing : kRing;
+                if (sb + k < NSG) {
+                    const int slot = (sb + k) & (kRing - 1);
 #pragma unroll
-                for (int a = 0; a < NF; a++) ring_f[slot][a][lane] = v[k * NF + a];
-                if (NB) ring_b[NB ? slot : 0][lane] = b[k];
+                    for (int j = 0; j < kSub; j++) {
+                        float4 x[NF];
+                        op.xform(v[k][j], x, acc);
+#pragma unroll
+                        for (int a = 0; a < NF; a++) L.ring_f[slot][j][a][lane] = x[a];
+                        if (NB) L.ring_b[NB ? slot : 0][j][lane] = b[k][j];
+                    }
+                }
             }
             if (lane == 0)
-                for (int gi = gb; gi <= last; gi++) lds_st_rel(&slot_tag[gi % kRing], (unsigned)gi + 1u);
+                for (int sg = sb; sg <= last; sg++) lds_st(&L.slot_tag[sg & (kRing - 1)], (unsigned)sg + 1u);
         }
+        op.loader_done(s, me, acc);
     } else if (consume) {
         // -------------------------------------------------------------- poller
+        // Steps in consumption order kk = 0 .. T-1 (step t = kk forward,
+        // T-1-kk backward); the edge lane reads column t - edge_lane.
         const int src_strip = BACKWARD ? s + 1 : s - 1;
         const unsigned long long* src = c.handoff + (size_t)src_strip * nx * NW;
         bool bad = false;
-        for (int k0 = 0; k0 < nx && !bad; k0 += 64) {
-            const int k = k0 + lane;
-            const bool mine = k < nx;
-            const int col = BACKWARD ? nx - 1 - k : k;
-            const int slot = (col + edge_lane) & (kHandRing - 1);  // = step of that column
+        for (int k0 = 0; k0 < T && !bad; k0 += 64) {
+            const int kk = k0 + lane;
+            const int t = BACKWARD ? T - 1 - kk : kk;
+            const int col = t - edge_lane;
+            const bool in_grid = kk < T && col >= 0 && col < nx;
             // ring entries of k0 .. k0+63 were used by k - kHandRing
-            for (int spin = 0; !bad && (int)lds_ld_acq(&hand_consumed) < k0 + 64 - kHandRing; spin++) {
-                if (spin >= kLdsSpin || lds_ld_acq(&abort_flag)) {
+            for (int spin = 0; !bad && 16 * (int)lds_ld(&L.consumed_sg) < k0 + 64 - kHandRing; spin++) {
+                if (spin >= kLdsSpin || lds_ld(&L.abort_flag)) {
                     bad = true;
                     raise();
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
+            if (!in_grid && kk < T)
+#pragma unroll
+                for (int w = 0; w < NW; w++) L.hand[w][t & (kHandRing - 1)] = 0u;  // boundary state
             unsigned long long hv[NW];
-            bool got = !mine;
 #pragma unroll
             for (int w = 0; w < NW; w++) hv[w] = 0;
+            bool got = !in_grid;
             for (int spin = 0; !bad; spin++) {
                 if (!got) {
                     bool ok = true;
@@ -368,16 +399,16 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(Op op, SkGeom g, S
                     }
                     if (ok) {
 #pragma unroll
-                        for (int w = 0; w < NW; w++) hand[w][slot] = (uint32_t)hv[w];
+                        for (int w = 0; w < NW; w++) L.hand[w][t & (kHandRing - 1)] = (uint32_t)hv[w];
                         got = true;
                     }
                 }
                 // publish the ready prefix of this chunk
                 const unsigned long long rdy = __ballot(got);
                 const int prefix = rdy == ~0ull ? 64 : __ffsll((long long)~rdy) - 1;
-                if (lane == 0) lds_st_rel(&hand_ready, (unsigned)min(k0 + prefix, nx));
+                if (lane == 0) lds_st(&L.hand_ready, (unsigned)min(k0 + prefix, T));
                 if (prefix == 64) break;
-                if (spin >= kSpinLimit || lds_ld_acq(&abort_flag)) {
+                if (spin >= kSpinLimit || lds_ld(&L.abort_flag)) {
                     bad = true;
                     if (lane == 0) raise();
                     break;
@@ -426,8 +457,8 @@ __device__ __forceinline__ double wave_sum_of(const double* a, int n) {
 // The first workgroup also finishes |rhs|_inf and initialises the solver
 // (:424-430, 447-448).
 struct FactorOp {
-    static constexpr int NW = 3, NF = 1, NB = 1, NOUT = 5;
-    const float* in[NF];   // Adiag
+    static constexpr int NW = 3, NIN = 1, NF = 1, NB = 1, NOUT = 5;
+    const float* in[NIN];  // Adiag
     const uint8_t* bytes;  // coupling bits
     float* out[NOUT];      // P, fx, fy, gx, gy
     double tau, sigma, sA;
@@ -447,15 +478,16 @@ struct FactorOp {
         }
         return false;
     }
-    __device__ void boundary(uint32_t* b) const { b[0] = b[1] = b[2] = 0; }
-    template <bool B>
-    __device__ void group(const float4* x, uint32_t b4, uint32_t* st, const uint32_t (*hb)[4], float (*o)[4],
-                          uint32_t (*prod)[4]) const {
+    __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double&) const { x[0] = in[0]; }
+    __device__ void loader_done(int, int, double) const {}
+    template <bool B, bool C>
+    __device__ void group(const float4 (&x)[NF], uint32_t b4, uint32_t (&st)[NW], const uint4 (&hb)[NW],
+                          float (&o)[NOUT][4], uint32_t (&prod)[NW][4]) const {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             uint32_t nb[NW];
 #pragma unroll
-            for (int w = 0; w < NW; w++) nb[w] = nbr<B>(hb[w][q], st[w]);
+            for (int w = 0; w < NW; w++) nb[w] = nbr<B, C>(u4get(hb[w], q), st[w]);
             const uint32_t f = (b4 >> (8 * q)) & 0xffu;
             const double A0 = f4get(x[0], q);
             const double PL = u2d(st), PB = u2d(nb);
@@ -484,21 +516,18 @@ struct FactorOp {
     __device__ void strip_done(int) const {}
 };
 
-// Forward solve q = Lt^-1 r (:400-409), q = P r + fx q_L + fy q_D.
-// Accumulates q.q per strip: z.r = r.M^-1 r = |Lt^-1 r|^2 = q.q is sigma
-// (:428, :457) without reading r again later.  In iteration mode the first
-// workgroup finishes |r|_inf and applies the stopping test (:453) before any
-// preconditioning, as the reference does.
+// Forward solve q = Lt^-1 r (:400-409), q = P r + fx q_L + fy q_D; the loaders
+// form P r.  In iteration mode the first workgroup finishes |r|_inf and
+// applies the stopping test (:453) before any preconditioning, as the
+// reference does.
 struct FwdOp {
-    static constexpr int NW = 1, NF = 4, NB = 0, NOUT = 1;
-    const float* in[NF];  // r, P, fx, fy
+    static constexpr int NW = 1, NIN = 4, NF = 3, NB = 0, NOUT = 1;
+    const float* in[NIN];  // r, P, fx, fy
     const uint8_t* bytes;
-    float* out[NOUT];     // q
+    float* out[NOUT];      // q
     PcgScalars* sc;
     const double* rmax_partials;
-    double* qq_partials;  // [S]
     int n_partials, check;
-    double acc = 0;
     __device__ bool skip(bool first) {
         if (sc->done) return true;
         if (!check) return false;
@@ -510,56 +539,67 @@ struct FwdOp {
         }
         return conv;
     }
-    __device__ void boundary(uint32_t* b) const { b[0] = 0; }
-    template <bool B>
-    __device__ void group(const float4* x, uint32_t, uint32_t* st, const uint32_t (*hb)[4], float (*o)[4],
-                          uint32_t (*prod)[4]) {
-        float qp = u2f(st[0]), sq = 0.0f;
+    __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double&) const {
+        x[0] = make_float4(in[1].x * in[0].x, in[1].y * in[0].y, in[1].z * in[0].z, in[1].w * in[0].w);
+        x[1] = in[2];
+        x[2] = in[3];
+    }
+    __device__ void loader_done(int, int, double) const {}
+    template <bool B, bool C>
+    __device__ void group(const float4 (&x)[NF], uint32_t, uint32_t (&st)[NW], const uint4 (&hb)[NW],
+                          float (&o)[NOUT][4], uint32_t (&prod)[NW][4]) const {
+        float qp = u2f(st[0]);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const float qd = u2f(nbr<B>(hb[0][q], f2u(qp)));
-            const float qn = fmaf(f4get(x[3], q), qd, fmaf(f4get(x[2], q), qp, f4get(x[1], q) * f4get(x[0], q)));
+            const float qd = u2f(nbr<B, C>(u4get(hb[0], q), f2u(qp)));
+            const float qn = fmaf(f4get(x[2], q), qd, fmaf(f4get(x[1], q), qp, f4get(x[0], q)));
             o[0][q] = qn;
             prod[0][q] = f2u(qn);
-            sq = fmaf(qn, qn, sq);
             qp = qn;
         }
-        acc += (double)sq;
         st[0] = f2u(qp);
     }
-    __device__ void strip_done(int s) {
-        double t = acc;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-        if ((threadIdx.x & 63) == 0) qq_partials[s] = t;
-    }
+    __device__ void strip_done(int) const {}
 };
 
 // Backward solve z = Lt^-T q (:410-419), z = P q + gx z_R + gy z_U; steps run
-// q = 3, 2, 1, 0 within a group.  The first workgroup records the iteration
-// count (:462).
+// q = 3, 2, 1, 0 within a group; the loaders form P q.  The loaders also sum
+// q.q per strip: z.r = r.M^-1 r = |Lt^-1 r|^2 = q.q is sigma (:428, :457).
+// The first workgroup records the iteration count (:462).
 struct BwdOp {
-    static constexpr int NW = 1, NF = 4, NB = 0, NOUT = 1;
-    const float* in[NF];  // q, P, gx, gy
+    static constexpr int NW = 1, NIN = 4, NF = 3, NB = 0, NOUT = 1;
+    const float* in[NIN];  // q, P, gx, gy
     const uint8_t* bytes;
-    float* out[NOUT];     // z
+    float* out[NOUT];      // z
     PcgScalars* sc;
-    int next_iter;        // iterations completed after this solve (0: initial solve)
+    double* qq_partials;   // [S][kLoaders]
+    int next_iter;         // iterations completed after this solve (0: initial solve)
     __device__ bool skip(bool first) const {
         if (sc->done) return true;
         if (first && (threadIdx.x & 63) == 0 && next_iter > 0) sc->iter = next_iter;
         return false;
     }
-    __device__ void boundary(uint32_t* b) const { b[0] = 0; }
-    template <bool B>
-    __device__ void group(const float4* x, uint32_t, uint32_t* st, const uint32_t (*hb)[4], float (*o)[4],
-                          uint32_t (*prod)[4]) const {
+    __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double& acc) const {
+        const float4 q = in[0], P = in[1];
+        x[0] = make_float4(P.x * q.x, P.y * q.y, P.z * q.z, P.w * q.w);
+        x[1] = in[2];
+        x[2] = in[3];
+        acc += (double)fmaf(q.w, q.w, fmaf(q.z, q.z, fmaf(q.y, q.y, q.x * q.x)));
+    }
+    __device__ void loader_done(int s, int me, double acc) const {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if ((threadIdx.x & 63) == 0) qq_partials[s * kLoaders + me] = acc;
+    }
+    template <bool B, bool C>
+    __device__ void group(const float4 (&x)[NF], uint32_t, uint32_t (&st)[NW], const uint4 (&hb)[NW],
+                          float (&o)[NOUT][4], uint32_t (&prod)[NW][4]) const {
         float zp = u2f(st[0]);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int q = 3 - i;
-            const float zu = u2f(nbr<B>(hb[0][q], f2u(zp)));
-            const float zn = fmaf(f4get(x[3], q), zu, fmaf(f4get(x[2], q), zp, f4get(x[1], q) * f4get(x[0], q)));
+            const float zu = u2f(nbr<B, C>(u4get(hb[0], q), f2u(zp)));
+            const float zn = fmaf(f4get(x[2], q), zu, fmaf(f4get(x[1], q), zp, f4get(x[0], q)));
             o[0][q] = zn;
             prod[0][q] = f2u(zn);
             zp = zn;
