@@ -671,7 +671,7 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
     A(wctl, 4); A(werr, 1);
     const int nstrips = (s->ny + 63) / 64;
     A(handoff, (size_t)nstrips * s->nx * 4);
-    A(sk_handoff, (size_t)nstrips * s->nx * 3);
+    A(sk_handoff, (size_t)nstrips * sk_handoff_stride(s->nx, 3));
 #undef A
     e = hipHostMalloc((void**)&s->h_sc, sizeof(PcgScalars), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_dg, sizeof(Diag), hipHostMallocDefault);

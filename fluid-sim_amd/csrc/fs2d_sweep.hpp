@@ -74,9 +74,10 @@ __host__ inline SkGeom make_geom(int nx, int ny) {
 
 struct SkCtl {
     unsigned long long* ticket;
-    unsigned long long* handoff;  // [S][nx][NW] tagged words
+    unsigned long long* handoff;  // per strip: [nx][NW] tagged words + 64 scratch words
     int* err;
 };
+__host__ __device__ inline size_t sk_handoff_stride(int nx, int nw) { return (size_t)nx * nw + 64; }
 
 // Coupling bits of a FLUID cell: the off-diagonal entries of row (i, j) are
 // -scaleA towards each coupled neighbour (FluidSim2D.cpp:266-301).
@@ -87,7 +88,8 @@ constexpr int kLoaders = 3;
 constexpr int kBatch = 2;       // supergroups per loader batch
 constexpr int kRing = 8;        // supergroups in the LDS ring (a power of two)
 constexpr int kHandRing = 256;  // steps in the hand-off ring
-constexpr int kSweepThreads = 64 * (kLoaders + 2);
+constexpr int kPollers = 2;     // poller waves: their polls interleave in time
+constexpr int kSweepThreads = 64 * (1 + kLoaders + kPollers);
 constexpr int kLdsSpin = 1 << 26;  // bounded LDS waits (~seconds)
 static_assert(kRing >= kLoaders * kBatch + 2, "ring too small for the loaders");
 static_assert((kRing & (kRing - 1)) == 0, "ring slots are taken modulo kRing");
@@ -107,6 +109,10 @@ __device__ __forceinline__ unsigned lds_ld(unsigned* p) {
 __device__ __forceinline__ void lds_st(unsigned* p, unsigned v) {
     asm volatile("" ::: "memory");
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_max(unsigned* p, unsigned v) {
+    asm volatile("" ::: "memory");
+    __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ unsigned uniform(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
 
@@ -163,57 +169,67 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
     };
     // first step of supergroup sg (processing order)
     auto sg_t0 = [&](int sg) { return B ? T - 16 * (sg + 1) : 16 * sg; };
-    auto ready = [&](int sg, unsigned tag, unsigned hr) {
-        return tag == (unsigned)sg + 1u && (!C || hr >= 16u * (unsigned)(sg + 1));
-    };
-    auto fetch = [&](int sg, Buf& f) {
+    // x data of a supergroup is in the ring (loaders); hand values of its
+    // steps are in the hand ring (poller)
+    auto x_ready = [&](int sg, unsigned tag) { return tag == (unsigned)sg + 1u; };
+    auto h_ready = [&](int sg, unsigned hr) { return hr >= 16u * (unsigned)(sg + 1); };
+    auto fetch_x = [&](int sg, Buf& f) {
         const int slot = sg & (kRing - 1);
-        const int t0 = sg_t0(sg);
 #pragma unroll
         for (int j = 0; j < kSub; j++) {
 #pragma unroll
             for (int a = 0; a < NF; a++) f.x[j][a] = L.ring_f[slot][j][a][lane];
             f.b[j] = NB ? L.ring_b[NB ? slot : 0][j][lane] : 0u;
-#pragma unroll
-            for (int w = 0; w < NW; w++)
-                f.h[j][w] = C ? *(const uint4*)&L.hand[w][(t0 + 4 * j) & (kHandRing - 1)] : make_uint4(0, 0, 0, 0);
         }
     };
+    // a supergroup's 16 steps sit at (t0 & 255) .. +15 of the hand ring (no wrap)
+    auto fetch_h = [&](int sg, Buf& f) {
+        const int h0 = sg_t0(sg) & (kHandRing - 1);
+#pragma unroll
+        for (int j = 0; j < kSub; j++)
+#pragma unroll
+            for (int w = 0; w < NW; w++)
+                f.h[j][w] = C ? *(const uint4*)&L.hand[w][h0 + 4 * j] : make_uint4(0, 0, 0, 0);
+    };
     bool bad = false;
-    auto wait = [&](int sg) {
-        for (int spin = 0; !bad; spin++) {
-            const unsigned tag = uniform(lds_ld(&L.slot_tag[sg & (kRing - 1)]));
-            const unsigned hr = C ? uniform(lds_ld(&L.hand_ready)) : 0u;
-            if (ready(sg, tag, hr)) break;
-            if (spin >= kLdsSpin || lds_ld(&L.abort_flag)) {
-                bad = true;
-                atomicExch(err, 1);
-                atomicExch(&L.abort_flag, 1u);
-            }
-            __builtin_amdgcn_s_sleep(0);
+    auto give_up = [&](int spin) {
+        if (spin >= kLdsSpin || lds_ld(&L.abort_flag)) {
+            bad = true;
+            atomicExch(err, 1);
+            atomicExch(&L.abort_flag, 1u);
         }
+        __builtin_amdgcn_s_sleep(0);
+    };
+    auto wait_x = [&](int sg) {
+        for (int spin = 0; !bad && !x_ready(sg, uniform(lds_ld(&L.slot_tag[sg & (kRing - 1)]))); spin++) give_up(spin);
+    };
+    auto wait_h = [&](int sg) {
+        for (int spin = 0; !bad && !h_ready(sg, uniform(lds_ld(&L.hand_ready))); spin++) give_up(spin);
     };
 
     uint32_t st[NW];
 #pragma unroll
     for (int w = 0; w < NW; w++) st[w] = 0u;
-    unsigned tag_n = 0, hr_n = 0;  // flags of the supergroup after the one being prefetched
+    // hand-off stores: lanes 16 w + q carry word w of step q; lanes that have
+    // nothing to store (past 16 NW, or a column outside the grid) write to a
+    // per-lane scratch slot after the strip's row, so the store needs no branch
+    unsigned long long* const dump = dst + (size_t)nx * NW + lane;
 
-    // One supergroup: prefetch sg+1 into `nxt` while computing `cur`.  The
+    // One supergroup: compute `cur` while the LDS reads of the next one go
+    // into `nxt` speculatively: the readiness flags are read just before the
+    // data (LDS executes a wave's operations in order), and checked after
+    // the compute; only a failed check branches (wait, then read again).  The
     // prefetch index is clamped, so the last supergroup re-reads its own slot
-    // and every iteration has the same shape (no partial waits on the LDS
-    // counter in the steady state).
+    // and every iteration has the same shape.
     auto step = [&](int sg, Buf& cur, Buf& nxt) {
         const int t0 = sg_t0(sg);
         const int sn = min(sg + 1, NSG - 1);
-        if (!ready(sn, uniform(tag_n), uniform(hr_n))) wait(sn);
-        // flags of the following supergroup, read before the prefetch so that
-        // checking them next time waits for these two reads only
-        tag_n = lds_ld(&L.slot_tag[min(sg + 2, NSG - 1) & (kRing - 1)]);
-        if (C) hr_n = lds_ld(&L.hand_ready);
-        fetch(sn, nxt);
+        const unsigned tagv = lds_ld(&L.slot_tag[sn & (kRing - 1)]);
+        const unsigned hrv = C ? lds_ld(&L.hand_ready) : 0u;
+        fetch_h(sn, nxt);
+        fetch_x(sn, nxt);
         uint32_t ob = 0;
-        if (P && lane < 16 * NW) ob = L.outbox[(sg - 1) & 1][lane >> 4][lane & 15];
+        if (P) ob = L.outbox[(sg - 1) & 1][min(lane >> 4, NW - 1)][lane & 15];
 
         uint32_t prod[kSub][NW][4];
 #pragma unroll
@@ -226,14 +242,16 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
                 float* base = op.out[n] + strip_base + (long)t0 * 64;  // uniform
                 *(float4*)(base + j * 256 + lane * 4) = make_float4(o[n][0], o[n][1], o[n][2], o[n][3]);
             }
+            // the previous supergroup's boundary values leave once the outbox
+            // read has had a sub-group's time to return
+            if (P && k == 0) {
+                const int col = sg_t0(sg - 1) + (lane & 15) - prod_lane;
+                const bool real = sg > 0 && lane < 16 * NW && col >= 0 && col < nx;
+                atomic_st64(real ? dst + (size_t)col * NW + (lane >> 4) : dump,
+                            (unsigned long long)ob | ((unsigned long long)epoch << 32));
+            }
         }
         if (P) {
-            if (sg > 0 && lane < 16 * NW) {
-                const int col = sg_t0(sg - 1) + (lane & 15) - prod_lane;
-                if (col >= 0 && col < nx)
-                    atomic_st64(dst + (size_t)col * NW + (lane >> 4),
-                                (unsigned long long)ob | ((unsigned long long)epoch << 32));
-            }
             if (lane == prod_lane)
 #pragma unroll
                 for (int w = 0; w < NW; w++)
@@ -243,19 +261,30 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
                             make_uint4(prod[j][w][0], prod[j][w][1], prod[j][w][2], prod[j][w][3]);
         }
         if (lane == 0) lds_st(&L.consumed_sg, (unsigned)sg + 1u);
+        const bool okx = x_ready(sn, uniform(tagv)), okh = !C || h_ready(sn, uniform(hrv));
+        if (!(okx && okh)) {
+            if (!okx) {
+                wait_x(sn);
+                fetch_x(sn, nxt);
+            }
+            if (!okh) {
+                wait_h(sn);
+                fetch_h(sn, nxt);
+            }
+        }
     };
 
     Buf b0, b1;
-    wait(0);
-    fetch(0, b0);
-    tag_n = lds_ld(&L.slot_tag[1 & (kRing - 1)]);
-    if (C) hr_n = lds_ld(&L.hand_ready);
+    wait_x(0);
+    if (C) wait_h(0);
+    fetch_h(0, b0);
+    fetch_x(0, b0);
     for (int sg = 0; sg < NSG; sg += 2) {  // NSG is even
         step(sg, b0, b1);
         step(sg + 1, b1, b0);
     }
     if (P && lane < 16 * NW) {
-        const uint32_t ob = L.outbox[(NSG - 1) & 1][lane >> 4][lane & 15];
+        const uint32_t ob = L.outbox[(NSG - 1) & 1][min(lane >> 4, NW - 1)][lane & 15];
         const int col = sg_t0(NSG - 1) + (lane & 15) - prod_lane;
         if (col >= 0 && col < nx)
             atomic_st64(dst + (size_t)col * NW + (lane >> 4),
@@ -301,7 +330,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(Op op, SkGeom g, S
     };
 
     if (wave == 0) {
-        unsigned long long* dst = c.handoff + (size_t)s * nx * NW;
+        unsigned long long* dst = c.handoff + (size_t)s * sk_handoff_stride(nx, NW);
         if (consume && produce)
             sweep_compute<Op, BACKWARD, true, true>(op, g, L, s, epoch, dst, c.err);
         else if (consume)
@@ -363,11 +392,15 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(Op op, SkGeom g, S
         }
         op.loader_done(s, me, acc);
     } else if (consume) {
-        // -------------------------------------------------------------- poller
+        // ------------------------------------------------------------- pollers
         // Steps in consumption order kk = 0 .. T-1 (step t = kk forward,
-        // T-1-kk backward); the edge lane reads column t - edge_lane.
+        // T-1-kk backward); the edge lane reads column t - edge_lane.  The
+        // pollers run the same loop, the second one half a poll behind, so a
+        // hand-off word is seen about half a memory round trip sooner; both
+        // write identical values and hand_ready only grows (LDS max).
         const int src_strip = BACKWARD ? s + 1 : s - 1;
-        const unsigned long long* src = c.handoff + (size_t)src_strip * nx * NW;
+        if (wave == kLoaders + 2) __builtin_amdgcn_s_sleep(4);
+        const unsigned long long* src = c.handoff + (size_t)src_strip * sk_handoff_stride(nx, NW);
         bool bad = false;
         for (int k0 = 0; k0 < T && !bad; k0 += 64) {
             const int kk = k0 + lane;
@@ -406,7 +439,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(Op op, SkGeom g, S
                 // publish the ready prefix of this chunk
                 const unsigned long long rdy = __ballot(got);
                 const int prefix = rdy == ~0ull ? 64 : __ffsll((long long)~rdy) - 1;
-                if (lane == 0) lds_st(&L.hand_ready, (unsigned)min(k0 + prefix, T));
+                if (lane == 0) lds_max(&L.hand_ready, (unsigned)min(k0 + prefix, T));
                 if (prefix == 64) break;
                 if (spin >= kSpinLimit || lds_ld(&L.abort_flag)) {
                     bad = true;
@@ -551,8 +584,10 @@ struct FwdOp {
         float qp = u2f(st[0]);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
+            // the row term first: the DPP then reads qp one instruction later
+            const float t1 = fmaf(f4get(x[1], q), qp, f4get(x[0], q));
             const float qd = u2f(nbr<B, C>(u4get(hb[0], q), f2u(qp)));
-            const float qn = fmaf(f4get(x[2], q), qd, fmaf(f4get(x[1], q), qp, f4get(x[0], q)));
+            const float qn = fmaf(f4get(x[2], q), qd, t1);
             o[0][q] = qn;
             prod[0][q] = f2u(qn);
             qp = qn;
@@ -598,8 +633,9 @@ struct BwdOp {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int q = 3 - i;
+            const float t1 = fmaf(f4get(x[1], q), zp, f4get(x[0], q));
             const float zu = u2f(nbr<B, C>(u4get(hb[0], q), f2u(zp)));
-            const float zn = fmaf(f4get(x[2], q), zu, fmaf(f4get(x[1], q), zp, f4get(x[0], q)));
+            const float zn = fmaf(f4get(x[2], q), zu, t1);
             o[0][q] = zn;
             prod[0][q] = f2u(zn);
             zp = zn;
