@@ -383,7 +383,7 @@ void launch_sweep(fs2d_sim* s, const Op& op) {
 fs2d_status stage_projection(fs2d_sim* s) {
     const SkGeom& g = s->geom;
     const long ns = g.size();
-    const int nb = blocks_for(ns);
+    const int nb = std::max(8, blocks_for(ns) / 8 * 8);  // a multiple of 8: see xcd_loop
     const double sA = s->dt / (s->rho * s->dx * s->dx);
     PcgScalars init{};
     init.tol = s->tol;

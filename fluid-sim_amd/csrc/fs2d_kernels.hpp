@@ -142,6 +142,24 @@ __global__ void gravity_kernel(float* __restrict__ u, float* __restrict__ v, lon
 // ---------------------------------------------------------------------------
 // Projection (src/FluidSim2D.cpp:252-467) on the SK4 layout (fs2d_skew.hpp).
 // ---------------------------------------------------------------------------
+// Element loop of a stencil kernel over an SK4 array.  Workgroups go to the 8
+// XCDs round-robin (workgroup b on XCD b % 8) and each XCD has its own L2, so a
+// grid-stride loop would fetch a cell's neighbours (one SK4 group, 256
+// elements, away) through another XCD's L2.  With a workgroup count that is a
+// multiple of 8, XCD x's workgroups share the contiguous range [x n/8,
+// (x+1) n/8) instead; otherwise this is a plain grid-stride loop.
+struct XcdLoop {
+    long e, end, stride;
+};
+__device__ __forceinline__ XcdLoop xcd_loop(long n) {
+    const int nb = (int)gridDim.x;
+    if (nb & 7) return {blockIdx.x * (long)blockDim.x + threadIdx.x, n, (long)nb * blockDim.x};
+    const int x = (int)blockIdx.x & 7, k = (int)blockIdx.x >> 3;
+    const long chunk = (n + 7) / 8;
+    const long b0 = x * chunk, b1 = b0 + chunk < n ? b0 + chunk : n;
+    return {b0 + k * (long)blockDim.x + threadIdx.x, b1, (long)(nb >> 3) * blockDim.x};
+}
+
 // Decode an SK4 element index into its cell; false for padding entries.
 __device__ __forceinline__ bool sk_cell(const SkGeom& g, long e, int& i, int& j) {
     const int e4 = (int)(e & 3), l = (int)((e >> 2) & 63);
@@ -229,7 +247,8 @@ __global__ __launch_bounds__(kBlock) void pcg_spmv_sk_kernel(
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->sigma = sig;
     const float beta = (float)beta_d;
     double acc = 0;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < g.size(); e += (long)gridDim.x * blockDim.x) {
+    const XcdLoop it = xcd_loop(g.size());
+    for (long e = it.e; e < it.end; e += it.stride) {
         const float sc_ = fmaf(beta, s_old[e], z[e]);
         s_new[e] = sc_;
         const float ad = Adiag[e];
@@ -263,7 +282,8 @@ __global__ __launch_bounds__(kBlock) void pcg_axpy_sk_kernel(
     const double ws = block_sum_bcast(ws_partials, n_ws);
     const float a = (float)(sig / ws);
     float m = 0.0f;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < g.size(); e += (long)gridDim.x * blockDim.x) {
+    const XcdLoop it = xcd_loop(g.size());
+    for (long e = it.e; e < it.end; e += it.stride) {
         p[e] = fmaf(a, s[e], p[e]);
         const float rv = fmaf(-a, w[e], r[e]);
         r[e] = rv;
