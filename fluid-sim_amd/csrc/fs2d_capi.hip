@@ -17,7 +17,7 @@
 #include <vector>
 
 #include "../../include/fs2d.h"
-#include "fs2d_kernels.hpp"
+#include "fs2d_adjoint.hpp"
 
 using namespace fs2d;
 
@@ -26,12 +26,13 @@ namespace {
 enum KernelClass {
     K_SL_ADVECT, K_GRAVITY, K_ASSEMBLE, K_FACTOR, K_PCG_FWD, K_PCG_BWD, K_PCG_SPMV, K_PCG_AXPY,
     K_UPDATE_VEL, K_EXTRAPOLATE, K_PARTICLE_SORT, K_P2G, K_G2P, K_PARTICLE_ADVECT,
-    K_LS_SWEEP, K_LS_OTHER, K_DIAGNOSTICS, K_MISC, K_COUNT
+    K_LS_SWEEP, K_LS_OTHER, K_DIAGNOSTICS, K_MISC, K_ADJ_SL, K_ADJ_OTHER, K_COUNT
 };
 const char* kKernelNames[K_COUNT] = {
     "sl_advect", "gravity", "assemble", "mic0_factor", "pcg_forward_solve", "pcg_backward_solve",
     "pcg_spmv_dot", "pcg_axpy_norm", "update_velocity", "extrapolate", "particle_sort", "p2g", "g2p",
-    "particle_advect", "levelset_sweep", "levelset_other", "diagnostics", "misc"};
+    "particle_advect", "levelset_sweep", "levelset_other", "diagnostics", "misc", "adjoint_sl",
+    "adjoint_other"};
 
 constexpr int kMaxBlocks = 2048;
 
@@ -380,19 +381,27 @@ void launch_sweep(fs2d_sim* s, const Op& op) {
                        sk_ctl(s));
 }
 
-fs2d_status stage_projection(fs2d_sim* s) {
+// Workgroups of the SK4 stencil kernels: a multiple of 8 (see xcd_loop).
+int sk_blocks(const fs2d_sim* s) { return std::max(8, blocks_for(s->geom.size()) / 8 * 8); }
+
+// Ghost-fluid matrix and rhs of the current velocity (src/FluidSim2D.cpp:252-362).
+void assemble(fs2d_sim* s) {
+    Timed t(s, K_ASSEMBLE);
+    const double sA = s->dt / (s->rho * s->dx * s->dx);
+    hipLaunchKernelGGL(assemble_sk_kernel, dim3(sk_blocks(s)), dim3(kBlock), 0, s->stream, s->geom, s->cell, s->phi,
+                       s->u, s->v, s->Adiag, s->cpl, s->r, s->pk, s->s[0], sA, 1.0 / s->dx, s->rmax_part);
+}
+
+// MIC(0) factor and PCG (src/FluidSim2D.cpp:364-466) on the assembled matrix
+// and the rhs in r (with p = s = 0 and the |rhs|_inf partials in rmax_part);
+// the solution goes to `out`, row-major.
+fs2d_status pcg_solve(fs2d_sim* s, float* out) {
     const SkGeom& g = s->geom;
-    const long ns = g.size();
-    const int nb = std::max(8, blocks_for(ns) / 8 * 8);  // a multiple of 8: see xcd_loop
+    const int nb = sk_blocks(s);
     const double sA = s->dt / (s->rho * s->dx * s->dx);
     PcgScalars init{};
     init.tol = s->tol;
     HIPCHK(s, hipMemcpyAsync(s->sc, &init, sizeof init, hipMemcpyHostToDevice, s->stream));
-    {
-        Timed t(s, K_ASSEMBLE);
-        hipLaunchKernelGGL(assemble_sk_kernel, dim3(nb), dim3(kBlock), 0, s->stream, g, s->cell, s->phi, s->u,
-                           s->v, s->Adiag, s->cpl, s->r, s->pk, s->s[0], sA, 1.0 / s->dx, s->rmax_part);
-    }
     {
         Timed t(s, K_FACTOR);
         FactorOp op{{s->Adiag}, s->cpl, {s->P, s->fx, s->fy, s->gx_, s->gy_}, s->tau, s->sigma, sA, s->rmax_part, nb,
@@ -450,11 +459,16 @@ fs2d_status stage_projection(fs2d_sim* s) {
     {
         Timed t(s, K_MISC);
         hipLaunchKernelGGL(sk_to_rowmajor_kernel, dim3(blocks_for(n_c(s))), dim3(kBlock), 0, s->stream, g, s->pk,
-                           s->p);
+                           out);
     }
     TRY(d2h(s, s->h_sc, s->sc, sizeof(PcgScalars)));
     s->last_iters_hint = std::max(2, s->h_sc->iter + 1);
     return s->h_sc->iter >= s->max_iters ? FS2D_WARN_MAXITER : FS2D_OK;
+}
+
+fs2d_status stage_projection(fs2d_sim* s) {
+    assemble(s);
+    return pcg_solve(s, s->p);
 }
 
 // updateVelocity (src/FluidSim2D.cpp:469-550).
@@ -1119,3 +1133,5 @@ const char* fs2d_status_string(int st) {
 }
 
 }  // extern "C"
+
+#include "fs2d_adjoint_api.hpp"

@@ -98,6 +98,14 @@ def load():
         "fs2d_get_kernel_time": (i, [vp, i, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_longlong)]),
         "fs2d_reset_kernel_times": (i, [vp]),
         "fs2d_last_error": (ctypes.c_char_p, [vp]),
+        "fs2d_adjoint_create": (i, [vp, i, i, ctypes.POINTER(vp)]),
+        "fs2d_adjoint_destroy": (None, [vp]),
+        "fs2d_adjoint_forward": (i, [vp, i]),
+        "fs2d_adjoint_backward": (i, [vp]),
+        "fs2d_adjoint_step": (i, [vp]),
+        "fs2d_adjoint_get_field": (i, [vp, i, i, vp, sz]),
+        "fs2d_adjoint_set_field": (i, [vp, i, i, vp, sz]),
+        "fs2d_adjoint_device_field": (i, [vp, i, ctypes.POINTER(vp)]),
         "fs2d_status_string": (ctypes.c_char_p, [i]),
     }
     for name, (res, args) in sigs.items():
@@ -258,6 +266,54 @@ class Sim:
 
     def reset_kernel_times(self):
         self._check(self.L.fs2d_reset_kernel_times(self.h))
+
+
+class Adjoint:
+    """Checkpointed reverse pass over rollouts of a Sim (fs2d_adjoint_* of the
+    C ABI).  The Sim must be semi-Lagrangian with static cells, no EMPTY cell
+    and no particles."""
+
+    def __init__(self, sim, checkpoint_every=8, max_steps=64):
+        self.sim = sim
+        self.L = sim.L
+        h = ctypes.c_void_p()
+        sim._check(self.L.fs2d_adjoint_create(sim.h, checkpoint_every, max_steps, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fs2d_adjoint_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, nsteps, allow=(OK, WARN_MAXITER)):
+        return self.sim._check(self.L.fs2d_adjoint_forward(self.h, nsteps), allow)
+
+    def backward(self, allow=(OK, WARN_MAXITER)):
+        return self.sim._check(self.L.fs2d_adjoint_backward(self.h), allow)
+
+    def step(self, allow=(OK, WARN_MAXITER)):
+        return self.sim._check(self.L.fs2d_adjoint_step(self.h), allow)
+
+    def get(self, f, dtype=np.float32):
+        n = self.L.fs2d_field_count(self.sim.h, f)
+        code = F64 if np.dtype(dtype) == np.float64 else F32
+        out = np.empty(n, dtype=dtype)
+        self.sim._check(self.L.fs2d_adjoint_get_field(self.h, f, code, _ptr(out), n))
+        return out.reshape(self.sim.shape(f))
+
+    def set(self, f, arr):
+        a = np.ascontiguousarray(arr).ravel()
+        if a.dtype == np.float64:
+            code = F64
+        else:
+            a, code = a.astype(np.float32), F32
+        self.sim._check(self.L.fs2d_adjoint_set_field(self.h, f, code, _ptr(a), a.size))
 
 
 def header_functions():

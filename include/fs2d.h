@@ -196,6 +196,47 @@ fs2d_status fs2d_reset_kernel_times(fs2d_sim* sim);
 const char* fs2d_last_error(const fs2d_sim* sim);
 const char* fs2d_status_string(int status);
 
+/*
+ * Adjoint (reverse-mode gradient) of a rollout of the all-fluid
+ * semi-Lagrangian step.  No reference counterpart: the reference has no
+ * adjoint (its step is runFrame, src/FluidSim2D.cpp:94-115, whose stages
+ * applySemiLagrangianAdvection :206-235, applyGravity :237-250,
+ * applyProjection :252-467 and updateVelocity :469-550 are differentiated
+ * here).  Scope: FS2D_SEMILAGRANGIAN, FS2D_FLAG_STATIC_CELLS, no EMPTY cell,
+ * no particles (k = 0); FS2D_FLAG_DENSITY adds the passive density to the
+ * differentiated state (u, v, density).  Errors are reported through
+ * fs2d_last_error of the simulation handle.  The adjoint handle must be
+ * destroyed before its simulation.
+ */
+typedef struct fs2d_adjoint fs2d_adjoint;
+
+/* Allocate checkpoints for rollouts of up to max_steps steps, keeping the
+ * state every checkpoint_every steps (1 keeps every state: no recomputation). */
+fs2d_status fs2d_adjoint_create(fs2d_sim* sim, int checkpoint_every, int max_steps, fs2d_adjoint** out);
+void fs2d_adjoint_destroy(fs2d_adjoint* adj);
+
+/* Run nsteps steps (fs2d_step) from the simulation's state, recording
+ * checkpoints.  The simulation ends at the rollout's final state. */
+fs2d_status fs2d_adjoint_forward(fs2d_adjoint* adj, int nsteps);
+
+/* Reverse pass over the last forward(): the adjoint fields hold the gradient
+ * of a scalar loss with respect to the final state (set them with
+ * fs2d_adjoint_set_field) and receive its gradient with respect to the initial
+ * state.  Recomputes each segment from its checkpoint; the simulation is left
+ * at the rollout's final state.  May be called again with new seeds. */
+fs2d_status fs2d_adjoint_backward(fs2d_adjoint* adj);
+
+/* One vector-Jacobian product at the simulation's current state: the adjoint
+ * fields hold the adjoints of the step's outputs and receive those of its
+ * inputs.  The state is not changed. */
+fs2d_status fs2d_adjoint_step(fs2d_adjoint* adj);
+
+/* Adjoint fields FS2D_FIELD_U, FS2D_FIELD_V, FS2D_FIELD_DENSITY (shapes and
+ * layout of the state fields; FP32 on the device). */
+fs2d_status fs2d_adjoint_get_field(fs2d_adjoint* adj, int field, int dtype, void* host, size_t count);
+fs2d_status fs2d_adjoint_set_field(fs2d_adjoint* adj, int field, int dtype, const void* host, size_t count);
+fs2d_status fs2d_adjoint_device_field(fs2d_adjoint* adj, int field, void** dptr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -604,9 +604,15 @@ struct Sim {
     void applyProjection() {
         assemble();
         factor();
+        pcgSolve(rhs, p);
+    }
+
+    // The PCG loop of applyProjection (:423-466) on the assembled and factored
+    // system: p = A^-1 b, starting from p = 0.
+    void pcgSolve(const ArrD& rhs, ArrD& p) {
         ArrD r, z, s;
         r.create(sizeX, sizeY); z.create(sizeX, sizeY); s.create(sizeX, sizeY);
-        p.reset();
+        p.create(sizeX, sizeY);
         r = rhs;
         applyPreconditioner(r, z);
         s = z;
@@ -704,6 +710,254 @@ struct Sim {
             pos[e] = nx;
         }
         nanFlag = nan;
+    }
+
+    // ======================================================================
+    // Adjoint of one step of the all-fluid semi-Lagrangian mode (no reference
+    // counterpart).  The step is: out-of-place SL advection of u, v and the
+    // density, applyGravity, applyProjection, updateVelocity with its
+    // extrapolation, on static cell flags without EMPTY cells (so A is the
+    // symmetric Laplacian) and without particles.  adjointStep() maps the
+    // adjoints of the step's outputs to the adjoints of its inputs at the
+    // current state, which it leaves unchanged.  FP64 throughout; the solve
+    // uses the same PCG as the forward step.
+    // ======================================================================
+
+    // Catmull-Rom weights of triCubic and their derivatives in the offset.
+    static void crWeights(double f, double w[4], double dw[4]) {
+        w[0] = -0.5 * (f * f * f) + (f * f) - 0.5 * f;
+        w[1] = 1.5 * (f * f * f) - 2.5 * (f * f) + 1;
+        w[2] = -1.5 * (f * f * f) + 2 * (f * f) + 0.5 * f;
+        w[3] = 0.5 * (f * f * f) - 0.5 * (f * f);
+        dw[0] = -1.5 * (f * f) + 2 * f - 0.5;
+        dw[1] = 4.5 * (f * f) - 5 * f;
+        dw[2] = -4.5 * (f * f) + 4 * f + 0.5;
+        dw[3] = 1.5 * (f * f) - f;
+    }
+
+    // Transpose of triCubic(A, px, py) applied to g: G(tap) += g * weight.
+    // Adds g * d triCubic / d(px, py) to (bx, by).
+    static void triCubicAdj(const ArrD& A, ArrD& G, double px, double py, double g, double& bx, double& by) {
+        int x = (int)px, y = (int)py;
+        if (x < 0 || x >= A.NX || y < 0 || y >= A.NY) return;
+        double wu[4], du[4], wv[4], dv[4];
+        crWeights(px - (double)x, wu, du);
+        crWeights(py - (double)y, wv, dv);
+        double sx = 0, sy = 0;
+        for (int r = 0; r < 4; r++) {
+            const int yr = clampi(y - 1 + r, 0, A.NY - 1);
+            double row = 0, drow = 0;
+            for (int c = 0; c < 4; c++) {
+                const int xc = clampi(x - 1 + c, 0, A.NX - 1);
+                const double a = A(xc, yr);
+                row += wu[c] * a;
+                drow += du[c] * a;
+                G(xc, yr) += g * wu[c] * wv[r];
+            }
+            sx += wv[r] * drow;
+            sy += dv[r] * row;
+        }
+        bx += g * sx;
+        by += g * sy;
+    }
+
+    // d clampd(x, lo, hi) / dx (one-sided 1 on the bounds, where clampd returns x).
+    static double clampGrad(double x, double lo, double hi) { return (x < lo || x > hi) ? 0.0 : 1.0; }
+
+    // Transposes of velInterpU / velInterpV / densInterp: scatter g into the
+    // field's adjoint and add g * d(sample)/d(position) to (bx, by).
+    void interpAdj(int which, const ArrD& A, ArrD& G, double px, double py, double g, double& bx,
+                   double& by) const {
+        double gx = px / dx, gy = py / dx;
+        if (which == 0) gy -= 0.5;
+        else if (which == 1) gx -= 0.5;
+        else { gx -= 0.5; gy -= 0.5; }
+        const double hx = (double)(sizeX - 1) - 1e-6, hy = (double)(sizeY - 1) - 1e-6;
+        double ax = 0, ay = 0;
+        triCubicAdj(A, G, clampd(gx, 1e-6, hx), clampd(gy, 1e-6, hy), g, ax, ay);
+        bx += clampGrad(gx, 1e-6, hx) * ax / dx;
+        by += clampGrad(gy, 1e-6, hy) * ay / dx;
+    }
+
+    // Transpose of backtrace(xp) at the current velocity: Xb is the adjoint of
+    // the departure point; the velocity samples of the three RK stages scatter
+    // into (ub, vb).
+    void backtraceAdj(V2 xp, V2 Xb, ArrD& ub, ArrD& vb) const {
+        V2 k1 = velInterp(xp);
+        V2 x2{xp.x - 0.5 * dt * k1.x, xp.y - 0.5 * dt * k1.y};
+        V2 k2 = velInterp(x2);
+        V2 x3{xp.x - 0.75 * dt * k2.x, xp.y - 0.75 * dt * k2.y};
+        V2 k3 = velInterp(x3);
+        double sx = (2. / 9.) * dt * k1.x + (3. / 9.) * dt * k2.x;
+        double sy = (2. / 9.) * dt * k1.y + (3. / 9.) * dt * k2.y;
+        sx = sx + (4. / 9.) * dt * k3.x;
+        sy = sy + (4. / 9.) * dt * k3.y;
+        const double off = 1e-3;
+        Xb.x *= clampGrad(xp.x - sx, (1.0 + off) * dx, (sizeX - 1.0 - off) * dx);
+        Xb.y *= clampGrad(xp.y - sy, (1.0 + off) * dx, (sizeY - 1.0 - off) * dx);
+        V2 k1b{-(2. / 9.) * dt * Xb.x, -(2. / 9.) * dt * Xb.y};
+        V2 k2b{-(3. / 9.) * dt * Xb.x, -(3. / 9.) * dt * Xb.y};
+        V2 k3b{-(4. / 9.) * dt * Xb.x, -(4. / 9.) * dt * Xb.y};
+        V2 x3b{0, 0}, x2b{0, 0}, x1b{0, 0};
+        interpAdj(0, u, ub, x3.x, x3.y, k3b.x, x3b.x, x3b.y);
+        interpAdj(1, v, vb, x3.x, x3.y, k3b.y, x3b.x, x3b.y);
+        k2b.x += -0.75 * dt * x3b.x;
+        k2b.y += -0.75 * dt * x3b.y;
+        interpAdj(0, u, ub, x2.x, x2.y, k2b.x, x2b.x, x2b.y);
+        interpAdj(1, v, vb, x2.x, x2.y, k2b.y, x2b.x, x2b.y);
+        k1b.x += -0.5 * dt * x2b.x;
+        k1b.y += -0.5 * dt * x2b.y;
+        interpAdj(0, u, ub, xp.x, xp.y, k1b.x, x1b.x, x1b.y);
+        interpAdj(1, v, vb, xp.x, xp.y, k1b.y, x1b.x, x1b.y);
+    }
+
+    // Transpose of the out-of-place SL stage (u, v and, with useDensity, the
+    // density): (ub1, vb1, db1) are the adjoints of its outputs.
+    void slAdjoint(const ArrD& ub1, const ArrD& vb1, const ArrD& db1, ArrD& ub, ArrD& vb, ArrD& db) const {
+        ub.create(sizeX + 1, sizeY); vb.create(sizeX, sizeY + 1); db.create(sizeX, sizeY);
+        for (int j = 0; j < sizeY; j++)
+            for (int i = 0; i < sizeX + 1; i++) {
+                const double g = ub1(i, j);
+                if (g == 0.0) continue;
+                V2 xp{(double)i * dx, ((double)j + 0.5) * dx};
+                V2 X = backtrace(xp), Xb{0, 0};
+                interpAdj(0, u, ub, X.x, X.y, g, Xb.x, Xb.y);
+                backtraceAdj(xp, Xb, ub, vb);
+            }
+        for (int j = 0; j < sizeY + 1; j++)
+            for (int i = 0; i < sizeX; i++) {
+                const double g = vb1(i, j);
+                if (g == 0.0) continue;
+                V2 xp{((double)i + 0.5) * dx, (double)j * dx};
+                V2 X = backtrace(xp), Xb{0, 0};
+                interpAdj(1, v, vb, X.x, X.y, g, Xb.x, Xb.y);
+                backtraceAdj(xp, Xb, ub, vb);
+            }
+        if (!useDensity) return;
+        for (int j = 0; j < sizeY; j++)
+            for (int i = 0; i < sizeX; i++) {
+                const double g = db1(i, j);
+                if (g == 0.0) continue;
+                V2 xp{((double)i + 0.5) * dx, ((double)j + 0.5) * dx};
+                V2 X = backtrace(xp), Xb{0, 0};
+                interpAdj(2, dens, db, X.x, X.y, g, Xb.x, Xb.y);
+                backtraceAdj(xp, Xb, ub, vb);
+            }
+    }
+
+    // Transpose of extrapolate(A, mask): the BFS's dequeue order reversed; an
+    // extrapolated entry hands its adjoint in equal parts to the neighbours it
+    // averaged and is zeroed, since extrapolation overwrote its input value.
+    static void extrapolateAdj(ArrD& G, Arr<uint32_t> mask) {
+        const int NX = G.NX, NY = G.NY;
+        std::vector<std::pair<int, int>> order;
+        std::deque<std::pair<int, int>> q;
+        for (int j = 0; j < NY; j++)
+            for (int i = 0; i < NX; i++)
+                if (mask(i, j) != 0 &&
+                    ((i > 0 && mask(i - 1, j) == 0) || (i < NX - 1 && mask(i + 1, j) == 0) ||
+                     (j > 0 && mask(i, j - 1) == 0) || (j < NY - 1 && mask(i, j + 1) == 0))) {
+                    mask(i, j) = 1;
+                    q.emplace_back(i, j);
+                }
+        while (!q.empty()) {
+            auto f = q.front();
+            q.pop_front();
+            order.push_back(f);
+            const int xs[4] = {f.first - 1, f.first + 1, f.first, f.first};
+            const int ys[4] = {f.second, f.second, f.second - 1, f.second + 1};
+            for (int k = 0; k < 4; k++) {
+                if (xs[k] < 0 || xs[k] >= NX || ys[k] < 0 || ys[k] >= NY) continue;
+                if (mask(xs[k], ys[k]) == UINT32_MAX) {
+                    mask(xs[k], ys[k]) = mask(f.first, f.second) + 1;
+                    q.emplace_back(xs[k], ys[k]);
+                }
+            }
+        }
+        for (size_t n = order.size(); n-- > 0;) {
+            const int x = order[n].first, y = order[n].second;
+            const int xs[4] = {x - 1, x + 1, x, x};
+            const int ys[4] = {y, y, y - 1, y + 1};
+            int cnt = 0;
+            for (int k = 0; k < 4; k++)
+                if (xs[k] >= 0 && xs[k] < NX && ys[k] >= 0 && ys[k] < NY && mask(xs[k], ys[k]) < mask(x, y)) cnt++;
+            const double g = G(x, y);
+            G(x, y) = 0;
+            if (cnt == 0) continue;
+            for (int k = 0; k < 4; k++)
+                if (xs[k] >= 0 && xs[k] < NX && ys[k] >= 0 && ys[k] < NY && mask(xs[k], ys[k]) < mask(x, y))
+                    G(xs[k], ys[k]) += g / cnt;
+        }
+        // entries the BFS never reached keep mask ~0 and their (unchanged) value
+    }
+
+    // Transpose of applyProjection + updateVelocity (with extrapolation) for
+    // static flags without EMPTY cells: (ub, vb) enter as the adjoints of the
+    // updated velocity and leave as the adjoints of the velocity the
+    // projection started from.
+    void projectAdjoint(ArrD& ub, ArrD& vb) {
+        Arr<uint32_t> du, dv;
+        du.create(sizeX + 1, sizeY); dv.create(sizeX, sizeY + 1);
+        auto fluidAdjU = [&](int i, int j) { return (i > 0 && cell(i - 1, j) == FLUID) || cell(i, j) == FLUID; };
+        auto fluidAdjV = [&](int i, int j) { return (j > 0 && cell(i, j - 1) == FLUID) || cell(i, j) == FLUID; };
+        for (int j = 0; j < sizeY; j++)
+            for (int i = 0; i < sizeX; i++) {
+                if (!fluidAdjU(i, j)) du(i, j) = UINT32_MAX;
+                if (!fluidAdjV(i, j)) dv(i, j) = UINT32_MAX;
+            }
+        extrapolateAdj(ub, du);
+        extrapolateAdj(vb, dv);
+        // updateVelocity: nu = u - scale (p(i) - p(i-1)) between two FLUID
+        // cells, 0 next to SOLID; u(sizeX, j) and v(i, sizeY) pass through.
+        const double scale = dt / (rho * dx);
+        ArrD pb;
+        pb.create(sizeX, sizeY);
+        for (int j = 0; j < sizeY; j++)
+            for (int i = 0; i < sizeX; i++) {
+                if (fluidAdjU(i, j)) {
+                    if ((i == 0 || cell(i - 1, j) == SOLID) || cell(i, j) == SOLID) ub(i, j) = 0;
+                    else { pb(i, j) -= scale * ub(i, j); pb(i - 1, j) += scale * ub(i, j); }
+                }
+                if (fluidAdjV(i, j)) {
+                    if ((j == 0 || cell(i, j - 1) == SOLID) || cell(i, j) == SOLID) vb(i, j) = 0;
+                    else { pb(i, j) -= scale * vb(i, j); pb(i, j - 1) += scale * vb(i, j); }
+                }
+            }
+        // p = A^-1 rhs with A symmetric: the adjoint solve uses A itself.
+        assemble();
+        factor();
+        ArrD lam;
+        pcgSolve(pb, lam);
+        // rhs = -(1/dx) div u, with the faces next to SOLID cancelled (assemble()).
+        const double sr = 1.0 / dx;
+        for (int j = 0; j < sizeY; j++)
+            for (int i = 0; i < sizeX; i++) {
+                if (cell(i, j) != FLUID) continue;
+                const double l = lam(i, j);
+                if (cell(i - 1, j) != SOLID) ub(i, j) += sr * l;
+                if (cell(i + 1, j) != SOLID) ub(i + 1, j) -= sr * l;
+                if (cell(i, j - 1) != SOLID) vb(i, j) += sr * l;
+                if (cell(i, j + 1) != SOLID) vb(i, j + 1) -= sr * l;
+            }
+    }
+
+    // True when the adjoint's assumptions hold: SL mode, static flags, no
+    // EMPTY cell, no particles, out-of-place advection.
+    bool adjointSupported() const {
+        if (mode != MODE_SL || levelSet || slInplace || !pos.empty()) return false;
+        for (auto c : cell.d) if (c == EMPTY) return false;
+        return true;
+    }
+
+    // Adjoint of runFrame() at the current state (left unchanged; the solver
+    // overwrites the assembled system and lastIters).  Inputs are the
+    // adjoints of u, v, density after the step; outputs those before it.
+    void adjointStep(const ArrD& ubOut, const ArrD& vbOut, const ArrD& dbOut, ArrD& ubIn, ArrD& vbIn,
+                     ArrD& dbIn) {
+        ArrD ub = ubOut, vb = vbOut;
+        projectAdjoint(ub, vb);
+        // applyGravity adds a constant: identity on the adjoint.
+        slAdjoint(ub, vb, dbOut, ubIn, vbIn, dbIn);
     }
 
     void runStage(int st) {
@@ -866,6 +1120,25 @@ void orc_extrapolate(double* data, uint32_t* mask, int nx, int ny) {
     extrapolate(a, m);
     std::memcpy(data, a.d.data(), sizeof(double) * a.d.size());
     std::memcpy(mask, m.d.data(), sizeof(uint32_t) * m.d.size());
+}
+
+// Adjoint of one step at the current state (Sim::adjointStep): u/v/density
+// adjoints after the step in, before the step out (row-major, field shapes).
+// Returns -1 when the state is outside the adjoint's scope.
+int orc_adjoint_step(void* h, const double* ub_out, const double* vb_out, const double* db_out,
+                     double* ub_in, double* vb_in, double* db_in) {
+    Sim* s = (Sim*)h;
+    if (!s->adjointSupported()) return -1;
+    ArrD uo, vo, dO, ui, vi, di;
+    uo.create(s->sizeX + 1, s->sizeY); vo.create(s->sizeX, s->sizeY + 1); dO.create(s->sizeX, s->sizeY);
+    std::memcpy(uo.d.data(), ub_out, sizeof(double) * uo.d.size());
+    std::memcpy(vo.d.data(), vb_out, sizeof(double) * vo.d.size());
+    std::memcpy(dO.d.data(), db_out, sizeof(double) * dO.d.size());
+    s->adjointStep(uo, vo, dO, ui, vi, di);
+    std::memcpy(ub_in, ui.d.data(), sizeof(double) * ui.d.size());
+    std::memcpy(vb_in, vi.d.data(), sizeof(double) * vi.d.size());
+    std::memcpy(db_in, di.d.data(), sizeof(double) * di.d.size());
+    return 0;
 }
 
 void orc_rand_stream(unsigned seed, uint32_t* out, int n) {

@@ -66,6 +66,8 @@ def lib():
         L.orc_tricubic.argtypes = [vp, i, i, d, d]
         L.orc_extrapolate.argtypes = [vp, vp, i, i]
         L.orc_rand_stream.argtypes = [ctypes.c_uint, vp, i]
+        L.orc_adjoint_step.restype = i
+        L.orc_adjoint_step.argtypes = [vp] * 7
         L.orc_set_threads.argtypes = [i]
         L.orc_max_threads.restype = i
         _lib = L
@@ -148,6 +150,18 @@ class Oracle:
 
     def level_set_parts(self, which):
         self.L.orc_level_set_parts(self.h, which)
+
+    def adjoint_step(self, ub, vb, db):
+        """Adjoint of one step at the current state: the adjoints of u, v and
+        density after the step in, those before it out (FP64)."""
+        ub = np.ascontiguousarray(ub, dtype=np.float64)
+        vb = np.ascontiguousarray(vb, dtype=np.float64)
+        db = np.ascontiguousarray(db, dtype=np.float64)
+        outs = [np.zeros_like(ub), np.zeros_like(vb), np.zeros_like(db)]
+        if self.L.orc_adjoint_step(self.h, _ptr(ub), _ptr(vb), _ptr(db), *[_ptr(o) for o in outs]) != 0:
+            raise ValueError("adjoint needs SL mode, static flags without EMPTY cells, no particles, "
+                             "out-of-place advection")
+        return outs
 
     def apply_precon(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)
