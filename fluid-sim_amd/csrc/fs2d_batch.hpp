@@ -1,0 +1,399 @@
+// fs2d_batch.hpp -- batched launches: one launch runs a kernel of the step
+// over every simulation of an ensemble, each with its own grid size.
+//
+// Each kernel's body takes the workgroup's index and count within its
+// simulation (vb, nvb).  A batched launch concatenates the simulations'
+// workgroup ranges, each as large as the simulation's own single launch and
+// starting at a multiple of 8 (so vb % 8 is the XCD, as xcd_loop assumes); a
+// map gives every workgroup its simulation and vb.  The arithmetic and the
+// reduction orders are therefore those of the single-simulation launches, and
+// a member of a batch steps bit-identically to the same simulation alone.
+//
+// The wavefront passes (level-set sweeps, MIC(0) factor, triangular solves)
+// take strips from one ticket over the concatenated strip lists of all
+// simulations: a strip's upstream neighbour belongs to the same simulation and
+// has a smaller position in the list, so it was taken earlier and the launch
+// cannot deadlock.  Per-simulation solver state (PcgScalars) masks converged
+// simulations: their workgroups return at once.
+#pragma once
+
+#include "fs2d_kernels.hpp"
+
+namespace fs2d {
+
+// Device pointers and scalars of one simulation, copied from its handle.
+struct InstBufs {
+    int nx, ny, mode, flags;
+    long np;
+    double dt, dx, dr, rho, gx, gy, alpha, tau, sigma;
+    double pmass;  // rho dx^2 / k^2 (0 without particles)
+    int max_iters;
+    SkGeom geom;
+    float *u, *v, *u2, *v2, *nu, *nv, *dens, *dens2;
+    uint8_t* cell;
+    double *phi, *phi2;
+    int *t0, *t1;
+    float *p, *r, *z, *q, *w, *s0, *s1, *pk, *Adiag, *P, *fx, *fy, *gx_, *gy_;
+    uint8_t* cpl;
+    PcgScalars* sc;
+    double *qq, *ws_part, *rmax_part;
+    uint32_t *mu, *mv;
+    int *fu0, *fu1, *fv0, *fv1, *fcount;
+    float *px, *py, *pvx, *pvy;
+    int *pkey, *counts, *starts, *cursor, *sorted, *tile_sums;
+    Diag* dg;
+    double* partials;
+    unsigned long long* ls_handoff;  // hand-off rows of the batched level-set passes
+    unsigned long long* sk_handoff;  // hand-off rows of the batched solver passes
+};
+
+// Workgroup -> (simulation, workgroup within it) of one batched launch.
+struct BlockMap {
+    const int* inst;   // per workgroup; -1 for padding
+    const int* first;  // per simulation
+    const int* count;  // per simulation
+};
+__device__ __forceinline__ bool bmap(const BlockMap& m, int& b, int& vb, int& nvb) {
+    b = m.inst[blockIdx.x];
+    if (b < 0) return false;
+    vb = (int)blockIdx.x - m.first[b];
+    nvb = m.count[b];
+    return true;
+}
+
+// Global strip -> (simulation, strip order within it).
+struct StripMap {
+    const int* inst;
+    const int* first;
+    int total;
+};
+
+// ---------------------------------------------------------------------------
+// Element-wise and stencil kernels.
+// ---------------------------------------------------------------------------
+enum BatchFill { BF_COUNTS, BF_FCOUNT };
+__global__ __launch_bounds__(kBlock) void b_fill_kernel(const InstBufs* __restrict__ ins, BlockMap bm, int what) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    if (what == BF_COUNTS) fill_body(I.counts, 0, (size_t)I.nx * I.ny, vb, nvb);
+    else fill_body(I.fcount, 0, 2, vb, nvb);
+}
+
+// Copies over the face domain (u then v) or the cells.
+enum BatchCopy { BC_UV_FROM_NEW, BC_UV_FROM_U2, BC_DENS_FROM_DENS2, BC_CURSOR_FROM_STARTS };
+__global__ __launch_bounds__(kBlock) void b_copy_kernel(const InstBufs* __restrict__ ins, BlockMap bm, int what) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    const long nU = (long)(I.nx + 1) * I.ny, nV = (long)I.nx * (I.ny + 1), nC = (long)I.nx * I.ny;
+    if (what == BC_UV_FROM_NEW || what == BC_UV_FROM_U2) {
+        const float* su = what == BC_UV_FROM_NEW ? I.nu : I.u2;
+        const float* sv = what == BC_UV_FROM_NEW ? I.nv : I.v2;
+        for (long e = vb * (long)blockDim.x + threadIdx.x; e < nU + nV; e += (long)nvb * blockDim.x) {
+            if (e < nU) I.u[e] = su[e];
+            else I.v[e - nU] = sv[e - nU];
+        }
+    } else if (what == BC_DENS_FROM_DENS2) {
+        copy_body(I.dens, (const float*)I.dens2, (size_t)nC, vb, nvb);
+    } else {
+        copy_body(I.cursor, (const int*)I.starts, (size_t)nC, vb, nvb);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void b_sl_advect_kernel(const InstBufs* __restrict__ ins, BlockMap bm,
+                                                             int with_gravity) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    const bool dens = (I.flags & 2) != 0;  // FS2D_FLAG_DENSITY
+    const float gu = with_gravity ? (float)(I.dt * I.gx) : 0.0f;
+    const float gv = with_gravity ? (float)(I.dt * I.gy) : 0.0f;
+    sl_advect_body(MacView{I.u, I.v, I.nx, I.ny, I.dx}, I.u2, I.v2, dens ? I.dens : nullptr, I.dens2, I.dt,
+                   clamp_box(I.nx, I.ny, I.dx), gu, gv, vb, nvb);
+}
+
+__global__ __launch_bounds__(kBlock) void b_gravity_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    gravity_body(I.u, I.v, (long)(I.nx + 1) * I.ny, (long)I.nx * (I.ny + 1), (float)(I.dt * I.gx),
+                 (float)(I.dt * I.gy), vb, nvb);
+}
+
+// PcgScalars of every simulation: zero, tolerance set (one workgroup each).
+__global__ void b_pcg_init_kernel(const InstBufs* __restrict__ ins, int n, double tol) {
+    const int b = blockIdx.x;
+    if (b >= n || threadIdx.x != 0) return;
+    PcgScalars z{};
+    z.tol = tol;
+    *ins[b].sc = z;
+}
+
+__global__ __launch_bounds__(kBlock) void b_assemble_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    const double sA = I.dt / (I.rho * I.dx * I.dx);
+    assemble_sk_body(I.geom, I.cell, I.phi, I.u, I.v, I.Adiag, I.cpl, I.r, I.pk, I.s0, sA, 1.0 / I.dx,
+                     I.rmax_part, vb, nvb);
+}
+
+// Iteration h of the PCG loop: s double-buffered by parity, sigma_h and
+// sigma_{h-1} in the two halves of qq.
+__global__ __launch_bounds__(kBlock) void b_spmv_kernel(const InstBufs* __restrict__ ins, BlockMap bm, int h) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    const double* qq0 = I.qq;
+    const double* qq1 = I.qq + (size_t)I.geom.S * kLoaders;
+    const float sA = (float)(I.dt / (I.rho * I.dx * I.dx));
+    pcg_spmv_sk_body(I.geom, I.Adiag, I.cpl, I.z, (h & 1) ? I.s1 : I.s0, (h & 1) ? I.s0 : I.s1, I.w, sA, I.sc,
+                     (h & 1) ? qq1 : qq0, (h & 1) ? qq0 : qq1, h == 0 ? 1 : 0, I.ws_part, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_axpy_kernel(const InstBufs* __restrict__ ins, BlockMap bm, int h) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    const double* qq0 = I.qq;
+    const double* qq1 = I.qq + (size_t)I.geom.S * kLoaders;
+    pcg_axpy_sk_body(I.geom, I.pk, I.r, (h & 1) ? I.s0 : I.s1, I.w, I.sc, (h & 1) ? qq1 : qq0, I.ws_part, nvb,
+                     I.rmax_part, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_sk_to_rowmajor_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    sk_to_rowmajor_body(I.geom, I.pk, I.p, vb, nvb);
+}
+
+__global__ __launch_bounds__(kBlock) void b_update_velocity_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    update_velocity_body(I.cell, I.phi, I.p, I.u, I.v, I.nu, I.nv, I.mu, I.mv, I.nx, I.ny, I.dt / (I.rho * I.dx),
+                         vb, nvb);
+}
+
+// Extrapolation of (u, v) (which = 0, after P2G) or (nu, nv) (which = 1).
+__device__ __forceinline__ void b_extra_arrays(const InstBufs& I, int which, ExtraArr& A, ExtraArr& B) {
+    A = ExtraArr{which ? I.nu : I.u, I.mu, I.fu0, I.fu1, I.fcount + 0, I.nx + 1, I.ny};
+    B = ExtraArr{which ? I.nv : I.v, I.mv, I.fv0, I.fv1, I.fcount + 1, I.nx, I.ny + 1};
+}
+__global__ __launch_bounds__(kBlock) void b_extrap_layer1_kernel(const InstBufs* __restrict__ ins, BlockMap bm,
+                                                                 int which) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    ExtraArr A, B;
+    b_extra_arrays(ins[b], which, A, B);
+    extrap_layer1_body(A, B, vb, nvb);
+}
+// Two workgroups per simulation: u and v.
+__global__ __launch_bounds__(1024) void b_extrap_layers_kernel(const InstBufs* __restrict__ ins, int n, int which) {
+    const int b = blockIdx.x >> 1;
+    if (b >= n) return;
+    ExtraArr A, B;
+    b_extra_arrays(ins[b], which, A, B);
+    extrap_layers_body(A, B, blockIdx.x & 1, 2);
+}
+
+// ---------------------------------------------------------------------------
+// Particles.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void b_particle_keys_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    particle_keys_body(I.px, I.py, I.np, I.dx, I.nx, I.ny, I.pkey, I.counts, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_scan_tiles_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    scan_tiles_body(I.counts, I.starts, I.tile_sums, (long)I.nx * I.ny, vb, nvb);
+}
+__global__ void b_scan_sums_kernel(const InstBufs* __restrict__ ins, int n) {
+    const int b = blockIdx.x;
+    if (b >= n) return;
+    const InstBufs& I = ins[b];
+    scan_sums_body(I.tile_sums, (int)(((long)I.nx * I.ny + 1023) / 1024), 0, 1);
+}
+__global__ __launch_bounds__(kBlock) void b_scan_add_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    scan_add_body(I.starts, I.tile_sums, (long)I.nx * I.ny, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_particle_scatter_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    particle_scatter_body(I.pkey, I.np, I.cursor, I.sorted, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_segment_sort_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    segment_sort_body(I.starts, I.counts, I.sorted, (long)I.nx * I.ny, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_p2g_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    p2g_body(I.px, I.py, I.pvx, I.pvy, I.starts, I.counts, I.sorted, I.u, I.v, I.mu, I.mv, I.nx, I.ny, I.dx, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_g2p_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    g2p_body(I.px, I.py, I.pvx, I.pvy, I.np, I.nu, I.nv, I.u, I.v, I.nx, I.ny, I.dx, (float)I.alpha, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_particle_advect_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    particle_advect_body(I.px, I.py, I.pvx, I.pvy, I.np, MacView{I.u, I.v, I.nx, I.ny, I.dx}, I.dt,
+                         clamp_box(I.nx, I.ny, I.dx), I.dg, I.partials, vb, nvb);
+}
+// One workgroup per simulation; `bm` is the map of the launch that wrote the partials.
+__global__ __launch_bounds__(kBlock) void b_cfl_finalize_kernel(const InstBufs* __restrict__ ins, int n, BlockMap bm) {
+    const int b = blockIdx.x;
+    if (b >= n) return;
+    const InstBufs& I = ins[b];
+    if (I.np == 0) return;
+    cfl_finalize_body(I.partials, bm.count[b], I.dg, 0, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Level set.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void b_ls_seed_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    ls_seed_body(I.px, I.py, I.starts, I.counts, I.sorted, I.t0, I.nx, (long)I.nx * I.ny, I.dx, I.dr, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_ls_phi_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    ls_phi_body(I.t0, I.px, I.py, I.phi2, I.nx, (long)I.nx * I.ny, I.dx, I.dr, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_ls_surface_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    ls_surface_body(I.phi2, I.phi, I.nx, I.ny, vb, nvb);
+}
+// Smoothing pass `parity`: phi -> phi2 (0) or phi2 -> phi (1).
+__global__ __launch_bounds__(kBlock) void b_ls_smooth_kernel(const InstBufs* __restrict__ ins, BlockMap bm,
+                                                             int parity) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    ls_smooth_body(parity ? I.phi2 : I.phi, parity ? I.phi : I.phi2, I.nx, I.ny, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_classify_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    classify_body(I.cell, I.phi, (long)I.nx * I.ny, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_diagnostics_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    diagnostics_body(I.cell, MacView{I.u, I.v, I.nx, I.ny, I.dx}, I.px, I.py, I.pvx, I.pvy, I.np, I.rho, I.gx, I.gy,
+                     I.pmass, I.partials, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_diag_finalize_kernel(const InstBufs* __restrict__ ins, int n,
+                                                                 BlockMap bm) {
+    const int b = blockIdx.x;
+    if (b >= n) return;
+    const InstBufs& I = ins[b];
+    diag_finalize_body(I.partials, bm.count[b], I.dx, I.dg, 0, 1);
+}
+
+// Flags of every simulation's solver and diagnostics, gathered for one host
+// read: out[4b] done, out[4b+1] iterations, out[4b+2] NaN flag, out[4b+3]
+// CFL warning (particle CFL > 5, src/FluidSim2D.cpp:582-585).
+__global__ void b_gather_status_kernel(const InstBufs* __restrict__ ins, int n, int* out) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x) {
+        out[4 * b] = ins[b].sc->done;
+        out[4 * b + 1] = ins[b].sc->iter;
+        out[4 * b + 2] = ins[b].dg->nan_flag;
+        out[4 * b + 3] = (ins[b].np > 0 && ins[b].dg->cfl_max > 5.0) ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront passes over all simulations.
+// ---------------------------------------------------------------------------
+// Level-set sweeps: pass `dir` (bit 0 flips x, bit 1 flips y) reading the
+// parity's input buffer.
+struct ClosestMaker {
+    int dir, parity;
+    __device__ ClosestOp operator()(const InstBufs& I) const {
+        return ClosestOp{parity ? I.t1 : I.t0, parity ? I.t0 : I.t1, I.px, I.py, I.nx, I.ny, dir & 1, dir >> 1,
+                         I.dx, I.dr};
+    }
+};
+struct EikonalMaker {
+    int dir, parity;
+    __device__ EikonalOp operator()(const InstBufs& I) const {
+        return EikonalOp{parity ? I.phi2 : I.phi, parity ? I.phi : I.phi2, I.nx, I.ny, dir & 1, dir >> 1, I.dx};
+    }
+};
+template <class Op, class Maker>
+__global__ __launch_bounds__(64) void b_wavefront_kernel(const InstBufs* __restrict__ ins, StripMap sm, WaveCtl c,
+                                                         Maker mk) {
+    const WaveTicket tk = wave_ticket(c);
+    if (tk.err == 0 && (int)tk.ticket < sm.total) {
+        const int b = sm.inst[tk.ticket];
+        const InstBufs& I = ins[b];
+        Op op = mk(I);
+        if (!op.skip()) wavefront_strip(op, I.nx, I.ny, (int)tk.ticket - sm.first[b], tk.epoch, I.ls_handoff, c.err);
+    }
+    wave_finish(c, (unsigned)sm.total, tk.epoch);
+}
+
+// Solver passes.  check: the forward solve applies the stopping test first;
+// h: the backward solve of iteration h (-1: the initial solve).
+struct FactorMaker {
+    const int* nparts;  // workgroups of the SK launches per simulation (the rhs partials)
+    __device__ FactorOp operator()(const InstBufs& I, int b) const {
+        const double sA = I.dt / (I.rho * I.dx * I.dx);
+        return FactorOp{{I.Adiag}, I.cpl, {I.P, I.fx, I.fy, I.gx_, I.gy_}, I.tau, I.sigma, sA, I.rmax_part,
+                        nparts[b], I.max_iters, I.sc};
+    }
+};
+struct FwdMaker {
+    const int* nparts;
+    int check;
+    __device__ FwdOp operator()(const InstBufs& I, int b) const {
+        return FwdOp{{I.r, I.P, I.fx, I.fy}, nullptr, {I.q}, I.sc, I.rmax_part, nparts[b], check};
+    }
+};
+struct BwdMaker {
+    int h;
+    __device__ BwdOp operator()(const InstBufs& I, int) const {
+        // sigma_{h+1} goes where sigma_{h-1} was: the half of parity h + 1
+        double* qq = I.qq + (size_t)(((h + 1) & 1) ? I.geom.S * kLoaders : 0);
+        return BwdOp{{I.q, I.P, I.gx_, I.gy_}, nullptr, {I.z}, I.sc, qq, h + 1};
+    }
+};
+template <class Op, bool BACKWARD, class Maker>
+__global__ __launch_bounds__(kSweepThreads) void b_sweep_kernel(const InstBufs* __restrict__ ins, StripMap sm,
+                                                                unsigned long long* ticket, int* err, Maker mk) {
+    __shared__ __align__(16) SweepLds<Op> L;
+    const unsigned long long t = sweep_ticket(L, ticket);
+    const int gs = (int)(t % (unsigned long long)sm.total);
+    const unsigned epoch = (unsigned)(t / (unsigned long long)sm.total) + 1u;
+    const int b = sm.inst[gs];
+    const InstBufs& I = ins[b];
+    Op op = mk(I, b);
+    const SkCtl c{ticket, I.sk_handoff, err};
+    sweep_run<Op, BACKWARD>(op, I.geom, c, gs - sm.first[b], epoch, L);
+}
+
+}  // namespace fs2d

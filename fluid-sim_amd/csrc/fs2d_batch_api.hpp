@@ -1,0 +1,601 @@
+// fs2d_batch_api.hpp -- the fs2d_batch_* entry points of include/fs2d.h: an
+// ensemble of independent simulations of different sizes stepped together,
+// each kernel of the step launched once for all of them (fs2d_batch.hpp).
+// Included at the end of fs2d_capi.hip (it uses that file's handles, stage
+// helpers and allocation code).
+//
+// Each member is an ordinary handle (fs2d_create): its state, fields and
+// statistics are reached through the fs2d_* calls on fs2d_batch_sim(b, i).
+// The batch owns the hand-off rows and tickets of its wavefront passes, so
+// batched and single launches never share them.  The PCG loop runs until
+// every member has converged; converged members' workgroups return at once.
+
+struct fs2d_batch {
+    int n = 0;
+    std::vector<fs2d_sim*> sims;
+    hipStream_t stream = nullptr;
+    int mode = 0, flags = 0, max_iters = 0;
+    double tol = 0;
+    bool any_particles = false;
+    std::vector<InstBufs> h_ins;
+    InstBufs* d_ins = nullptr;
+    int* d_maps = nullptr;
+    BlockMap m_cells{}, m_faces{}, m_sl{}, m_part{}, m_diag{}, m_sk{}, m_tiles{}, m_one{};
+    int g_cells = 0, g_faces = 0, g_sl = 0, g_part = 0, g_diag = 0, g_sk = 0, g_tiles = 0, g_one = 0;
+    StripMap strips{};
+    unsigned* ls_ctl = nullptr;
+    int* werr = nullptr;
+    unsigned long long* sk_ticket = nullptr;
+    int* d_status = nullptr;
+    int* h_status = nullptr;  // pinned: per member done, iterations, NaN flag, CFL warning
+    int* h_err = nullptr;
+    bool sorted_valid = false;
+    int last_iters_hint = 8;
+    std::vector<void*> allocs;
+    std::string err;
+
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> event_pool;
+    double ktime[K_COUNT] = {};
+    long long kcount[K_COUNT] = {};
+};
+
+namespace {
+
+fs2d_status bfail(fs2d_batch* B, fs2d_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    B->err = buf;
+    return st;
+}
+
+#define BHIPCHK(B, x)                                                                                 \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess)                                                                         \
+            return bfail(B, e_ == hipErrorOutOfMemory ? FS2D_ERR_OOM : FS2D_ERR_DEVICE, "%s: %s", #x, \
+                         hipGetErrorString(e_));                                                      \
+    } while (0)
+
+template <class T>
+fs2d_status balloc(fs2d_batch* B, T** p, size_t n) {
+    void* ptr = nullptr;
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&ptr, bytes);
+    if (e != hipSuccess) return bfail(B, FS2D_ERR_OOM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    B->allocs.push_back(ptr);
+    *p = (T*)ptr;
+    BHIPCHK(B, hipMemsetAsync(ptr, 0, bytes, B->stream));
+    return FS2D_OK;
+}
+
+// Descriptor of member s (the hand-off rows are the batch's).
+InstBufs inst_bufs(const fs2d_sim* s, unsigned long long* ls_handoff, unsigned long long* sk_handoff) {
+    InstBufs I;
+    memset(&I, 0, sizeof I);  // padding too: descriptors are compared bytewise
+    I.nx = s->nx; I.ny = s->ny; I.mode = s->mode; I.flags = s->flags; I.np = s->np;
+    I.dt = s->dt; I.dx = s->dx; I.dr = s->dr; I.rho = s->rho; I.gx = s->gx; I.gy = s->gy; I.alpha = s->alpha;
+    I.tau = s->tau; I.sigma = s->sigma; I.max_iters = s->max_iters;
+    I.pmass = s->k > 0 ? s->rho * s->dx * s->dx / (s->k * s->k) : 0.0;
+    I.geom = s->geom;
+    I.u = s->u; I.v = s->v; I.u2 = s->u2; I.v2 = s->v2; I.nu = s->nu; I.nv = s->nv; I.dens = s->dens; I.dens2 = s->dens2;
+    I.cell = s->cell; I.phi = s->phi; I.phi2 = s->phi2; I.t0 = s->t0; I.t1 = s->t1;
+    I.p = s->p; I.r = s->r; I.z = s->z; I.q = s->q; I.w = s->w; I.s0 = s->s[0]; I.s1 = s->s[1]; I.pk = s->pk;
+    I.Adiag = s->Adiag; I.P = s->P; I.fx = s->fx; I.fy = s->fy; I.gx_ = s->gx_; I.gy_ = s->gy_; I.cpl = s->cpl;
+    I.sc = s->sc; I.qq = s->qq; I.ws_part = s->ws_part; I.rmax_part = s->rmax_part;
+    I.mu = s->mu; I.mv = s->mv; I.fu0 = s->fu0; I.fu1 = s->fu1; I.fv0 = s->fv0; I.fv1 = s->fv1; I.fcount = s->fcount;
+    I.px = s->px; I.py = s->py; I.pvx = s->pvx; I.pvy = s->pvy;
+    I.pkey = s->pkey; I.counts = s->counts; I.starts = s->starts; I.cursor = s->cursor; I.sorted = s->sorted;
+    I.tile_sums = s->tile_sums; I.dg = s->dg; I.partials = s->partials;
+    I.ls_handoff = ls_handoff; I.sk_handoff = sk_handoff;
+    return I;
+}
+
+// Appends a block map for per-member workgroup counts `cnt` to H; member
+// ranges start at multiples of 8.  Returns the launch's workgroup count.
+struct MapOff {
+    size_t inst, first, count;
+};
+int add_map(std::vector<int>& H, const std::vector<int>& cnt, MapOff& off) {
+    const int n = (int)cnt.size();
+    std::vector<int> first(n);
+    int total = 0;
+    for (int b = 0; b < n; b++) {
+        total = (total + 7) & ~7;
+        first[b] = total;
+        total += cnt[b];
+    }
+    off.inst = H.size();
+    H.resize(H.size() + total, -1);
+    for (int b = 0; b < n; b++)
+        for (int k = 0; k < cnt[b]; k++) H[off.inst + first[b] + k] = b;
+    off.first = H.size();
+    H.insert(H.end(), first.begin(), first.end());
+    off.count = H.size();
+    H.insert(H.end(), cnt.begin(), cnt.end());
+    return total;
+}
+
+// Launch maps: every kernel's per-member workgroup count is the count of that
+// member's single launch (blocks_for / sk_blocks of the same sizes).
+fs2d_status build_maps(fs2d_batch* B) {
+    const int n = B->n;
+    std::vector<int> c_cells(n), c_faces(n), c_sl(n), c_part(n), c_diag(n), c_sk(n), c_tiles(n), c_one(n, 1);
+    std::vector<int> nstrips(n);
+    B->any_particles = false;
+    for (int b = 0; b < n; b++) {
+        fs2d_sim* s = B->sims[b];
+        const long nc = n_c(s), nf = n_u(s) + n_v(s);
+        c_cells[b] = blocks_for(nc);
+        c_faces[b] = blocks_for(nf);
+        c_sl[b] = blocks_for(nf + ((s->flags & FS2D_FLAG_DENSITY) ? nc : 0));
+        c_part[b] = s->np > 0 ? blocks_for(s->np) : 0;
+        c_diag[b] = blocks_for(nc + s->np);
+        c_sk[b] = sk_blocks(s);
+        c_tiles[b] = (int)((nc + 1023) / 1024);
+        nstrips[b] = (s->ny + 63) / 64;
+        B->any_particles = B->any_particles || s->np > 0;
+    }
+    std::vector<int> H;
+    MapOff o_cells, o_faces, o_sl, o_part, o_diag, o_sk, o_tiles, o_one;
+    B->g_cells = add_map(H, c_cells, o_cells);
+    B->g_faces = add_map(H, c_faces, o_faces);
+    B->g_sl = add_map(H, c_sl, o_sl);
+    B->g_part = add_map(H, c_part, o_part);
+    B->g_diag = add_map(H, c_diag, o_diag);
+    B->g_sk = add_map(H, c_sk, o_sk);
+    B->g_tiles = add_map(H, c_tiles, o_tiles);
+    B->g_one = add_map(H, c_one, o_one);
+    const size_t strip_inst = H.size();
+    int total = 0;
+    std::vector<int> sfirst(n);
+    for (int b = 0; b < n; b++) {
+        sfirst[b] = total;
+        for (int k = 0; k < nstrips[b]; k++) H.push_back(b);
+        total += nstrips[b];
+    }
+    const size_t strip_first = H.size();
+    H.insert(H.end(), sfirst.begin(), sfirst.end());
+    int* maps = nullptr;
+    TRY(balloc(B, &maps, H.size()));
+    BHIPCHK(B, hipMemcpyAsync(maps, H.data(), H.size() * sizeof(int), hipMemcpyHostToDevice, B->stream));
+    BHIPCHK(B, hipStreamSynchronize(B->stream));
+    B->d_maps = maps;
+    auto mk = [&](const MapOff& o) { return BlockMap{maps + o.inst, maps + o.first, maps + o.count}; };
+    B->m_cells = mk(o_cells); B->m_faces = mk(o_faces); B->m_sl = mk(o_sl); B->m_part = mk(o_part);
+    B->m_diag = mk(o_diag); B->m_sk = mk(o_sk); B->m_tiles = mk(o_tiles); B->m_one = mk(o_one);
+    B->strips = StripMap{maps + strip_inst, maps + strip_first, total};
+    return FS2D_OK;
+}
+
+// Re-reads members' buffer pointers (a single-simulation SL stage swaps them)
+// and uploads the descriptors if any changed.
+fs2d_status refresh_inst(fs2d_batch* B) {
+    bool changed = false, counts = false;
+    for (int b = 0; b < B->n; b++) {
+        InstBufs I = inst_bufs(B->sims[b], B->h_ins[b].ls_handoff, B->h_ins[b].sk_handoff);
+        if (memcmp(&I, &B->h_ins[b], sizeof I) != 0) {
+            counts = counts || I.np != B->h_ins[b].np;
+            B->h_ins[b] = I;
+            changed = true;
+        }
+    }
+    if (counts) TRY(build_maps(B));
+    if (changed) {
+        BHIPCHK(B, hipMemcpyAsync(B->d_ins, B->h_ins.data(), B->n * sizeof(InstBufs), hipMemcpyHostToDevice,
+                                  B->stream));
+        BHIPCHK(B, hipStreamSynchronize(B->stream));
+    }
+    return FS2D_OK;
+}
+
+template <class K, class... A>
+void blaunch(fs2d_batch* B, K kernel, int grid, int block, A... args) {
+    if (grid > 0) hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, B->stream, args...);
+}
+
+// ---- stages, in the order of fs2d_step ------------------------------------
+void bstage_sort(fs2d_batch* B) {
+    if (B->sorted_valid) return;
+    Timed t(B, K_PARTICLE_SORT);
+    blaunch(B, b_fill_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells, (int)BF_COUNTS);
+    if (B->any_particles) blaunch(B, b_particle_keys_kernel, B->g_part, kBlock, B->d_ins, B->m_part);
+    blaunch(B, b_scan_tiles_kernel, B->g_tiles, kBlock, B->d_ins, B->m_tiles);
+    blaunch(B, b_scan_sums_kernel, B->n, 64, B->d_ins, B->n);
+    blaunch(B, b_scan_add_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
+    if (B->any_particles) {
+        blaunch(B, b_copy_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells, (int)BC_CURSOR_FROM_STARTS);
+        blaunch(B, b_particle_scatter_kernel, B->g_part, kBlock, B->d_ins, B->m_part);
+        blaunch(B, b_segment_sort_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
+    }
+    B->sorted_valid = true;
+}
+
+void bstage_diagnostics(fs2d_batch* B) {
+    Timed t(B, K_DIAGNOSTICS);
+    blaunch(B, b_diagnostics_kernel, B->g_diag, kBlock, B->d_ins, B->m_diag);
+    blaunch(B, b_diag_finalize_kernel, B->n, kBlock, B->d_ins, B->n, B->m_diag);
+}
+
+void bstage_level_set(fs2d_batch* B) {
+    if (!(B->flags & FS2D_FLAG_STATIC_CELLS)) {
+        bstage_sort(B);
+        {
+            Timed t(B, K_LS_OTHER);
+            blaunch(B, b_ls_seed_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
+        }
+        const WaveCtl c{nullptr, B->ls_ctl, B->werr};
+        for (int k = 0; k < 16; k++) {
+            Timed t(B, K_LS_SWEEP);
+            blaunch(B, b_wavefront_kernel<ClosestOp, ClosestMaker>, B->strips.total, 64, B->d_ins, B->strips, c,
+                    ClosestMaker{k & 3, k & 1});
+        }
+        {
+            Timed t(B, K_LS_OTHER);
+            blaunch(B, b_ls_phi_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
+            blaunch(B, b_ls_surface_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
+        }
+        for (int k = 0; k < 16; k++) {
+            Timed t(B, K_LS_SWEEP);
+            blaunch(B, b_wavefront_kernel<EikonalOp, EikonalMaker>, B->strips.total, 64, B->d_ins, B->strips, c,
+                    EikonalMaker{k & 3, k & 1});
+        }
+        {
+            Timed t(B, K_LS_OTHER);
+            blaunch(B, b_ls_smooth_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells, 0);
+            blaunch(B, b_ls_smooth_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells, 1);
+            blaunch(B, b_classify_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
+        }
+    }
+    bstage_diagnostics(B);
+}
+
+void bstage_extrapolate(fs2d_batch* B, int which) {
+    Timed t(B, K_EXTRAPOLATE);
+    blaunch(B, b_fill_kernel, B->g_one, kBlock, B->d_ins, B->m_one, (int)BF_FCOUNT);
+    blaunch(B, b_extrap_layer1_kernel, B->g_faces, kBlock, B->d_ins, B->m_faces, which);
+    blaunch(B, b_extrap_layers_kernel, 2 * B->n, 1024, B->d_ins, B->n, which);
+}
+
+void bstage_p2g(fs2d_batch* B) {
+    bstage_sort(B);
+    {
+        Timed t(B, K_P2G);
+        blaunch(B, b_p2g_kernel, B->g_faces, kBlock, B->d_ins, B->m_faces);
+    }
+    bstage_extrapolate(B, 0);
+}
+
+void bstage_sl(fs2d_batch* B, bool with_gravity) {
+    Timed t(B, K_SL_ADVECT);
+    blaunch(B, b_sl_advect_kernel, B->g_sl, kBlock, B->d_ins, B->m_sl, with_gravity ? 1 : 0);
+    blaunch(B, b_copy_kernel, B->g_faces, kBlock, B->d_ins, B->m_faces, (int)BC_UV_FROM_U2);
+    if (B->flags & FS2D_FLAG_DENSITY)
+        blaunch(B, b_copy_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells, (int)BC_DENS_FROM_DENS2);
+}
+
+void bstage_gravity(fs2d_batch* B) {
+    Timed t(B, K_GRAVITY);
+    blaunch(B, b_gravity_kernel, B->g_faces, kBlock, B->d_ins, B->m_faces);
+}
+
+fs2d_status gather_status(fs2d_batch* B) {
+    blaunch(B, b_gather_status_kernel, (B->n + 255) / 256, 256, B->d_ins, B->n, B->d_status);
+    BHIPCHK(B, hipMemcpyAsync(B->h_status, B->d_status, 4 * B->n * sizeof(int), hipMemcpyDeviceToHost, B->stream));
+    BHIPCHK(B, hipStreamSynchronize(B->stream));
+    return FS2D_OK;
+}
+
+fs2d_status bstage_projection(fs2d_batch* B) {
+    blaunch(B, b_pcg_init_kernel, B->n, 64, B->d_ins, B->n, B->tol);
+    {
+        Timed t(B, K_ASSEMBLE);
+        blaunch(B, b_assemble_kernel, B->g_sk, kBlock, B->d_ins, B->m_sk);
+    }
+    const int S = B->strips.total;
+    {
+        Timed t(B, K_FACTOR);
+        blaunch(B, b_sweep_kernel<FactorOp, false, FactorMaker>, S, kSweepThreads, B->d_ins, B->strips,
+                B->sk_ticket, B->werr, FactorMaker{B->m_sk.count});
+    }
+    {
+        Timed t(B, K_PCG_FWD);
+        blaunch(B, b_sweep_kernel<FwdOp, false, FwdMaker>, S, kSweepThreads, B->d_ins, B->strips, B->sk_ticket,
+                B->werr, FwdMaker{B->m_sk.count, 0});
+    }
+    {
+        Timed t(B, K_PCG_BWD);
+        blaunch(B, b_sweep_kernel<BwdOp, true, BwdMaker>, S, kSweepThreads, B->d_ins, B->strips, B->sk_ticket,
+                B->werr, BwdMaker{-1});
+    }
+    int h = 0;
+    int chunk = std::max(2, std::min(B->last_iters_hint, B->max_iters));
+    int iters = 0;
+    bool hit = false;
+    while (h < B->max_iters) {
+        const int nchunk = std::min(chunk, B->max_iters - h);
+        for (int k = 0; k < nchunk; k++, h++) {
+            {
+                Timed t(B, K_PCG_SPMV);
+                blaunch(B, b_spmv_kernel, B->g_sk, kBlock, B->d_ins, B->m_sk, h);
+            }
+            {
+                Timed t(B, K_PCG_AXPY);
+                blaunch(B, b_axpy_kernel, B->g_sk, kBlock, B->d_ins, B->m_sk, h);
+            }
+            {
+                Timed t(B, K_PCG_FWD);
+                blaunch(B, b_sweep_kernel<FwdOp, false, FwdMaker>, S, kSweepThreads, B->d_ins, B->strips,
+                        B->sk_ticket, B->werr, FwdMaker{B->m_sk.count, 1});
+            }
+            {
+                Timed t(B, K_PCG_BWD);
+                blaunch(B, b_sweep_kernel<BwdOp, true, BwdMaker>, S, kSweepThreads, B->d_ins, B->strips,
+                        B->sk_ticket, B->werr, BwdMaker{h});
+            }
+        }
+        TRY(gather_status(B));
+        bool all = true;
+        for (int b = 0; b < B->n; b++) all = all && B->h_status[4 * b] != 0;
+        if (all) break;
+        chunk = 8;
+    }
+    {
+        Timed t(B, K_MISC);
+        blaunch(B, b_sk_to_rowmajor_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
+    }
+    TRY(gather_status(B));
+    for (int b = 0; b < B->n; b++) {
+        iters = std::max(iters, B->h_status[4 * b + 1]);
+        hit = hit || B->h_status[4 * b + 1] >= B->max_iters;
+        B->sims[b]->last_iters_hint = std::max(2, B->h_status[4 * b + 1] + 1);
+    }
+    B->last_iters_hint = std::max(2, iters + 1);
+    return hit ? FS2D_WARN_MAXITER : FS2D_OK;
+}
+
+void bstage_update_velocity(fs2d_batch* B) {
+    {
+        Timed t(B, K_UPDATE_VEL);
+        blaunch(B, b_update_velocity_kernel, B->g_faces, kBlock, B->d_ins, B->m_faces);
+    }
+    bstage_extrapolate(B, 1);
+    if (B->mode == FS2D_SEMILAGRANGIAN) {
+        Timed t(B, K_MISC);
+        blaunch(B, b_copy_kernel, B->g_faces, kBlock, B->d_ins, B->m_faces, (int)BC_UV_FROM_NEW);
+    }
+}
+
+void bstage_g2p(fs2d_batch* B) {
+    if (B->any_particles) {
+        Timed t(B, K_G2P);
+        blaunch(B, b_g2p_kernel, B->g_part, kBlock, B->d_ins, B->m_part);
+    }
+    Timed t(B, K_MISC);
+    blaunch(B, b_copy_kernel, B->g_faces, kBlock, B->d_ins, B->m_faces, (int)BC_UV_FROM_NEW);
+}
+
+void bstage_particle_advect(fs2d_batch* B) {
+    if (!B->any_particles) return;
+    Timed t(B, K_PARTICLE_ADVECT);
+    blaunch(B, b_particle_advect_kernel, B->g_part, kBlock, B->d_ins, B->m_part);
+    blaunch(B, b_cfl_finalize_kernel, B->n, kBlock, B->d_ins, B->n, B->m_part);
+    B->sorted_valid = false;
+}
+
+fs2d_status bstage(fs2d_batch* B, int stage, bool fuse_gravity, fs2d_status* worst) {
+    fs2d_status st = FS2D_OK;
+    switch (stage) {
+        case FS2D_STAGE_INIT: break;
+        case FS2D_STAGE_CREATE_WATER_LEVEL_SET: bstage_level_set(B); break;
+        case FS2D_STAGE_TRANSFER_VELOCITY_TO_GRID: bstage_p2g(B); break;
+        case FS2D_STAGE_SEMI_LAGRANGIAN_ADVECTION: bstage_sl(B, fuse_gravity); break;
+        case FS2D_STAGE_APPLY_GRAVITY: bstage_gravity(B); break;
+        case FS2D_STAGE_CREATE_SOLID_LEVEL_SET: break;
+        case FS2D_STAGE_APPLY_PROJECTION: st = bstage_projection(B); break;
+        case FS2D_STAGE_UPDATE_VELOCITY: bstage_update_velocity(B); break;
+        case FS2D_STAGE_UPDATE_PARTICLE_VELOCITIES: bstage_g2p(B); break;
+        case FS2D_STAGE_APPLY_ADVECTION: bstage_particle_advect(B); break;
+        default: return bfail(B, FS2D_ERR_ARG, "unknown stage %d", stage);
+    }
+    if (st < 0) return st;
+    if (st > *worst) *worst = st;
+    return FS2D_OK;
+}
+
+// One host round trip: NaN, hang and CFL status of every member.
+fs2d_status bfinish(fs2d_batch* B, fs2d_status st) {
+    TRY(gather_status(B));
+    BHIPCHK(B, hipMemcpyAsync(B->h_err, B->werr, sizeof(int), hipMemcpyDeviceToHost, B->stream));
+    BHIPCHK(B, hipStreamSynchronize(B->stream));
+    BHIPCHK(B, hipGetLastError());
+    if (B->timing) resolve_timing(B);
+    for (auto* s : B->sims) s->sorted_valid = false;
+    if (*B->h_err) return bfail(B, FS2D_ERR_HANG, "a wavefront hand-off wait expired");
+    for (int b = 0; b < B->n; b++)
+        if (B->h_status[4 * b + 2]) return bfail(B, FS2D_ERR_NAN, "member %d: advected particle position is NaN", b);
+    if (st == FS2D_WARN_MAXITER) {
+        bfail(B, st, "PCG: maximum iteration limit (%d) exceeded by a member", B->max_iters);
+        return st;
+    }
+    for (int b = 0; b < B->n; b++)
+        if (B->h_status[4 * b + 3]) {
+            bfail(B, FS2D_WARN_CFL, "member %d: particle velocity violates the CFL condition", b);
+            return FS2D_WARN_CFL;
+        }
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+fs2d_status fs2d_batch_create(const fs2d_config* cfgs, int n, fs2d_batch** out) {
+    if (!cfgs || !out || n < 1) return FS2D_ERR_ARG;
+    *out = nullptr;
+    for (int b = 1; b < n; b++) {
+        const fs2d_config &a = cfgs[0], &c = cfgs[b];
+        if (c.mode != a.mode || c.flags != a.flags || c.tau != a.tau || c.sigma != a.sigma ||
+            c.pcg_tol != a.pcg_tol || c.pcg_max_iters != a.pcg_max_iters) {
+            g_create_error = "batch members must share mode, flags and solver parameters (member " +
+                             std::to_string(b) + " differs from member 0)";
+            return FS2D_ERR_CONFIG;
+        }
+    }
+    fs2d_batch* B = new fs2d_batch();
+    B->n = n;
+    B->mode = cfgs[0].mode;
+    B->flags = cfgs[0].flags;
+    B->tol = cfgs[0].pcg_tol;
+    B->max_iters = cfgs[0].pcg_max_iters;
+    auto bail = [&](fs2d_status st) {
+        g_create_error = B->err;
+        fs2d_batch_destroy(B);
+        return st;
+    };
+    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) {
+        B->err = "HIP device unavailable";
+        return bail(FS2D_ERR_DEVICE);
+    }
+    for (int b = 0; b < n; b++) {
+        fs2d_sim* s = nullptr;
+        fs2d_status st = fs2d_create(&cfgs[b], &s);
+        if (st != FS2D_OK) {
+            B->err = "member " + std::to_string(b) + ": " + g_create_error;
+            return bail(st);
+        }
+        B->sims.push_back(s);
+    }
+    // hand-off rows and descriptors, then the launch maps
+    B->h_ins.resize(n);
+    fs2d_status st = FS2D_OK;
+    for (int b = 0; b < n; b++) {
+        fs2d_sim* s = B->sims[b];
+        const int ns = (s->ny + 63) / 64;
+        unsigned long long *lsh = nullptr, *skh = nullptr;
+        if ((st = balloc(B, &lsh, (size_t)ns * s->nx * 4)) != FS2D_OK) return bail(st);
+        if ((st = balloc(B, &skh, (size_t)ns * sk_handoff_stride(s->nx, 3))) != FS2D_OK) return bail(st);
+        B->h_ins[b] = inst_bufs(s, lsh, skh);
+    }
+    if ((st = balloc(B, &B->d_ins, (size_t)n)) != FS2D_OK) return bail(st);
+    if ((st = build_maps(B)) != FS2D_OK) return bail(st);
+    if ((st = balloc(B, &B->ls_ctl, 4)) != FS2D_OK || (st = balloc(B, &B->werr, 1)) != FS2D_OK ||
+        (st = balloc(B, &B->sk_ticket, 1)) != FS2D_OK || (st = balloc(B, &B->d_status, 4 * (size_t)n)) != FS2D_OK)
+        return bail(st);
+    if (hipHostMalloc((void**)&B->h_status, 4 * n * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&B->h_err, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        B->err = "hipHostMalloc failed";
+        return bail(FS2D_ERR_OOM);
+    }
+    const unsigned ctl0[4] = {0u, 0u, 1u, 0u};
+    if (hipMemcpyAsync(B->ls_ctl, ctl0, sizeof ctl0, hipMemcpyHostToDevice, B->stream) != hipSuccess ||
+        hipMemcpyAsync(B->d_ins, B->h_ins.data(), n * sizeof(InstBufs), hipMemcpyHostToDevice, B->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(B->stream) != hipSuccess) {
+        B->err = "batch upload failed";
+        return bail(FS2D_ERR_DEVICE);
+    }
+    *out = B;
+    return FS2D_OK;
+}
+
+void fs2d_batch_destroy(fs2d_batch* B) {
+    if (!B) return;
+    if (B->stream) (void)hipStreamSynchronize(B->stream);
+    for (auto* s : B->sims) fs2d_destroy(s);
+    for (void* p : B->allocs) (void)hipFree(p);
+    for (auto& t : B->pending) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    for (auto e : B->event_pool) (void)hipEventDestroy(e);
+    if (B->h_status) (void)hipHostFree(B->h_status);
+    if (B->h_err) (void)hipHostFree(B->h_err);
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+    delete B;
+}
+
+int fs2d_batch_size(const fs2d_batch* B) { return B ? B->n : 0; }
+
+fs2d_sim* fs2d_batch_sim(fs2d_batch* B, int i) { return (B && i >= 0 && i < B->n) ? B->sims[i] : nullptr; }
+
+fs2d_status fs2d_batch_step(fs2d_batch* B) {
+    if (!B) return FS2D_ERR_ARG;
+    TRY(refresh_inst(B));
+    B->sorted_valid = false;
+    fs2d_status worst = FS2D_OK, st;
+    if (B->mode == FS2D_SEMILAGRANGIAN) {
+        const int stages[] = {FS2D_STAGE_CREATE_WATER_LEVEL_SET, FS2D_STAGE_SEMI_LAGRANGIAN_ADVECTION,
+                              FS2D_STAGE_CREATE_SOLID_LEVEL_SET, FS2D_STAGE_APPLY_PROJECTION,
+                              FS2D_STAGE_UPDATE_VELOCITY, FS2D_STAGE_APPLY_ADVECTION};
+        for (int stg : stages)
+            if ((st = bstage(B, stg, true, &worst)) != FS2D_OK) return st;
+    } else {
+        const int stages[] = {FS2D_STAGE_CREATE_WATER_LEVEL_SET, FS2D_STAGE_TRANSFER_VELOCITY_TO_GRID,
+                              FS2D_STAGE_APPLY_GRAVITY, FS2D_STAGE_CREATE_SOLID_LEVEL_SET,
+                              FS2D_STAGE_APPLY_PROJECTION, FS2D_STAGE_UPDATE_VELOCITY,
+                              FS2D_STAGE_UPDATE_PARTICLE_VELOCITIES, FS2D_STAGE_APPLY_ADVECTION};
+        for (int stg : stages)
+            if ((st = bstage(B, stg, false, &worst)) != FS2D_OK) return st;
+    }
+    for (auto* s : B->sims) {
+        s->current_time += s->dt;
+        s->steps++;
+    }
+    return bfinish(B, worst);
+}
+
+fs2d_status fs2d_batch_run_stage(fs2d_batch* B, int stage) {
+    if (!B) return FS2D_ERR_ARG;
+    TRY(refresh_inst(B));
+    if (stage == FS2D_STAGE_CREATE_WATER_LEVEL_SET || stage == FS2D_STAGE_TRANSFER_VELOCITY_TO_GRID) {
+        // a member's particles may have been changed through its handle
+        bool valid = true;
+        for (auto* s : B->sims) valid = valid && s->sorted_valid;
+        if (!valid) B->sorted_valid = false;
+    }
+    fs2d_status worst = FS2D_OK;
+    fs2d_status st = bstage(B, stage, false, &worst);
+    if (st != FS2D_OK) return st;
+    const bool keep = B->sorted_valid;
+    st = bfinish(B, worst);
+    if (keep)
+        for (auto* s : B->sims) s->sorted_valid = true;
+    return st;
+}
+
+fs2d_status fs2d_batch_set_timing(fs2d_batch* B, int enable) {
+    if (!B) return FS2D_ERR_ARG;
+    BHIPCHK(B, hipStreamSynchronize(B->stream));
+    resolve_timing(B);
+    B->timing = enable != 0;
+    return FS2D_OK;
+}
+
+fs2d_status fs2d_batch_get_kernel_time(fs2d_batch* B, int k, double* total_ms, long long* launches) {
+    if (!B || k < 0 || k >= K_COUNT || !total_ms || !launches) return FS2D_ERR_ARG;
+    BHIPCHK(B, hipStreamSynchronize(B->stream));
+    resolve_timing(B);
+    *total_ms = B->ktime[k];
+    *launches = B->kcount[k];
+    return FS2D_OK;
+}
+
+fs2d_status fs2d_batch_reset_kernel_times(fs2d_batch* B) {
+    if (!B) return FS2D_ERR_ARG;
+    BHIPCHK(B, hipStreamSynchronize(B->stream));
+    resolve_timing(B);
+    for (int k = 0; k < K_COUNT; k++) {
+        B->ktime[k] = 0;
+        B->kcount[k] = 0;
+    }
+    return FS2D_OK;
+}
+
+const char* fs2d_batch_last_error(const fs2d_batch* B) { return B ? B->err.c_str() : g_create_error.c_str(); }
+
+}  // extern "C"

@@ -18,6 +18,7 @@
 
 #include "../../include/fs2d.h"
 #include "fs2d_adjoint.hpp"
+#include "fs2d_batch.hpp"
 
 using namespace fs2d;
 
@@ -126,7 +127,10 @@ fs2d_status fail(fs2d_sim* sim, fs2d_status st, const char* fmt, ...) {
                         hipGetErrorString(e_));                                             \
     } while (0)
 
-hipEvent_t get_event(fs2d_sim* s) {
+// Per-kernel timing state lives in the handle (fs2d_sim) or the batch
+// (fs2d_batch): members timing, stream, pending, event_pool, ktime, kcount.
+template <class H>
+hipEvent_t get_event(H* s) {
     if (!s->event_pool.empty()) {
         hipEvent_t e = s->event_pool.back();
         s->event_pool.pop_back();
@@ -137,12 +141,13 @@ hipEvent_t get_event(fs2d_sim* s) {
     return e;
 }
 
-// Times one launch on the handle's stream when timing is enabled.
+// Times one launch on the owner's stream when timing is enabled.
+template <class H>
 struct Timed {
-    fs2d_sim* s;
+    H* s;
     int cls;
     hipEvent_t a = nullptr;
-    Timed(fs2d_sim* s_, int c) : s(s_), cls(c) {
+    Timed(H* s_, int c) : s(s_), cls(c) {
         if (s->timing) {
             a = get_event(s);
             if (a) (void)hipEventRecord(a, s->stream);
@@ -161,7 +166,8 @@ struct Timed {
     }
 };
 
-void resolve_timing(fs2d_sim* s) {
+template <class H>
+void resolve_timing(H* s) {
     for (auto& t : s->pending) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
@@ -1135,3 +1141,4 @@ const char* fs2d_status_string(int st) {
 }  // extern "C"
 
 #include "fs2d_adjoint_api.hpp"
+#include "fs2d_batch_api.hpp"

@@ -84,14 +84,22 @@ __device__ __forceinline__ double max_partials(const double* partials, int n) {
 }
 
 template <class T>
-__global__ void fill_kernel(T* a, T v, size_t n) {
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+__device__ __forceinline__ void fill_body(T* a, T v, size_t n, int vb, int nvb) {
+    for (size_t e = vb * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)nvb * blockDim.x)
         a[e] = v;
 }
 template <class T>
-__global__ void copy_kernel(T* dst, const T* src, size_t n) {
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+__global__ void fill_kernel(T* a, T v, size_t n) {
+    fill_body(a, v, n, blockIdx.x, gridDim.x);
+}
+template <class T>
+__device__ __forceinline__ void copy_body(T* dst, const T* src, size_t n, int vb, int nvb) {
+    for (size_t e = vb * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)nvb * blockDim.x)
         dst[e] = src[e];
+}
+template <class T>
+__global__ void copy_kernel(T* dst, const T* src, size_t n) {
+    copy_body(dst, src, n, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -99,16 +107,16 @@ __global__ void copy_kernel(T* dst, const T* src, size_t n) {
 // applyGravity (:237-250) optionally fused into the store, and the passive
 // density (no reference counterpart) traced back from cell centres.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void sl_advect_kernel(MacView m, float* __restrict__ u_out,
+__device__ __forceinline__ void sl_advect_body(MacView m, float* __restrict__ u_out,
                                                            float* __restrict__ v_out,
                                                            const float* __restrict__ d_in,
                                                            float* __restrict__ d_out, double dt,
-                                                           ClampBox cb, float gu, float gv) {
+                                                           ClampBox cb, float gu, float gv, int vb, int nvb) {
     const int nx = m.nx, ny = m.ny;
     const long nu = (long)(nx + 1) * ny, nv = (long)nx * (ny + 1);
     const long nd = d_in ? (long)nx * ny : 0;
     const long n = nu + nv + nd;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    for (long e = vb * (long)blockDim.x + threadIdx.x; e < n; e += (long)nvb * blockDim.x) {
         double ox, oy;
         if (e < nu) {
             const int i = (int)(e % (nx + 1)), j = (int)(e / (nx + 1));
@@ -129,14 +137,25 @@ __global__ __launch_bounds__(kBlock) void sl_advect_kernel(MacView m, float* __r
         }
     }
 }
+__global__ __launch_bounds__(kBlock) void sl_advect_kernel(MacView m, float* __restrict__ u_out,
+                                                           float* __restrict__ v_out,
+                                                           const float* __restrict__ d_in,
+                                                           float* __restrict__ d_out, double dt,
+                                                           ClampBox cb, float gu, float gv) {
+    sl_advect_body(m, u_out, v_out, d_in, d_out, dt, cb, gu, gv, blockIdx.x, gridDim.x);
+}
 
 // FluidSim2D::applyGravity (src/FluidSim2D.cpp:237-250): every face.
-__global__ void gravity_kernel(float* __restrict__ u, float* __restrict__ v, long nu, long nv, float gu,
-                               float gv) {
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nu + nv; e += (long)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void gravity_body(float* __restrict__ u, float* __restrict__ v, long nu, long nv, float gu,
+                               float gv, int vb, int nvb) {
+    for (long e = vb * (long)blockDim.x + threadIdx.x; e < nu + nv; e += (long)nvb * blockDim.x) {
         if (e < nu) u[e] += gu;
         else v[e - nu] += gv;
     }
+}
+__global__ void gravity_kernel(float* __restrict__ u, float* __restrict__ v, long nu, long nv, float gu,
+                               float gv) {
+    gravity_body(u, v, nu, nv, gu, gv, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -147,14 +166,17 @@ __global__ void gravity_kernel(float* __restrict__ u, float* __restrict__ v, lon
 // grid-stride loop would fetch a cell's neighbours (one SK4 group, 256
 // elements, away) through another XCD's L2.  With a workgroup count that is a
 // multiple of 8, XCD x's workgroups share the contiguous range [x n/8,
-// (x+1) n/8) instead; otherwise this is a plain grid-stride loop.
+// (x+1) n/8) instead; otherwise this is a plain grid-stride loop.  vb / nvb
+// are the workgroup's index and count within its simulation (a batched launch
+// starts every simulation's workgroups at a multiple of 8, so vb % 8 is still
+// the XCD).
 struct XcdLoop {
     long e, end, stride;
 };
-__device__ __forceinline__ XcdLoop xcd_loop(long n) {
-    const int nb = (int)gridDim.x;
-    if (nb & 7) return {blockIdx.x * (long)blockDim.x + threadIdx.x, n, (long)nb * blockDim.x};
-    const int x = (int)blockIdx.x & 7, k = (int)blockIdx.x >> 3;
+__device__ __forceinline__ XcdLoop xcd_loop(long n, int vb, int nvb) {
+    const int nb = nvb;
+    if (nb & 7) return {vb * (long)blockDim.x + threadIdx.x, n, (long)nb * blockDim.x};
+    const int x = vb & 7, k = vb >> 3;
     const long chunk = (n + 7) / 8;
     const long b0 = x * chunk, b1 = b0 + chunk < n ? b0 + chunk : n;
     return {b0 + k * (long)blockDim.x + threadIdx.x, b1, (long)(nb >> 3) * blockDim.x};
@@ -175,13 +197,13 @@ __device__ __forceinline__ bool sk_cell(const SkGeom& g, long e, int& i, int& j)
 // diagonal, and the coupling bits of each FLUID cell (its off-diagonals are
 // -scaleA towards FLUID neighbours).  The PCG vectors p and s start at zero
 // (:424).  Partial |rhs|_inf per block.
-__global__ __launch_bounds__(kBlock) void assemble_sk_kernel(
+__device__ __forceinline__ void assemble_sk_body(
     SkGeom g, const uint8_t* __restrict__ cell, const double* __restrict__ phi, const float* __restrict__ u,
     const float* __restrict__ v, float* __restrict__ Adiag, uint8_t* __restrict__ flags, float* __restrict__ r,
-    float* __restrict__ p, float* __restrict__ s0, double scaleA, double scale, double* partials) {
+    float* __restrict__ p, float* __restrict__ s0, double scaleA, double scale, double* partials, int vb, int nvb) {
     const int nx = g.nx;
     double m = 0;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < g.size(); e += (long)gridDim.x * blockDim.x) {
+    for (long e = vb * (long)blockDim.x + threadIdx.x; e < g.size(); e += (long)nvb * blockDim.x) {
         int i, j;
         double ad = 0, rhs = 0;
         uint32_t f = 0;
@@ -216,7 +238,13 @@ __global__ __launch_bounds__(kBlock) void assemble_sk_kernel(
         m = a > m ? a : m;
     }
     double bm = block_max(m);
-    if (threadIdx.x == 0) partials[blockIdx.x] = bm;
+    if (threadIdx.x == 0) partials[vb] = bm;
+}
+__global__ __launch_bounds__(kBlock) void assemble_sk_kernel(
+    SkGeom g, const uint8_t* __restrict__ cell, const double* __restrict__ phi, const float* __restrict__ u,
+    const float* __restrict__ v, float* __restrict__ Adiag, uint8_t* __restrict__ flags, float* __restrict__ r,
+    float* __restrict__ p, float* __restrict__ s0, double scaleA, double scale, double* partials) {
+    assemble_sk_body(g, cell, phi, u, v, Adiag, flags, r, p, s0, scaleA, scale, partials, blockIdx.x, gridDim.x);
 }
 
 // Sum of a small partial array by one wave, broadcast to every thread of the
@@ -236,18 +264,18 @@ __device__ __forceinline__ double block_sum_bcast(const double* a, int n) {
 // w.s' (:450).  sigma_k = z.r comes from the forward solves' per-strip q.q
 // partials; beta = sigma_k / sigma_{k-1} (:457-458).  s is double-buffered
 // because every cell also needs s' of its four neighbours.
-__global__ __launch_bounds__(kBlock) void pcg_spmv_sk_kernel(
+__device__ __forceinline__ void pcg_spmv_sk_body(
     SkGeom g, const float* __restrict__ Adiag, const uint8_t* __restrict__ flags, const float* __restrict__ z,
     const float* __restrict__ s_old, float* __restrict__ s_new, float* __restrict__ w, float sA,
-    PcgScalars* sc, const double* sig_cur, const double* sig_prev, int first, double* partials) {
+    PcgScalars* sc, const double* sig_cur, const double* sig_prev, int first, double* partials, int vb, int nvb) {
     if (sc->done) return;
     const double sig = block_sum_bcast(sig_cur, g.S * kLoaders);
     double beta_d = 0;
     if (!first) beta_d = sig / block_sum_bcast(sig_prev, g.S * kLoaders);
-    if (blockIdx.x == 0 && threadIdx.x == 0) sc->sigma = sig;
+    if (vb == 0 && threadIdx.x == 0) sc->sigma = sig;
     const float beta = (float)beta_d;
     double acc = 0;
-    const XcdLoop it = xcd_loop(g.size());
+    const XcdLoop it = xcd_loop(g.size(), vb, nvb);
     for (long e = it.e; e < it.end; e += it.stride) {
         const float sc_ = fmaf(beta, s_old[e], z[e]);
         s_new[e] = sc_;
@@ -268,21 +296,27 @@ __global__ __launch_bounds__(kBlock) void pcg_spmv_sk_kernel(
         acc += (double)wv * (double)sc_;
     }
     double bs = block_sum(acc);
-    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+    if (threadIdx.x == 0) partials[vb] = bs;
+}
+__global__ __launch_bounds__(kBlock) void pcg_spmv_sk_kernel(
+    SkGeom g, const float* __restrict__ Adiag, const uint8_t* __restrict__ flags, const float* __restrict__ z,
+    const float* __restrict__ s_old, float* __restrict__ s_new, float* __restrict__ w, float sA,
+    PcgScalars* sc, const double* sig_cur, const double* sig_prev, int first, double* partials) {
+    pcg_spmv_sk_body(g, Adiag, flags, z, s_old, s_new, w, sA, sc, sig_cur, sig_prev, first, partials, blockIdx.x, gridDim.x);
 }
 
 // alpha = sigma / (w.s') (:450); p += alpha s', r -= alpha w (:451-452);
 // partial |r|_inf for the stopping test (:453), finished by the forward solve.
-__global__ __launch_bounds__(kBlock) void pcg_axpy_sk_kernel(
+__device__ __forceinline__ void pcg_axpy_sk_body(
     SkGeom g, float* __restrict__ p, float* __restrict__ r, const float* __restrict__ s,
     const float* __restrict__ w, const PcgScalars* sc, const double* sig_cur, const double* ws_partials,
-    int n_ws, double* partials) {
+    int n_ws, double* partials, int vb, int nvb) {
     if (sc->done) return;
     const double sig = block_sum_bcast(sig_cur, g.S * kLoaders);
     const double ws = block_sum_bcast(ws_partials, n_ws);
     const float a = (float)(sig / ws);
     float m = 0.0f;
-    const XcdLoop it = xcd_loop(g.size());
+    const XcdLoop it = xcd_loop(g.size(), vb, nvb);
     for (long e = it.e; e < it.end; e += it.stride) {
         p[e] = fmaf(a, s[e], p[e]);
         const float rv = fmaf(-a, w[e], r[e]);
@@ -290,17 +324,27 @@ __global__ __launch_bounds__(kBlock) void pcg_axpy_sk_kernel(
         m = fmaxf(m, fabsf(rv));
     }
     double bm = block_max((double)m);
-    if (threadIdx.x == 0) partials[blockIdx.x] = bm;
+    if (threadIdx.x == 0) partials[vb] = bm;
+}
+__global__ __launch_bounds__(kBlock) void pcg_axpy_sk_kernel(
+    SkGeom g, float* __restrict__ p, float* __restrict__ r, const float* __restrict__ s,
+    const float* __restrict__ w, const PcgScalars* sc, const double* sig_cur, const double* ws_partials,
+    int n_ws, double* partials) {
+    pcg_axpy_sk_body(g, p, r, s, w, sc, sig_cur, ws_partials, n_ws, partials, blockIdx.x, gridDim.x);
 }
 
 // SK4 -> row-major copy of the pressure.
-__global__ __launch_bounds__(kBlock) void sk_to_rowmajor_kernel(SkGeom g, const float* __restrict__ sk,
-                                                               float* __restrict__ rm) {
+__device__ __forceinline__ void sk_to_rowmajor_body(SkGeom g, const float* __restrict__ sk,
+                                                               float* __restrict__ rm, int vb, int nvb) {
     const long n = (long)g.nx * g.ny;
-    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+    for (long c = vb * (long)blockDim.x + threadIdx.x; c < n; c += (long)nvb * blockDim.x) {
         const int i = (int)(c % g.nx), j = (int)(c / g.nx);
         rm[c] = sk[g.idx(i, j)];
     }
+}
+__global__ __launch_bounds__(kBlock) void sk_to_rowmajor_kernel(SkGeom g, const float* __restrict__ sk,
+                                                               float* __restrict__ rm) {
+    sk_to_rowmajor_body(g, sk, rm, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -309,12 +353,12 @@ __global__ __launch_bounds__(kBlock) void sk_to_rowmajor_kernel(SkGeom g, const 
 // marked unknown (mask ~0) for extrapolation.  Faces u(nx, j) and v(i, ny)
 // lie outside the reference's loops and keep mac's value (mask 0).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void update_velocity_kernel(
+__device__ __forceinline__ void update_velocity_body(
     const uint8_t* __restrict__ cell, const double* __restrict__ phi, const float* __restrict__ p,
     const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ nu, float* __restrict__ nv,
-    uint32_t* __restrict__ mu, uint32_t* __restrict__ mv, int nx, int ny, double scale) {
+    uint32_t* __restrict__ mu, uint32_t* __restrict__ mv, int nx, int ny, double scale, int vb, int nvb) {
     const long nU = (long)(nx + 1) * ny, nV = (long)nx * (ny + 1);
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nU + nV; e += (long)gridDim.x * blockDim.x) {
+    for (long e = vb * (long)blockDim.x + threadIdx.x; e < nU + nV; e += (long)nvb * blockDim.x) {
         if (e < nU) {
             const int i = (int)(e % (nx + 1)), j = (int)(e / (nx + 1));
             double val = u[e];
@@ -363,6 +407,12 @@ __global__ __launch_bounds__(kBlock) void update_velocity_kernel(
         }
     }
 }
+__global__ __launch_bounds__(kBlock) void update_velocity_kernel(
+    const uint8_t* __restrict__ cell, const double* __restrict__ phi, const float* __restrict__ p,
+    const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ nu, float* __restrict__ nv,
+    uint32_t* __restrict__ mu, uint32_t* __restrict__ mv, int nx, int ny, double scale) {
+    update_velocity_body(cell, phi, p, u, v, nu, nv, mu, mv, nx, ny, scale, blockIdx.x, gridDim.x);
+}
 
 // ---------------------------------------------------------------------------
 // Extrapolation (Array2D<double>::extrapolate, include/Array2D.h:552-591).
@@ -380,9 +430,9 @@ struct ExtraArr {
     int w, h;
 };
 
-__global__ __launch_bounds__(kBlock) void extrap_layer1_kernel(ExtraArr A, ExtraArr B) {
+__device__ __forceinline__ void extrap_layer1_body(ExtraArr A, ExtraArr B, int vb, int nvb) {
     const long nA = (long)A.w * A.h, nB = (long)B.w * B.h;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nA + nB; e += (long)gridDim.x * blockDim.x) {
+    for (long e = vb * (long)blockDim.x + threadIdx.x; e < nA + nB; e += (long)nvb * blockDim.x) {
         const ExtraArr& X = e < nA ? A : B;
         const long f = e < nA ? e : e - nA;
         const int i = (int)(f % X.w), j = (int)(f / X.w);
@@ -397,9 +447,12 @@ __global__ __launch_bounds__(kBlock) void extrap_layer1_kernel(ExtraArr A, Extra
         }
     }
 }
+__global__ __launch_bounds__(kBlock) void extrap_layer1_kernel(ExtraArr A, ExtraArr B) {
+    extrap_layer1_body(A, B, blockIdx.x, gridDim.x);
+}
 
-__global__ __launch_bounds__(1024) void extrap_layers_kernel(ExtraArr A, ExtraArr B) {
-    ExtraArr X = blockIdx.x == 0 ? A : B;
+__device__ __forceinline__ void extrap_layers_body(ExtraArr A, ExtraArr B, int vb, int nvb) {
+    ExtraArr X = vb == 0 ? A : B;
     __shared__ int n_cur, n_next;
     if (threadIdx.x == 0) {
         n_cur = *X.count;
@@ -446,6 +499,9 @@ __global__ __launch_bounds__(1024) void extrap_layers_kernel(ExtraArr A, ExtraAr
         nxt = t;
     }
 }
+__global__ __launch_bounds__(1024) void extrap_layers_kernel(ExtraArr A, ExtraArr B) {
+    extrap_layers_body(A, B, blockIdx.x, gridDim.x);
+}
 
 // ---------------------------------------------------------------------------
 // Particles.  Cell keys use the reference's truncation (int)(x/dx)
@@ -455,11 +511,11 @@ __global__ __launch_bounds__(1024) void extrap_layers_kernel(ExtraArr A, ExtraAr
 // particles by key produces, and the order in which the reference's serial
 // loops meet the particles of one cell.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void particle_keys_kernel(const float* __restrict__ px,
+__device__ __forceinline__ void particle_keys_body(const float* __restrict__ px,
                                                                const float* __restrict__ py, long np,
                                                                double dx, int nx, int ny, int* __restrict__ key,
-                                                               int* __restrict__ counts) {
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < np; e += (long)gridDim.x * blockDim.x) {
+                                                               int* __restrict__ counts, int vb, int nvb) {
+    for (long e = vb * (long)blockDim.x + threadIdx.x; e < np; e += (long)nvb * blockDim.x) {
         int x = (int)((double)px[e] / dx), y = (int)((double)py[e] / dx);
         x = clampi(x, 0, nx - 1);
         y = clampi(y, 0, ny - 1);
@@ -468,12 +524,18 @@ __global__ __launch_bounds__(kBlock) void particle_keys_kernel(const float* __re
         atomicAdd(&counts[k], 1);
     }
 }
+__global__ __launch_bounds__(kBlock) void particle_keys_kernel(const float* __restrict__ px,
+                                                               const float* __restrict__ py, long np,
+                                                               double dx, int nx, int ny, int* __restrict__ key,
+                                                               int* __restrict__ counts) {
+    particle_keys_body(px, py, np, dx, nx, ny, key, counts, blockIdx.x, gridDim.x);
+}
 
 // Exclusive scan in three passes over 1024-element tiles.
-__global__ __launch_bounds__(kBlock) void scan_tiles_kernel(const int* __restrict__ in, int* __restrict__ out,
-                                                            int* __restrict__ tile_sums, long n) {
+__device__ __forceinline__ void scan_tiles_body(const int* __restrict__ in, int* __restrict__ out,
+                                                            int* __restrict__ tile_sums, long n, int vb, int nvb) {
     __shared__ int sh[1024];
-    const long base = (long)blockIdx.x * 1024;
+    const long base = (long)vb * 1024;
     for (int k = threadIdx.x; k < 1024; k += kBlock) sh[k] = base + k < n ? in[base + k] : 0;
     __syncthreads();
     if (threadIdx.x == 0) {  // 1024 adds: small next to the memory traffic of the tile
@@ -483,14 +545,18 @@ __global__ __launch_bounds__(kBlock) void scan_tiles_kernel(const int* __restric
             sh[k] = run;
             run += v;
         }
-        tile_sums[blockIdx.x] = run;
+        tile_sums[vb] = run;
     }
     __syncthreads();
     for (int k = threadIdx.x; k < 1024; k += kBlock)
         if (base + k < n) out[base + k] = sh[k];
 }
-__global__ void scan_sums_kernel(int* tile_sums, int ntiles) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+__global__ __launch_bounds__(kBlock) void scan_tiles_kernel(const int* __restrict__ in, int* __restrict__ out,
+                                                            int* __restrict__ tile_sums, long n) {
+    scan_tiles_body(in, out, tile_sums, n, blockIdx.x, gridDim.x);
+}
+__device__ __forceinline__ void scan_sums_body(int* tile_sums, int ntiles, int vb, int nvb) {
+    if (threadIdx.x == 0 && vb == 0) {
         int run = 0;
         for (int k = 0; k < ntiles; k++) {
             int v = tile_sums[k];
@@ -499,24 +565,36 @@ __global__ void scan_sums_kernel(int* tile_sums, int ntiles) {
         }
     }
 }
-__global__ __launch_bounds__(kBlock) void scan_add_kernel(int* __restrict__ out, const int* __restrict__ tile_sums,
-                                                          long n) {
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+__global__ void scan_sums_kernel(int* tile_sums, int ntiles) {
+    scan_sums_body(tile_sums, ntiles, blockIdx.x, gridDim.x);
+}
+__device__ __forceinline__ void scan_add_body(int* __restrict__ out, const int* __restrict__ tile_sums,
+                                                          long n, int vb, int nvb) {
+    for (long e = vb * (long)blockDim.x + threadIdx.x; e < n; e += (long)nvb * blockDim.x)
         out[e] += tile_sums[e / 1024];
 }
+__global__ __launch_bounds__(kBlock) void scan_add_kernel(int* __restrict__ out, const int* __restrict__ tile_sums,
+                                                          long n) {
+    scan_add_body(out, tile_sums, n, blockIdx.x, gridDim.x);
+}
 
-__global__ __launch_bounds__(kBlock) void particle_scatter_kernel(const int* __restrict__ key, long np,
+__device__ __forceinline__ void particle_scatter_body(const int* __restrict__ key, long np,
                                                                   int* __restrict__ cursor,
-                                                                  int* __restrict__ sorted) {
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < np; e += (long)gridDim.x * blockDim.x) {
+                                                                  int* __restrict__ sorted, int vb, int nvb) {
+    for (long e = vb * (long)blockDim.x + threadIdx.x; e < np; e += (long)nvb * blockDim.x) {
         int slot = atomicAdd(&cursor[key[e]], 1);
         sorted[slot] = (int)e;
     }
 }
-__global__ __launch_bounds__(kBlock) void segment_sort_kernel(const int* __restrict__ start,
+__global__ __launch_bounds__(kBlock) void particle_scatter_kernel(const int* __restrict__ key, long np,
+                                                                  int* __restrict__ cursor,
+                                                                  int* __restrict__ sorted) {
+    particle_scatter_body(key, np, cursor, sorted, blockIdx.x, gridDim.x);
+}
+__device__ __forceinline__ void segment_sort_body(const int* __restrict__ start,
                                                               const int* __restrict__ counts,
-                                                              int* __restrict__ sorted, long ncells) {
-    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < ncells; c += (long)gridDim.x * blockDim.x) {
+                                                              int* __restrict__ sorted, long ncells, int vb, int nvb) {
+    for (long c = vb * (long)blockDim.x + threadIdx.x; c < ncells; c += (long)nvb * blockDim.x) {
         const int s0 = start[c], n = counts[c];
         for (int a = 1; a < n; a++) {
             int v = sorted[s0 + a], b = a - 1;
@@ -528,19 +606,24 @@ __global__ __launch_bounds__(kBlock) void segment_sort_kernel(const int* __restr
         }
     }
 }
+__global__ __launch_bounds__(kBlock) void segment_sort_kernel(const int* __restrict__ start,
+                                                              const int* __restrict__ counts,
+                                                              int* __restrict__ sorted, long ncells) {
+    segment_sort_body(start, counts, sorted, ncells, blockIdx.x, gridDim.x);
+}
 
 // FluidSim2D::transferVelocityToGrid (src/FluidSim2D.cpp:144-204) as a gather:
 // every face sums the tent weights (Array2D::linearDistribute, Array2D.h:361-376)
 // of the particles of the cells that can reach it, divides by the weight sum,
 // and faces whose value is exactly 0 become unknown (:192-201).
-__global__ __launch_bounds__(kBlock) void p2g_kernel(const float* __restrict__ px, const float* __restrict__ py,
+__device__ __forceinline__ void p2g_body(const float* __restrict__ px, const float* __restrict__ py,
                                                      const float* __restrict__ pvx, const float* __restrict__ pvy,
                                                      const int* __restrict__ start, const int* __restrict__ counts,
                                                      const int* __restrict__ sorted, float* __restrict__ u,
                                                      float* __restrict__ v, uint32_t* __restrict__ mu,
-                                                     uint32_t* __restrict__ mv, int nx, int ny, double dx) {
+                                                     uint32_t* __restrict__ mv, int nx, int ny, double dx, int vb, int nvb) {
     const long nU = (long)(nx + 1) * ny, nV = (long)nx * (ny + 1);
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nU + nV; e += (long)gridDim.x * blockDim.x) {
+    for (long e = vb * (long)blockDim.x + threadIdx.x; e < nU + nV; e += (long)nvb * blockDim.x) {
         const bool isU = e < nU;
         const long f = isU ? e : e - nU;
         const int W = isU ? nx + 1 : nx, H = isU ? ny : ny + 1;
@@ -591,6 +674,14 @@ __global__ __launch_bounds__(kBlock) void p2g_kernel(const float* __restrict__ p
         }
     }
 }
+__global__ __launch_bounds__(kBlock) void p2g_kernel(const float* __restrict__ px, const float* __restrict__ py,
+                                                     const float* __restrict__ pvx, const float* __restrict__ pvy,
+                                                     const int* __restrict__ start, const int* __restrict__ counts,
+                                                     const int* __restrict__ sorted, float* __restrict__ u,
+                                                     float* __restrict__ v, uint32_t* __restrict__ mu,
+                                                     uint32_t* __restrict__ mv, int nx, int ny, double dx) {
+    p2g_body(px, py, pvx, pvy, start, counts, sorted, u, v, mu, mv, nx, ny, dx, blockIdx.x, gridDim.x);
+}
 
 // FluidSim2D::updateParticleVelocities (src/FluidSim2D.cpp:552-568): PIC/FLIP
 // blend of bilinear samples of newMac and of (newMac - mac).
@@ -607,12 +698,12 @@ __device__ __forceinline__ void bilinear_pair(const float* __restrict__ a_new, c
            (a_new[i22] - a_old[i22]) * w22;
 }
 
-__global__ __launch_bounds__(kBlock) void g2p_kernel(const float* __restrict__ px, const float* __restrict__ py,
+__device__ __forceinline__ void g2p_body(const float* __restrict__ px, const float* __restrict__ py,
                                                      float* __restrict__ pvx, float* __restrict__ pvy, long np,
                                                      const float* __restrict__ nu, const float* __restrict__ nv,
                                                      const float* __restrict__ u, const float* __restrict__ v,
-                                                     int nx, int ny, double dx, float alpha) {
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < np; e += (long)gridDim.x * blockDim.x) {
+                                                     int nx, int ny, double dx, float alpha, int vb, int nvb) {
+    for (long e = vb * (long)blockDim.x + threadIdx.x; e < np; e += (long)nvb * blockDim.x) {
         const double gx = (double)px[e] / dx, gy = (double)py[e] / dx;
         float picu, du, picv, dv;
         bilinear_pair(nu, u, nx + 1, ny, gx, gy - 0.5, picu, du);
@@ -622,17 +713,24 @@ __global__ __launch_bounds__(kBlock) void g2p_kernel(const float* __restrict__ p
         pvy[e] = alpha * picv + (1 - alpha) * fv;
     }
 }
+__global__ __launch_bounds__(kBlock) void g2p_kernel(const float* __restrict__ px, const float* __restrict__ py,
+                                                     float* __restrict__ pvx, float* __restrict__ pvy, long np,
+                                                     const float* __restrict__ nu, const float* __restrict__ nv,
+                                                     const float* __restrict__ u, const float* __restrict__ v,
+                                                     int nx, int ny, double dx, float alpha) {
+    g2p_body(px, py, pvx, pvy, np, nu, nv, u, v, nx, ny, dx, alpha, blockIdx.x, gridDim.x);
+}
 
 // FluidSim2D::applyAdvection (src/FluidSim2D.cpp:570-604): CFL scan, forward
 // Ralston RK3 through the Catmull-Rom field, clampPos, NaN flag (on the
 // position before the move, as the reference checks it).
-__global__ __launch_bounds__(kBlock) void particle_advect_kernel(float* __restrict__ px, float* __restrict__ py,
+__device__ __forceinline__ void particle_advect_body(float* __restrict__ px, float* __restrict__ py,
                                                                  const float* __restrict__ pvx,
                                                                  const float* __restrict__ pvy, long np, MacView m,
-                                                                 double dt, ClampBox cb, Diag* dg, double* partials) {
+                                                                 double dt, ClampBox cb, Diag* dg, double* partials, int vb, int nvb) {
     double cmax = 0;
     int nan = 0;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < np; e += (long)gridDim.x * blockDim.x) {
+    for (long e = vb * (long)blockDim.x + threadIdx.x; e < np; e += (long)nvb * blockDim.x) {
         const double c = ((double)pvx[e] + (double)pvy[e]) * dt / m.dx;
         cmax = c > cmax ? c : cmax;
         const double x = px[e], y = py[e];
@@ -657,15 +755,24 @@ __global__ __launch_bounds__(kBlock) void particle_advect_kernel(float* __restri
     if (nan) atomicOr(&sh_nan, 1);
     __syncthreads();
     if (threadIdx.x == 0) {
-        partials[blockIdx.x] = bm;
+        partials[vb] = bm;
         if (sh_nan) atomicOr(&dg->nan_flag, 1);
     }
 }
+__global__ __launch_bounds__(kBlock) void particle_advect_kernel(float* __restrict__ px, float* __restrict__ py,
+                                                                 const float* __restrict__ pvx,
+                                                                 const float* __restrict__ pvy, long np, MacView m,
+                                                                 double dt, ClampBox cb, Diag* dg, double* partials) {
+    particle_advect_body(px, py, pvx, pvy, np, m, dt, cb, dg, partials, blockIdx.x, gridDim.x);
+}
 
 // Max of the per-block CFL partials (one block).
-__global__ __launch_bounds__(kBlock) void cfl_finalize_kernel(const double* partials, int n, Diag* dg) {
+__device__ __forceinline__ void cfl_finalize_body(const double* partials, int n, Diag* dg, int vb, int nvb) {
     double t = max_partials(partials, n);
     if (threadIdx.x == 0) dg->cfl_max = t;
+}
+__global__ __launch_bounds__(kBlock) void cfl_finalize_kernel(const double* partials, int n, Diag* dg) {
+    cfl_finalize_body(partials, n, dg, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -681,11 +788,11 @@ __device__ __forceinline__ double corner_dist(int i, int j, double x, double y, 
 
 // Scatter-min of constructFromParticles (:764-773): the first particle (in
 // index order) with the smallest distance to the cell's corner wins.
-__global__ __launch_bounds__(kBlock) void ls_seed_kernel(const float* __restrict__ px, const float* __restrict__ py,
+__device__ __forceinline__ void ls_seed_body(const float* __restrict__ px, const float* __restrict__ py,
                                                          const int* __restrict__ start, const int* __restrict__ counts,
                                                          const int* __restrict__ sorted, int* __restrict__ t,
-                                                         int nx, long ncells, double dx, double dr) {
-    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < ncells; c += (long)gridDim.x * blockDim.x) {
+                                                         int nx, long ncells, double dx, double dr, int vb, int nvb) {
+    for (long c = vb * (long)blockDim.x + threadIdx.x; c < ncells; c += (long)nvb * blockDim.x) {
         const int i = (int)(c % nx), j = (int)(c / nx);
         double best = HUGE_VAL;
         int bt = -1;
@@ -700,6 +807,12 @@ __global__ __launch_bounds__(kBlock) void ls_seed_kernel(const float* __restrict
         }
         t[c] = bt;
     }
+}
+__global__ __launch_bounds__(kBlock) void ls_seed_kernel(const float* __restrict__ px, const float* __restrict__ py,
+                                                         const int* __restrict__ start, const int* __restrict__ counts,
+                                                         const int* __restrict__ sorted, int* __restrict__ t,
+                                                         int nx, long ncells, double dx, double dr) {
+    ls_seed_body(px, py, start, counts, sorted, t, nx, ncells, dx, dr, blockIdx.x, gridDim.x);
 }
 
 // One directional Gauss-Seidel sweep of the closest-particle propagation
@@ -775,19 +888,24 @@ struct ClosestOp {
 
 // phi from the adopted particle; then the surface marks (:803-834) and the
 // interior fill (:836-842) of redistance, reading phi of the finished sweeps.
-__global__ __launch_bounds__(kBlock) void ls_phi_kernel(const int* __restrict__ t, const float* __restrict__ px,
+__device__ __forceinline__ void ls_phi_body(const int* __restrict__ t, const float* __restrict__ px,
                                                         const float* __restrict__ py, double* __restrict__ phi,
-                                                        int nx, long ncells, double dx, double dr) {
-    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < ncells; c += (long)gridDim.x * blockDim.x) {
+                                                        int nx, long ncells, double dx, double dr, int vb, int nvb) {
+    for (long c = vb * (long)blockDim.x + threadIdx.x; c < ncells; c += (long)nvb * blockDim.x) {
         const int i = (int)(c % nx), j = (int)(c / nx);
         const int e = t[c];
         phi[c] = e >= 0 ? corner_dist(i, j, px[e], py[e], dx, dr) : HUGE_VAL;
     }
 }
-__global__ __launch_bounds__(kBlock) void ls_surface_kernel(const double* __restrict__ phi, double* __restrict__ out,
-                                                            int nx, int ny) {
+__global__ __launch_bounds__(kBlock) void ls_phi_kernel(const int* __restrict__ t, const float* __restrict__ px,
+                                                        const float* __restrict__ py, double* __restrict__ phi,
+                                                        int nx, long ncells, double dx, double dr) {
+    ls_phi_body(t, px, py, phi, nx, ncells, dx, dr, blockIdx.x, gridDim.x);
+}
+__device__ __forceinline__ void ls_surface_body(const double* __restrict__ phi, double* __restrict__ out,
+                                                            int nx, int ny, int vb, int nvb) {
     const long n = (long)nx * ny;
-    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+    for (long c = vb * (long)blockDim.x + threadIdx.x; c < n; c += (long)nvb * blockDim.x) {
         const int i = (int)(c % nx), j = (int)(c / nx);
         const double p0 = phi[c];
         bool surf = false;
@@ -797,6 +915,10 @@ __global__ __launch_bounds__(kBlock) void ls_surface_kernel(const double* __rest
         if (j >= 1 && i < nx - 1) surf = surf || (phi[c - nx] * p0 < 0);
         out[c] = (!surf && p0 < 0) ? -HUGE_VAL : p0;
     }
+}
+__global__ __launch_bounds__(kBlock) void ls_surface_kernel(const double* __restrict__ phi, double* __restrict__ out,
+                                                            int nx, int ny) {
+    ls_surface_body(phi, out, nx, ny, blockIdx.x, gridDim.x);
 }
 
 // One directional sweep of the eikonal update (:845-902) on cells with phi < 0,
@@ -831,11 +953,11 @@ struct EikonalOp {
 };
 
 // Smoothing pass phi = min(phi, mean of 4 neighbours) on 1..n-2 (:904-919).
-__global__ __launch_bounds__(kBlock) void ls_smooth_kernel(const double* __restrict__ in, double* __restrict__ out,
-                                                           int nx, int ny) {
+__device__ __forceinline__ void ls_smooth_body(const double* __restrict__ in, double* __restrict__ out,
+                                                           int nx, int ny, int vb, int nvb) {
 #pragma clang fp contract(off)
     const long n = (long)nx * ny;
-    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+    for (long c = vb * (long)blockDim.x + threadIdx.x; c < n; c += (long)nvb * blockDim.x) {
         const int i = (int)(c % nx), j = (int)(c / nx);
         double v = in[c];
         if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2) {
@@ -845,26 +967,33 @@ __global__ __launch_bounds__(kBlock) void ls_smooth_kernel(const double* __restr
         out[c] = v;
     }
 }
+__global__ __launch_bounds__(kBlock) void ls_smooth_kernel(const double* __restrict__ in, double* __restrict__ out,
+                                                           int nx, int ny) {
+    ls_smooth_body(in, out, nx, ny, blockIdx.x, gridDim.x);
+}
 
 // Classification (:657-664): non-solid cells are FLUID iff phi < 0.
-__global__ void classify_kernel(uint8_t* __restrict__ cell, const double* __restrict__ phi, long n) {
-    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x)
+__device__ __forceinline__ void classify_body(uint8_t* __restrict__ cell, const double* __restrict__ phi, long n, int vb, int nvb) {
+    for (long c = vb * (long)blockDim.x + threadIdx.x; c < n; c += (long)nvb * blockDim.x)
         if (cell[c] != SOLID) cell[c] = phi[c] < 0.0 ? FLUID : EMPTY;
+}
+__global__ void classify_kernel(uint8_t* __restrict__ cell, const double* __restrict__ phi, long n) {
+    classify_body(cell, phi, n, blockIdx.x, gridDim.x);
 }
 
 // Diagnostics (:709-731): volume, grid energy (Catmull-Rom velocity at cell
 // corners), particle energy; FP64 sums.
-__global__ __launch_bounds__(kBlock) void diagnostics_kernel(const uint8_t* __restrict__ cell, MacView m,
+__device__ __forceinline__ void diagnostics_body(const uint8_t* __restrict__ cell, MacView m,
                                                              const float* __restrict__ px,
                                                              const float* __restrict__ py,
                                                              const float* __restrict__ pvx,
                                                              const float* __restrict__ pvy, long np, double rho,
                                                              double gx, double gy, double pmass,
-                                                             double* partials) {
+                                                             double* partials, int vb, int nvb) {
     const long nc = (long)m.nx * m.ny;
     const double dx = m.dx, cm = rho * dx * dx;
     double vol = 0, eg = 0, ep = 0;
-    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < nc + np; c += (long)gridDim.x * blockDim.x) {
+    for (long c = vb * (long)blockDim.x + threadIdx.x; c < nc + np; c += (long)nvb * blockDim.x) {
         if (c < nc) {
             if (cell[c] != FLUID) continue;
             const int i = (int)(c % m.nx), j = (int)(c / m.nx);
@@ -882,14 +1011,23 @@ __global__ __launch_bounds__(kBlock) void diagnostics_kernel(const uint8_t* __re
     }
     double a = block_sum(vol), b = block_sum(eg), c2 = block_sum(ep);
     if (threadIdx.x == 0) {
-        partials[3 * blockIdx.x + 0] = a;
-        partials[3 * blockIdx.x + 1] = b;
-        partials[3 * blockIdx.x + 2] = c2;
+        partials[3 * vb + 0] = a;
+        partials[3 * vb + 1] = b;
+        partials[3 * vb + 2] = c2;
     }
+}
+__global__ __launch_bounds__(kBlock) void diagnostics_kernel(const uint8_t* __restrict__ cell, MacView m,
+                                                             const float* __restrict__ px,
+                                                             const float* __restrict__ py,
+                                                             const float* __restrict__ pvx,
+                                                             const float* __restrict__ pvy, long np, double rho,
+                                                             double gx, double gy, double pmass,
+                                                             double* partials) {
+    diagnostics_body(cell, m, px, py, pvx, pvy, np, rho, gx, gy, pmass, partials, blockIdx.x, gridDim.x);
 }
 
 // Sums of the per-block diagnostics partials in index order (one block).
-__global__ __launch_bounds__(kBlock) void diag_finalize_kernel(const double* partials, int n, double dx, Diag* dg) {
+__device__ __forceinline__ void diag_finalize_body(const double* partials, int n, double dx, Diag* dg, int vb, int nvb) {
     double s0 = 0, s1 = 0, s2 = 0;
     for (int k = threadIdx.x; k < n; k += kBlock) {
         s0 += partials[3 * k];
@@ -904,6 +1042,9 @@ __global__ __launch_bounds__(kBlock) void diag_finalize_kernel(const double* par
         dg->grid_energy = s1;
         dg->particle_energy = s2;
     }
+}
+__global__ __launch_bounds__(kBlock) void diag_finalize_kernel(const double* partials, int n, double dx, Diag* dg) {
+    diag_finalize_body(partials, n, dx, dg, blockIdx.x, gridDim.x);
 }
 
 }  // namespace fs2d

@@ -293,24 +293,30 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
     op.strip_done(s);
 }
 
-template <class Op, bool BACKWARD>
-__global__ __launch_bounds__(kSweepThreads) void sweep_kernel(Op op, SkGeom g, SkCtl c) {
-    constexpr int NW = Op::NW, NIN = Op::NIN, NF = Op::NF, NB = Op::NB;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __shared__ __align__(16) SweepLds<Op> L;
+// Workgroup prologue: take a ticket and clear the LDS counters.
+template <class Op>
+__device__ __forceinline__ unsigned long long sweep_ticket(SweepLds<Op>& L, unsigned long long* ticket) {
     __shared__ unsigned long long sh_ticket;
-    __shared__ int sh_skip;
-
     if (threadIdx.x == 0) {
-        sh_ticket = atomicAdd(c.ticket, 1ull);
+        sh_ticket = atomicAdd(ticket, 1ull);
         L.consumed_sg = 0;
         L.hand_ready = 0;
         L.abort_flag = 0;
     }
     if (threadIdx.x < kRing) L.slot_tag[threadIdx.x] = 0;
     __syncthreads();
-    const int order = (int)(sh_ticket % (unsigned long long)g.S);
-    const unsigned epoch = (unsigned)(sh_ticket / (unsigned long long)g.S) + 1u;
+    return sh_ticket;
+}
+
+// One strip of a pass: `order` is the strip's position in the pass (strip
+// order for forward passes, reversed for backward ones), `epoch` tags its
+// hand-off words.
+template <class Op, bool BACKWARD>
+__device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& c, int order, unsigned epoch,
+                                          SweepLds<Op>& L) {
+    constexpr int NW = Op::NW, NIN = Op::NIN, NF = Op::NF, NB = Op::NB;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ int sh_skip;
     const int s = BACKWARD ? g.S - 1 - order : order;
     if (wave == 0) {
         int sk = __hip_atomic_load(c.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
@@ -450,6 +456,15 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(Op op, SkGeom g, S
             }
         }
     }
+}
+
+template <class Op, bool BACKWARD>
+__global__ __launch_bounds__(kSweepThreads) void sweep_kernel(Op op, SkGeom g, SkCtl c) {
+    __shared__ __align__(16) SweepLds<Op> L;
+    const unsigned long long ticket = sweep_ticket(L, c.ticket);
+    const int order = (int)(ticket % (unsigned long long)g.S);
+    const unsigned epoch = (unsigned)(ticket / (unsigned long long)g.S) + 1u;
+    sweep_run<Op, BACKWARD>(op, g, c, order, epoch, L);
 }
 
 // ---------------------------------------------------------------------------

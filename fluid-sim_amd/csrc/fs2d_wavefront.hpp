@@ -61,25 +61,31 @@ __device__ __forceinline__ unsigned readlane_u32(unsigned v, int lane) {
 //                uint32_t out[NW]);                new state of (a, b); stores results
 //   void strip_done(int strip);                    once per strip, whole wave
 //   void last_done(int nstrips);                   by the last workgroup, whole wave
-template <class Op>
-__global__ __launch_bounds__(64) void wavefront_kernel(Op op, int NA, int NB, WaveCtl c) {
-    constexpr int NW = Op::NW;
-    const int lane = threadIdx.x;
-    const int nstrips = (NB + 63) >> 6;
+// Workgroup prologue: ticket, epoch and the sticky error flag.
+struct WaveTicket {
+    unsigned ticket, epoch;
+    int err;
+};
+__device__ __forceinline__ WaveTicket wave_ticket(const WaveCtl& c) {
     __shared__ unsigned sh_strip, sh_epoch;
     __shared__ int sh_err;
-    if (lane == 0) {
+    if (threadIdx.x == 0) {
         sh_strip = atomicAdd(&c.ctl[0], 1u);
         sh_epoch = __hip_atomic_load(&c.ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         sh_err = __hip_atomic_load(c.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
-    const int s = (int)sh_strip;
-    const unsigned epoch = sh_epoch;
-    // After an expired wait (sticky error) later launches do no work at all.
-    const bool skip = op.skip() || sh_err != 0;
+    return WaveTicket{sh_strip, sh_epoch, sh_err};
+}
 
-    if (!skip && s < nstrips) {
+// Strip s of an NA x NB pass; hand-off rows of strip k at handoff + k NA NW.
+template <class Op>
+__device__ __forceinline__ void wavefront_strip(Op& op, int NA, int NB, int s, unsigned epoch,
+                                                unsigned long long* handoff, int* err) {
+    constexpr int NW = Op::NW;
+    const int lane = threadIdx.x;
+    const int nstrips = (NB + 63) >> 6;
+    {
         const int b = s * 64 + lane;
         const int rows = min(64, NB - s * 64);
         const bool row_ok = lane < rows;
@@ -88,8 +94,8 @@ __global__ __launch_bounds__(64) void wavefront_kernel(Op op, int NA, int NB, Wa
 #pragma unroll
         for (int w = 0; w < NW; w++) left[w] = out[w] = bnd[w];
 
-        const unsigned long long* src = c.handoff + (size_t)(s > 0 ? s - 1 : 0) * NA * NW;
-        unsigned long long* dst = c.handoff + (size_t)s * NA * NW;
+        const unsigned long long* src = handoff + (size_t)(s > 0 ? s - 1 : 0) * NA * NW;
+        unsigned long long* dst = handoff + (size_t)s * NA * NW;
         const bool produce = (s + 1 < nstrips);
         unsigned long long hv[NW];
 #pragma unroll
@@ -120,7 +126,7 @@ __global__ __launch_bounds__(64) void wavefront_kernel(Op op, int NA, int NB, Wa
                     if (ok || failed) break;
                     if (spin >= kSpinLimit) {
                         failed = true;
-                        if (lane == 0) atomicExch(c.err, 1);
+                        if (lane == 0) atomicExch(err, 1);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
@@ -152,21 +158,22 @@ __global__ __launch_bounds__(64) void wavefront_kernel(Op op, int NA, int NB, Wa
         }
         op.strip_done(s);
     }
+}
 
-    // Completion protocol: the last workgroup resets the ticket, advances the
-    // epoch and runs the operator's final reduction.
+// Completion protocol: the last of `total` workgroups resets the ticket and
+// advances the epoch.  Returns true in that workgroup.
+__device__ __forceinline__ bool wave_finish(const WaveCtl& c, unsigned total, unsigned epoch) {
     __syncthreads();
     __threadfence();
     __shared__ unsigned sh_last;
-    if (lane == 0) {
+    if (threadIdx.x == 0) {
         unsigned done = atomicAdd(&c.ctl[1], 1u);
-        sh_last = (done == (unsigned)nstrips - 1u);
+        sh_last = (done == total - 1u);
     }
     __syncthreads();
     if (sh_last) {
         __threadfence();
-        if (!skip) op.last_done(nstrips);
-        if (lane == 0) {
+        if (threadIdx.x == 0) {
             unsigned next = epoch + 1u;
             if (next == 0u) next = 1u;
             c.ctl[0] = 0u;
@@ -175,6 +182,17 @@ __global__ __launch_bounds__(64) void wavefront_kernel(Op op, int NA, int NB, Wa
             __threadfence();
         }
     }
+    return sh_last != 0;
+}
+
+template <class Op>
+__global__ __launch_bounds__(64) void wavefront_kernel(Op op, int NA, int NB, WaveCtl c) {
+    const int nstrips = (NB + 63) >> 6;
+    const WaveTicket tk = wave_ticket(c);
+    // After an expired wait (sticky error) later launches do no work at all.
+    const bool skip = op.skip() || tk.err != 0;
+    if (!skip && (int)tk.ticket < nstrips) wavefront_strip(op, NA, NB, (int)tk.ticket, tk.epoch, c.handoff, c.err);
+    if (wave_finish(c, (unsigned)nstrips, tk.epoch) && !skip) op.last_done(nstrips);
 }
 
 // Bit casts for carried state.

@@ -98,6 +98,16 @@ def load():
         "fs2d_get_kernel_time": (i, [vp, i, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_longlong)]),
         "fs2d_reset_kernel_times": (i, [vp]),
         "fs2d_last_error": (ctypes.c_char_p, [vp]),
+        "fs2d_batch_create": (i, [vp, i, ctypes.POINTER(vp)]),
+        "fs2d_batch_destroy": (None, [vp]),
+        "fs2d_batch_size": (i, [vp]),
+        "fs2d_batch_sim": (vp, [vp, i]),
+        "fs2d_batch_step": (i, [vp]),
+        "fs2d_batch_run_stage": (i, [vp, i]),
+        "fs2d_batch_set_timing": (i, [vp, i]),
+        "fs2d_batch_get_kernel_time": (i, [vp, i, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_longlong)]),
+        "fs2d_batch_reset_kernel_times": (i, [vp]),
+        "fs2d_batch_last_error": (ctypes.c_char_p, [vp]),
         "fs2d_adjoint_create": (i, [vp, i, i, ctypes.POINTER(vp)]),
         "fs2d_adjoint_destroy": (None, [vp]),
         "fs2d_adjoint_forward": (i, [vp, i]),
@@ -136,31 +146,46 @@ _SHAPES = {
 }
 
 
+def make_config(cells, k=2, dt=0.005, dx=0.01, rho=997.0, gravity=(0.0, -9.81), mode=PICFLIP,
+                alpha=1.0, tau=0.999, sigma=0.25, tol=1e-12, max_iters=200, seed=1,
+                static_cells=False, density=False):
+    """A Config for `cells` (uint8, row-major [j, i]); the caller keeps `cells`
+    alive until the handle is created."""
+    c = default_config()
+    c.size_y, c.size_x = cells.shape
+    c.particles_per_cell_sqrt = k
+    c.dt, c.dx, c.rho = dt, dx, rho
+    c.gravity_x, c.gravity_y = gravity
+    c.mode, c.pic_flip_alpha = mode, alpha
+    c.initial_values = cells.ctypes.data
+    c.tau, c.sigma, c.pcg_tol, c.pcg_max_iters = tau, sigma, tol, max_iters
+    c.seed = seed
+    c.flags = (FLAG_STATIC_CELLS if static_cells else 0) | (FLAG_DENSITY if density else 0)
+    return c
+
+
 class Sim:
     """One FluidSim2D on the GPU (a handle of the C ABI)."""
 
-    def __init__(self, cells, k=2, dt=0.005, dx=0.01, rho=997.0, gravity=(0.0, -9.81),
-                 mode=PICFLIP, alpha=1.0, tau=0.999, sigma=0.25, tol=1e-12, max_iters=200,
-                 seed=1, static_cells=False, density=False):
+    def __init__(self, cells, **kw):
         L = load()
         cells = np.ascontiguousarray(cells, dtype=np.uint8)
         self.ny, self.nx = cells.shape
-        c = default_config()
-        c.size_x, c.size_y = self.nx, self.ny
-        c.particles_per_cell_sqrt = k
-        c.dt, c.dx, c.rho = dt, dx, rho
-        c.gravity_x, c.gravity_y = gravity
-        c.mode, c.pic_flip_alpha = mode, alpha
-        c.initial_values = cells.ctypes.data
-        c.tau, c.sigma, c.pcg_tol, c.pcg_max_iters = tau, sigma, tol, max_iters
-        c.seed = seed
-        c.flags = (FLAG_STATIC_CELLS if static_cells else 0) | (FLAG_DENSITY if density else 0)
+        c = make_config(cells, **kw)
         h = ctypes.c_void_p()
         st = L.fs2d_create(ctypes.byref(c), ctypes.byref(h))
         if st != OK:
             raise FS2DError(st, L.fs2d_last_error(None).decode())
         self.h = h
         self.L = L
+        self._owned = True
+
+    @classmethod
+    def _member(cls, L, h, nx, ny):
+        """A non-owning wrapper of a batch member's handle."""
+        self = cls.__new__(cls)
+        self.L, self.h, self.nx, self.ny, self._owned = L, ctypes.c_void_p(h), nx, ny, False
+        return self
 
     def _check(self, st, allow=(OK,)):
         if st not in allow:
@@ -169,7 +194,8 @@ class Sim:
 
     def close(self):
         if getattr(self, "h", None):
-            self.L.fs2d_destroy(self.h)
+            if getattr(self, "_owned", True):
+                self.L.fs2d_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -266,6 +292,68 @@ class Sim:
 
     def reset_kernel_times(self):
         self._check(self.L.fs2d_reset_kernel_times(self.h))
+
+
+class Batch:
+    """An ensemble stepped together (fs2d_batch_* of the C ABI).  `members` is
+    a list of (cells, kwargs) with the keyword arguments of Sim."""
+
+    def __init__(self, members):
+        L = load()
+        self.L = L
+        keep = []
+        cfgs = (Config * len(members))()
+        for k, (cells, kw) in enumerate(members):
+            cells = np.ascontiguousarray(cells, dtype=np.uint8)
+            keep.append(cells)
+            cfgs[k] = make_config(cells, **kw)
+        h = ctypes.c_void_p()
+        st = L.fs2d_batch_create(cfgs, len(members), ctypes.byref(h))
+        if st != OK:
+            raise FS2DError(st, L.fs2d_batch_last_error(None).decode())
+        self.h = h
+        self.sims = [Sim._member(L, L.fs2d_batch_sim(h, k), c.shape[1], c.shape[0]) for k, c in enumerate(keep)]
+
+    def _check(self, st, allow=(OK,)):
+        if st not in allow:
+            raise FS2DError(st, self.L.fs2d_batch_last_error(self.h).decode())
+        return st
+
+    def __len__(self):
+        return len(self.sims)
+
+    def close(self):
+        if getattr(self, "h", None):
+            for s in self.sims:
+                s.h = None
+            self.L.fs2d_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, allow=(OK, WARN_MAXITER, WARN_CFL)):
+        return self._check(self.L.fs2d_batch_step(self.h), allow)
+
+    def run_stage(self, stage, allow=(OK, WARN_MAXITER, WARN_CFL)):
+        return self._check(self.L.fs2d_batch_run_stage(self.h, stage), allow)
+
+    def set_timing(self, on):
+        self._check(self.L.fs2d_batch_set_timing(self.h, int(on)))
+
+    def kernel_times(self):
+        out = {}
+        for k in range(self.L.fs2d_kernel_count()):
+            t, n = ctypes.c_double(), ctypes.c_longlong()
+            self._check(self.L.fs2d_batch_get_kernel_time(self.h, k, ctypes.byref(t), ctypes.byref(n)))
+            out[self.L.fs2d_kernel_name(k).decode()] = (t.value, n.value)
+        return out
+
+    def reset_kernel_times(self):
+        self._check(self.L.fs2d_batch_reset_kernel_times(self.h))
 
 
 class Adjoint:

@@ -58,3 +58,20 @@ def smoke_state(n, seed, sigma=8.0, nblobs=16):
 def target_density(n, seed=4, count=8):
     """The differentiable rollout's target: `count` blobs from default_rng(seed)."""
     return blobs(n, np.random.default_rng(seed), count).astype(np.float32)
+
+
+def ensemble_members(count=256, seed=2, dt=0.002, tau=0.97, tol=1e-6, max_iters=500):
+    """The ensemble workload: `count` PIC/FLIP dam breaks, side uniform in
+    {64, 96, ..., 512} and fluid block width and height uniform in [0.2, 0.7]
+    of the side (numpy default_rng(seed): sides first, then the fractions),
+    4 particles per fluid cell, FLIP blend 0.95 (PIC weight 0.05), dx = 1 /
+    side.  Returns (cells, keyword arguments of fs2d.Sim) per member."""
+    rng = np.random.default_rng(seed)
+    sides = rng.choice(np.arange(64, 513, 32), size=count)
+    fr = rng.uniform(0.2, 0.7, size=(count, 2))
+    out = []
+    for n, (w, h) in zip(sides, fr):
+        n = int(n)
+        out.append((dam_break(n, w, h), dict(k=2, dt=dt, dx=1.0 / n, mode=1, alpha=0.05, tau=tau, tol=tol,
+                                              max_iters=max_iters)))
+    return out

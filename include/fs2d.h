@@ -197,6 +197,37 @@ const char* fs2d_last_error(const fs2d_sim* sim);
 const char* fs2d_status_string(int status);
 
 /*
+ * Ensembles: many independent simulations of different sizes stepped
+ * together.  No reference counterpart (the reference steps one FluidSim2D per
+ * runFrame, src/FluidSim2D.cpp:94-138); each member steps exactly as
+ * fs2d_step steps it alone (the same kernels, bit-identical results), but
+ * every kernel of the step is launched once for all members and the PCG loop
+ * masks members that have converged.  Members share mode, flags and the
+ * solver parameters (tau, sigma, pcg_tol, pcg_max_iters); sizes, dt, dx,
+ * gravity and pic_flip_alpha may differ.
+ */
+typedef struct fs2d_batch fs2d_batch;
+
+/* Create n members from cfgs[0..n) (each as fs2d_create would). */
+fs2d_status fs2d_batch_create(const fs2d_config* cfgs, int n, fs2d_batch** out);
+/* Destroys the members too. */
+void fs2d_batch_destroy(fs2d_batch* batch);
+int fs2d_batch_size(const fs2d_batch* batch);
+/* Member i's handle, for fields, statistics and set-up calls (owned by the
+ * batch; do not destroy it).  Calls on a member's handle complete before they
+ * return, so they can be interleaved with batch steps. */
+fs2d_sim* fs2d_batch_sim(fs2d_batch* batch, int i);
+/* One fs2d_step of every member.  Returns the worst status. */
+fs2d_status fs2d_batch_step(fs2d_batch* batch);
+/* One stage method (FluidSim2D.h:124-140) on every member. */
+fs2d_status fs2d_batch_run_stage(fs2d_batch* batch, int stage);
+/* Per-kernel timing of the batched launches (kernel classes of fs2d_kernel_name). */
+fs2d_status fs2d_batch_set_timing(fs2d_batch* batch, int enable);
+fs2d_status fs2d_batch_get_kernel_time(fs2d_batch* batch, int kernel, double* total_ms, long long* launches);
+fs2d_status fs2d_batch_reset_kernel_times(fs2d_batch* batch);
+const char* fs2d_batch_last_error(const fs2d_batch* batch);
+
+/*
  * Adjoint (reverse-mode gradient) of a rollout of the all-fluid
  * semi-Lagrangian step.  No reference counterpart: the reference has no
  * adjoint (its step is runFrame, src/FluidSim2D.cpp:94-115, whose stages
