@@ -22,6 +22,19 @@ events on the library's stream inside the timed region.  Launches that start
 after convergence exit at once; the average is taken over the launches that
 did work (PCG iterations + 1 per step for the two solves).
 
+`ensemble` (the batched half of the metric): 256 independent PIC/FLIP dam
+breaks, side uniform in {64, 96, ..., 512} (seed 2), fluid block width and
+height uniform in [0.2, 0.7], 4 particles per fluid cell, FLIP blend 0.95,
+dt 0.002, PCG to 1e-6 with converged members masked, stepped as one batch
+(fs2d_batch_step) and, for comparison, by looping fs2d_step over the same
+256 simulations.  Value = total cells x steps / wall time.
+
+`adjoint`: the differentiable rollout, 256 x 256 all-fluid smoke, 64 steps,
+dt 0.01, velocity seeded like the 2048^2 workload (seed 3), loss = L2 of the
+final density against 8 blobs (seed 4): forward rollout with a checkpoint
+every 8 steps, then the reverse pass (recomputing each segment); ratio =
+reverse time / forward time, with checkpoints and with every state kept.
+
 `cpu_baseline` times the oracle (FP64 CPU restatement of the reference step,
 OpenMP) on the same starting state with PCG capped at two iteration counts,
 and scales to the GPU's mean iteration count: step = fixed + iters x
@@ -141,6 +154,96 @@ def cpu_baseline(state, cells, n, dt, tol, tau, threads, iters_gpu, caps=(20, 60
                                                      iters_gpu)}
 
 
+def ensemble_leg(count, steps, warmup, loop_steps):
+    """The batched ensemble and the loop over the same simulations."""
+    import fs2d
+    from fs2d import scenes
+    members = scenes.ensemble_members(count)
+    cells = int(sum(c.shape[0] * c.shape[1] for c, _ in members))
+    batch = fs2d.Batch(members)
+    for _ in range(warmup):
+        batch.step()
+    batch.reset_kernel_times()
+    batch.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.step()
+    tb = (time.perf_counter() - t0) / steps
+    kt = batch.kernel_times()
+    its = [s.stats()["pcg_iters"] for s in batch.sims]
+    batch.close()
+    singles = [fs2d.Sim(c, **kw) for c, kw in members]
+    for _ in range(warmup):
+        for s in singles:
+            s.step()
+    t0 = time.perf_counter()
+    for _ in range(loop_steps):
+        for s in singles:
+            s.step()
+    tl = (time.perf_counter() - t0) / loop_steps
+    for s in singles:
+        s.close()
+    return {
+        "metric": "cell-steps/s of the batched ensemble", "value": cells / tb, "unit": "cell-steps/s",
+        "ms_per_step": tb * 1e3, "steps": steps, "warmup": warmup,
+        "loop_value": cells / tl, "loop_ms_per_step": tl * 1e3, "loop_steps": loop_steps,
+        "speedup_vs_loop": tl / tb,
+        "config": {"workload": "%d PIC/FLIP dam breaks, sides 64..512 (seed 2), block fractions U[0.2,0.7], "
+                               "k=2, FLIP 0.95, dt 0.002, PCG 1e-6 masked per member" % count,
+                   "members": count, "cells": cells, "pcg_iters_max": int(max(its)),
+                   "pcg_iters_mean": float(np.mean(its))},
+        "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items() if v[1]},
+    }
+
+
+def adjoint_leg(n=256, steps=64, every=8):
+    """Forward rollout and checkpointed reverse pass of the smoke workload."""
+    import fs2d
+    from fs2d import scenes
+    sim = fs2d.Sim(scenes.all_fluid(n), k=0, dt=0.01, dx=1.0 / n, mode=fs2d.SEMILAGRANGIAN, gravity=(0.0, 0.0),
+                   tau=0.97, tol=1e-6, max_iters=4000, static_cells=True, density=True)
+    u, v, d = scenes.smoke_state(n, 3)
+    for f, a in zip((fs2d.F_U, fs2d.F_V, fs2d.F_DENSITY), (u, v, d)):
+        sim.set(f, a)
+    sim.run_stage(fs2d.STAGE_PROJECT)
+    sim.run_stage(fs2d.STAGE_UPDATE_VELOCITY)
+    x0 = [sim.get(f) for f in (fs2d.F_U, fs2d.F_V, fs2d.F_DENSITY)]
+    target = scenes.target_density(n)
+    res = {}
+    for ev in (every, 1):
+        adj = fs2d.Adjoint(sim, checkpoint_every=ev, max_steps=steps)
+        for rep in range(2):  # the first pass warms up
+            for f, a in zip((fs2d.F_U, fs2d.F_V, fs2d.F_DENSITY), x0):
+                sim.set(f, a)
+            sim.synchronize()
+            t0 = time.perf_counter()
+            adj.forward(steps)
+            sim.synchronize()
+            tf = time.perf_counter() - t0
+            dT = sim.get(fs2d.F_DENSITY)
+            adj.set(fs2d.F_U, np.zeros_like(x0[0]))
+            adj.set(fs2d.F_V, np.zeros_like(x0[1]))
+            adj.set(fs2d.F_DENSITY, 2.0 * (dT - target))
+            t0 = time.perf_counter()
+            adj.backward()
+            sim.synchronize()
+            tb = time.perf_counter() - t0
+        res[ev] = (tf, tb, float(((dT - target) ** 2).sum()),
+                   float(np.linalg.norm(adj.get(fs2d.F_U))) + float(np.linalg.norm(adj.get(fs2d.F_V))))
+        adj.close()
+    sim.close()
+    tf, tb, loss, gnorm = res[every]
+    return {
+        "metric": "reverse-pass time / forward time per step", "value": tb / tf, "unit": "ratio",
+        "higher_is_better": False,
+        "fwd_ms_per_step": tf / steps * 1e3, "bwd_ms_per_step": tb / steps * 1e3,
+        "ratio_every_state_kept": res[1][1] / res[1][0], "bwd_ms_per_step_every_state_kept": res[1][1] / steps * 1e3,
+        "loss": loss, "grad_norm_uv": gnorm,
+        "config": {"workload": "%dx%d all-fluid SL smoke rollout, %d steps, dt 0.01, checkpoint every %d, "
+                               "loss = |d_T - target|^2" % (n, n, steps, every)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +255,11 @@ def main():
     ap.add_argument("--max-iters", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--ensemble-members", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=5)
+    ap.add_argument("--loop-steps", type=int, default=2)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -215,6 +323,10 @@ def main():
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kt.items() if v[1]},
     }
     sim.close()
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_leg(args.ensemble_members, args.ensemble_steps, 2, args.loop_steps)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(state, cells, n, dt, args.tol, args.tau, args.cpu_threads,
                                            float(np.mean(iters)))

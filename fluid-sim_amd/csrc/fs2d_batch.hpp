@@ -151,14 +151,27 @@ __global__ __launch_bounds__(kBlock) void b_spmv_kernel(const InstBufs* __restri
     pcg_spmv_sk_body(I.geom, I.Adiag, I.cpl, I.z, (h & 1) ? I.s1 : I.s0, (h & 1) ? I.s0 : I.s1, I.w, sA, I.sc,
                      (h & 1) ? qq1 : qq0, (h & 1) ? qq0 : qq1, h == 0 ? 1 : 0, I.ws_part, vb, nvb);
 }
+// w.s of every member still iterating, summed once per member in the order
+// block_sum_bcast sums it (one wave per member), so that the axpy's workgroups
+// need not each re-read all of the member's partials.  `nparts`: the
+// member's SpMV workgroup count.
+__global__ __launch_bounds__(64) void b_ws_reduce_kernel(const InstBufs* __restrict__ ins, const int* members,
+                                                         int n, const int* nparts) {
+    if ((int)blockIdx.x >= n) return;
+    const int b = members[blockIdx.x];
+    const InstBufs& I = ins[b];
+    if (I.sc->done) return;
+    const double t = wave_sum_of(I.ws_part, nparts[b]);
+    if (threadIdx.x == 0) I.sc->ws = t;
+}
 __global__ __launch_bounds__(kBlock) void b_axpy_kernel(const InstBufs* __restrict__ ins, BlockMap bm, int h) {
     int b, vb, nvb;
     if (!bmap(bm, b, vb, nvb)) return;
     const InstBufs& I = ins[b];
-    const double* qq0 = I.qq;
-    const double* qq1 = I.qq + (size_t)I.geom.S * kLoaders;
-    pcg_axpy_sk_body(I.geom, I.pk, I.r, (h & 1) ? I.s0 : I.s1, I.w, I.sc, (h & 1) ? qq1 : qq0, I.ws_part, nvb,
-                     I.rmax_part, vb, nvb);
+    (void)h;
+    // sigma (written by the SpMV) and w.s come reduced in the member's scalars
+    pcg_axpy_sk_body(I.geom, I.pk, I.r, (h & 1) ? I.s0 : I.s1, I.w, I.sc, nullptr, nullptr, -1, I.rmax_part, vb,
+                     nvb);
 }
 __global__ __launch_bounds__(kBlock) void b_sk_to_rowmajor_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
     int b, vb, nvb;

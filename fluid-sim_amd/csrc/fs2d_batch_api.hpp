@@ -28,6 +28,15 @@ struct fs2d_batch {
     unsigned long long* sk_ticket = nullptr;
     int* d_status = nullptr;
     int* h_status = nullptr;  // pinned: per member done, iterations, NaN flag, CFL warning
+    // SpMV / axpy launches over the members still iterating (rebuilt between
+    // chunks of PCG iterations); the sweeps keep the full strip lists, since
+    // their epochs count launches of a fixed strip total.
+    std::vector<int> h_act;
+    int* d_act = nullptr;
+    size_t act_cap = 0;
+    BlockMap m_act{};
+    const int* act_members = nullptr;
+    int g_act = 0, n_act = 0;
     int* h_err = nullptr;
     bool sorted_valid = false;
     int last_iters_hint = 8;
@@ -169,6 +178,30 @@ fs2d_status build_maps(fs2d_batch* B) {
     B->m_cells = mk(o_cells); B->m_faces = mk(o_faces); B->m_sl = mk(o_sl); B->m_part = mk(o_part);
     B->m_diag = mk(o_diag); B->m_sk = mk(o_sk); B->m_tiles = mk(o_tiles); B->m_one = mk(o_one);
     B->strips = StripMap{maps + strip_inst, maps + strip_first, total};
+    return FS2D_OK;
+}
+
+// Launch map of the SpMV / axpy over the members with active[b] != 0, and
+// the list of those members (for the w.s reduction).
+fs2d_status build_active(fs2d_batch* B, const std::vector<int>& active) {
+    const int n = B->n;
+    std::vector<int> cnt(n);
+    for (int b = 0; b < n; b++) cnt[b] = active[b] ? sk_blocks(B->sims[b]) : 0;
+    B->h_act.clear();
+    MapOff o;
+    B->g_act = add_map(B->h_act, cnt, o);
+    const size_t mem = B->h_act.size();
+    for (int b = 0; b < n; b++)
+        if (active[b]) B->h_act.push_back(b);
+    B->n_act = (int)(B->h_act.size() - mem);
+    if (B->h_act.size() > B->act_cap) {
+        B->act_cap = B->h_act.size() + B->h_act.size() / 2;
+        TRY(balloc(B, &B->d_act, B->act_cap));
+    }
+    BHIPCHK(B, hipMemcpyAsync(B->d_act, B->h_act.data(), B->h_act.size() * sizeof(int), hipMemcpyHostToDevice,
+                              B->stream));
+    B->m_act = BlockMap{B->d_act + o.inst, B->d_act + o.first, B->d_act + o.count};
+    B->act_members = B->d_act + mem;
     return FS2D_OK;
 }
 
@@ -316,16 +349,19 @@ fs2d_status bstage_projection(fs2d_batch* B) {
     int chunk = std::max(2, std::min(B->last_iters_hint, B->max_iters));
     int iters = 0;
     bool hit = false;
+    std::vector<int> active(B->n, 1);
+    TRY(build_active(B, active));
     while (h < B->max_iters) {
         const int nchunk = std::min(chunk, B->max_iters - h);
         for (int k = 0; k < nchunk; k++, h++) {
             {
                 Timed t(B, K_PCG_SPMV);
-                blaunch(B, b_spmv_kernel, B->g_sk, kBlock, B->d_ins, B->m_sk, h);
+                blaunch(B, b_spmv_kernel, B->g_act, kBlock, B->d_ins, B->m_act, h);
             }
             {
                 Timed t(B, K_PCG_AXPY);
-                blaunch(B, b_axpy_kernel, B->g_sk, kBlock, B->d_ins, B->m_sk, h);
+                blaunch(B, b_ws_reduce_kernel, B->n_act, 64, B->d_ins, B->act_members, B->n_act, B->m_sk.count);
+                blaunch(B, b_axpy_kernel, B->g_act, kBlock, B->d_ins, B->m_act, h);
             }
             {
                 Timed t(B, K_PCG_FWD);
@@ -339,9 +375,15 @@ fs2d_status bstage_projection(fs2d_batch* B) {
             }
         }
         TRY(gather_status(B));
-        bool all = true;
-        for (int b = 0; b < B->n; b++) all = all && B->h_status[4 * b] != 0;
+        bool all = true, changed = false;
+        for (int b = 0; b < B->n; b++) {
+            const int a = B->h_status[4 * b] == 0;
+            all = all && !a;
+            changed = changed || a != active[b];
+            active[b] = a;
+        }
         if (all) break;
+        if (changed) TRY(build_active(B, active));
         chunk = 8;
     }
     {

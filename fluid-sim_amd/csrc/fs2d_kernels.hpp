@@ -312,8 +312,9 @@ __device__ __forceinline__ void pcg_axpy_sk_body(
     const float* __restrict__ w, const PcgScalars* sc, const double* sig_cur, const double* ws_partials,
     int n_ws, double* partials, int vb, int nvb) {
     if (sc->done) return;
-    const double sig = block_sum_bcast(sig_cur, g.S * kLoaders);
-    const double ws = block_sum_bcast(ws_partials, n_ws);
+    // n_ws < 0: sigma and w.s were already reduced into sc, in the same order
+    const double sig = n_ws < 0 ? sc->sigma : block_sum_bcast(sig_cur, g.S * kLoaders);
+    const double ws = n_ws < 0 ? sc->ws : block_sum_bcast(ws_partials, n_ws);
     const float a = (float)(sig / ws);
     float m = 0.0f;
     const XcdLoop it = xcd_loop(g.size(), vb, nvb);

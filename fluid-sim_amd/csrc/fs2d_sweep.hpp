@@ -476,6 +476,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(Op op, SkGeom g, S
 // kernel that produces it and read by later kernels.
 struct PcgScalars {
     double rhs_norm, r_norm, sigma, tol;
+    double ws;  // w.s of the current iteration when a separate reduction formed it
     int iter, done, pad0, pad1;
 };
 
