@@ -819,7 +819,8 @@ __global__ __launch_bounds__(kBlock) void ls_seed_kernel(const float* __restrict
 // One directional Gauss-Seidel sweep of the closest-particle propagation
 // (FluidSim2D.cpp:775-792).  Carried state: (index, x, y) of the adopted
 // particle, so the two "new" candidates need no memory access; the current
-// value and the two "old" neighbours come from the previous sweep's array.
+// value and the two "old" neighbours come from the previous sweep's array,
+// loaded ahead of the sweep front (indices, then the particles' positions).
 struct ClosestOp {
     static constexpr int NW = 3;
     const int* __restrict__ t_in;
@@ -828,28 +829,56 @@ struct ClosestOp {
     const float* __restrict__ py;
     int nx, ny, flipx, flipy;
     double dx, dr;
+    // the cell's own entry and its "old" x and y neighbours (-1: none or outside)
+    struct Pre {
+        int tc, tx, ty;
+    };
+    struct Rec {
+        int tc, tx, ty;
+        float cx, cy, xx, xy, yx, yy;
+    };
     __device__ bool skip() const { return false; }
     __device__ void boundary(uint32_t* s) const {
         s[0] = 0xFFFFFFFFu;
         s[1] = s[2] = 0;
     }
-    __device__ void compute(int a, int b, const uint32_t* left, const uint32_t* below, uint32_t* out) {
+    // Loads are branch-free: out-of-grid addresses are clamped into the array
+    // and their values replaced (wait counts stay exact, see wavefront_strip).
+    __device__ Pre pre(int a, int b, bool ok) const {
+        const int i = clampi(flipx ? nx - 1 - a : a, 0, nx - 1);
+        const int j = clampi(flipy ? ny - 1 - b : b, 0, ny - 1);
+        const long e = (long)j * nx + i;
+        // not yet updated in this sweep: (i+1, j) sweeping +x, (i-1, j) sweeping -x; likewise y
+        const int ii = flipx ? i - 1 : i + 1, jj = flipy ? j - 1 : j + 1;
+        const bool xin = ii >= 0 && ii < nx, yin = jj >= 0 && jj < ny;
+        const int tc = t_in[e];
+        const int tx = t_in[(long)j * nx + clampi(ii, 0, nx - 1)];
+        const int ty = t_in[(long)clampi(jj, 0, ny - 1) * nx + i];
+        return Pre{ok ? tc : -1, ok && xin ? tx : -1, ok && yin ? ty : -1};
+    }
+    __device__ Rec gather(const Pre& p) const {
+        const int c = max(p.tc, 0), x = max(p.tx, 0), y = max(p.ty, 0);
+        const float cx = px[c], cy = py[c], xx = px[x], xy = py[x], yx = px[y], yy = py[y];
+        return Rec{p.tc, p.tx, p.ty, cx, cy, xx, xy, yx, yy};
+    }
+    __device__ void compute(int a, int b, const Rec& r, const uint32_t* left, const uint32_t* below,
+                            uint32_t* out) {
         const int i = flipx ? nx - 1 - a : a;
         const int j = flipy ? ny - 1 - b : b;
         const long e = (long)j * nx + i;
-        int tc = t_in[e];
+        int tc = r.tc;
         double cphi = HUGE_VAL;
         float cx = 0, cy = 0;
         if (tc >= 0) {
-            cx = px[tc];
-            cy = py[tc];
+            cx = r.cx;
+            cy = r.cy;
             cphi = corner_dist(i, j, cx, cy, dx, dr);
         }
         // candidates in the reference's order: (i-1,j), (i+1,j), (i,j-1), (i,j+1)
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             int tn;
-            float qx = 0, qy = 0;
+            float qx, qy;
             const bool xs = k < 2;
             const bool minus = (k & 1) == 0;
             // is this neighbour already updated in the current sweep?
@@ -860,14 +889,9 @@ struct ClosestOp {
                 qx = u2f(st[1]);
                 qy = u2f(st[2]);
             } else {
-                const int ii = xs ? (minus ? i - 1 : i + 1) : i;
-                const int jj = xs ? j : (minus ? j - 1 : j + 1);
-                if (ii < 0 || ii >= nx || jj < 0 || jj >= ny) continue;
-                tn = t_in[(long)jj * nx + ii];
-                if (tn >= 0) {
-                    qx = px[tn];
-                    qy = py[tn];
-                }
+                tn = xs ? r.tx : r.ty;
+                qx = xs ? r.xx : r.yx;
+                qy = xs ? r.xy : r.yy;
             }
             if (tn < 0) continue;
             const double d = corner_dist(i, j, qx, qy, dx, dr);
@@ -930,14 +954,26 @@ struct EikonalOp {
     double* __restrict__ outp;
     int nx, ny, flipx, flipy;
     double dx;
+    struct Pre {
+        double v;
+    };
+    using Rec = Pre;
     __device__ bool skip() const { return false; }
     __device__ void boundary(uint32_t* s) const { s[0] = s[1] = 0; }
-    __device__ void compute(int a, int b, const uint32_t* left, const uint32_t* below, uint32_t* out) {
+    __device__ Pre pre(int a, int b, bool ok) const {
+        const int i = clampi(flipx ? nx - 1 - a : a, 0, nx - 1);
+        const int j = clampi(flipy ? ny - 1 - b : b, 0, ny - 1);
+        const double v = in[(long)j * nx + i];  // branch-free (see ClosestOp::pre)
+        return Pre{ok ? v : 0.0};
+    }
+    __device__ Rec gather(const Pre& p) const { return p; }
+    __device__ void compute(int a, int b, const Rec& r, const uint32_t* left, const uint32_t* below,
+                            uint32_t* out) {
 #pragma clang fp contract(off)
         const int i = flipx ? nx - 1 - a : a;
         const int j = flipy ? ny - 1 - b : b;
         const long e = (long)j * nx + i;
-        double cur = in[e];
+        double cur = r.v;
         if (a >= 1 && b >= 1 && !(cur >= 0)) {
             const double A = fabs(u2d(left)), B = fabs(u2d(below));
             const double phi0 = A < B ? A : B;

@@ -57,10 +57,19 @@ __device__ __forceinline__ unsigned readlane_u32(unsigned v, int lane) {
 //   static constexpr int NW;                       32-bit words of carried state
 //   bool skip() const;                             whole pass is a no-op
 //   void boundary(uint32_t s[NW]) const;           state of out-of-grid neighbours
-//   void compute(int a, int b, const uint32_t left[NW], const uint32_t below[NW],
-//                uint32_t out[NW]);                new state of (a, b); stores results
+//   struct Pre, struct Rec;                        a cell's inputs, in two stages
+//   Pre pre(int a, int b, bool ok) const;          first-stage loads (ok: (a, b) in the grid)
+//   Rec gather(const Pre&) const;                  second-stage loads that need the first
+//   void compute(int a, int b, const Rec& r, const uint32_t left[NW],
+//                const uint32_t below[NW], uint32_t out[NW]);
+//                                                  new state of (a, b); stores results
 //   void strip_done(int strip);                    once per strip, whole wave
 //   void last_done(int nstrips);                   by the last workgroup, whole wave
+// A cell's inputs never depend on the carried state, so the engine issues
+// pre() kWavePrefetch steps before gather() and gather() kWavePrefetch steps
+// before compute(): the loads of a step are in flight while earlier steps
+// compute, and a step waits only for the DPP of its neighbour's result.
+constexpr int kWavePrefetch = 8;
 // Workgroup prologue: ticket, epoch and the sticky error flag.
 struct WaveTicket {
     unsigned ticket, epoch;
@@ -79,73 +88,108 @@ __device__ __forceinline__ WaveTicket wave_ticket(const WaveCtl& c) {
 }
 
 // Strip s of an NA x NB pass; hand-off rows of strip k at handoff + k NA NW.
+// Lane 0 needs the row below's value of column t at step t; it loads that
+// hand-off word 2 kWavePrefetch steps ahead with the cells' inputs (so no wait
+// ever drains the prefetch stream) and re-polls only a word whose epoch is not
+// yet current.  Strip 0 reads its own (never consumed) row instead: the
+// address stays valid and the value is unused.
 template <class Op>
 __device__ __forceinline__ void wavefront_strip(Op& op, int NA, int NB, int s, unsigned epoch,
                                                 unsigned long long* handoff, int* err) {
     constexpr int NW = Op::NW;
+    constexpr int D = kWavePrefetch;
     const int lane = threadIdx.x;
     const int nstrips = (NB + 63) >> 6;
-    {
-        const int b = s * 64 + lane;
-        const int rows = min(64, NB - s * 64);
-        const bool row_ok = lane < rows;
-        uint32_t bnd[NW], left[NW], out[NW];
-        op.boundary(bnd);
+    const int b = s * 64 + lane;
+    const int rows = min(64, NB - s * 64);
+    const bool row_ok = lane < rows;
+    uint32_t bnd[NW], left[NW], out[NW];
+    op.boundary(bnd);
 #pragma unroll
-        for (int w = 0; w < NW; w++) left[w] = out[w] = bnd[w];
+    for (int w = 0; w < NW; w++) left[w] = out[w] = bnd[w];
 
-        const unsigned long long* src = handoff + (size_t)(s > 0 ? s - 1 : 0) * NA * NW;
-        unsigned long long* dst = handoff + (size_t)s * NA * NW;
-        const bool produce = (s + 1 < nstrips);
-        unsigned long long hv[NW];
+    const unsigned long long* src = handoff + (size_t)(s > 0 ? s - 1 : 0) * NA * NW;
+    unsigned long long* dst = handoff + (size_t)s * NA * NW;
+    const bool produce = (s + 1 < nstrips);
+    const bool consume = s > 0;
+    bool failed = false;
+    auto in_grid = [&](int a) { return a >= 0 && a < NA && row_ok; };
+    // Every lane loads (a clamped, in-bounds address) and lane 0's words are
+    // the ones used: branch-free loads keep the compiler's wait counts exact.
+    auto hand = [&](int col, unsigned long long (&h)[NW]) {
+        const int c = min(col, NA - 1);
 #pragma unroll
-        for (int w = 0; w < NW; w++) hv[w] = 0;
-        int c0 = -64;
-        bool failed = false;
+        for (int w = 0; w < NW; w++) h[w] = atomic_ld64(src + (size_t)c * NW + w);
+    };
 
-        const int T = NA + rows - 1;
-        for (int t = 0; t < T; ++t) {
+    const int T = NA + rows - 1;
+    typename Op::Pre pq[D];
+    typename Op::Rec rq[D];
+    unsigned long long hp[D][NW], hr[D][NW];
+    // steps 0 .. D-1 fully loaded, steps D .. 2D-1 first stage
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        rq[k] = op.gather(op.pre(k - lane, b, in_grid(k - lane)));
+        hand(k, hr[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        pq[k] = op.pre(D + k - lane, b, in_grid(D + k - lane));
+        hand(D + k, hp[k]);
+    }
+    for (int t0 = 0; t0 < T; t0 += D) {
+#pragma unroll
+        for (int kq = 0; kq < D; kq++) {
+            const int t = t0 + kq;
+            if (t >= T) break;  // uniform
             const int a = t - lane;
+            const typename Op::Rec cur = rq[kq];
+            unsigned long long hw[NW];
+#pragma unroll
+            for (int w = 0; w < NW; w++) hw[w] = hr[kq][w];
+            rq[kq] = op.gather(pq[kq]);  // step t + D
+#pragma unroll
+            for (int w = 0; w < NW; w++) hr[kq][w] = hp[kq][w];
+            pq[kq] = op.pre(t + 2 * D - lane, b, in_grid(t + 2 * D - lane));  // step t + 2D
+            hand(t + 2 * D, hp[kq]);
+
             uint32_t bl0[NW];
-            if (s == 0 || t >= NA) {
+            if (!consume || t >= NA) {
 #pragma unroll
                 for (int w = 0; w < NW; w++) bl0[w] = bnd[w];
             } else {
-                if (t >= c0 + 64) {
-                    c0 += 64;
-                    const int col = c0 + lane;
-#pragma unroll
-                    for (int w = 0; w < NW; w++) hv[w] = col < NA ? atomic_ld64(src + (size_t)col * NW + w) : 0ull;
-                }
-                const int k = t - c0;
-                for (int spin = 0;; ++spin) {
+                // check first: the re-poll loop (whose entry waits for every
+                // outstanding load) runs only when the word is not yet current
+                auto current = [&]() {
                     bool ok = true;
 #pragma unroll
-                    for (int w = 0; w < NW; w++)
-                        ok = ok && (readlane_u32((unsigned)(hv[w] >> 32), k) == epoch);
-                    if (ok || failed) break;
-                    if (spin >= kSpinLimit) {
-                        failed = true;
-                        if (lane == 0) atomicExch(err, 1);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    const int col = c0 + lane;
+                    for (int w = 0; w < NW; w++) ok = ok && (readlane_u32((unsigned)(hw[w] >> 32), 0) == epoch);
+                    return ok;
+                };
+                if (!current() && !failed) {
+                    int spin = 0;
+                    do {
+                        if (spin++ >= kSpinLimit) {
+                            failed = true;
+                            if (lane == 0) atomicExch(err, 1);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
 #pragma unroll
-                    for (int w = 0; w < NW; w++)
-                        if (col < NA && (unsigned)(hv[w] >> 32) != epoch)
-                            hv[w] = atomic_ld64(src + (size_t)col * NW + w);
+                        for (int w = 0; w < NW; w++)
+                            if (lane == 0) hw[w] = atomic_ld64(src + (size_t)t * NW + w);
+                    } while (!current());
                 }
 #pragma unroll
-                for (int w = 0; w < NW; w++) bl0[w] = readlane_u32((unsigned)hv[w], k);
+                for (int w = 0; w < NW; w++) bl0[w] = (uint32_t)hw[w];
             }
             uint32_t below[NW];
 #pragma unroll
             for (int w = 0; w < NW; w++)
                 below[w] = (uint32_t)__builtin_amdgcn_update_dpp((int)bl0[w], (int)out[w], 0x138 /* wave_shr:1 */,
                                                                  0xf, 0xf, false);
-            if (a >= 0 && a < NA && row_ok) {
-                op.compute(a, b, left, below, out);
+            if (in_grid(a)) {
+                op.compute(a, b, cur, left, below, out);
 #pragma unroll
                 for (int w = 0; w < NW; w++) left[w] = out[w];
                 if (produce && lane == 63) {
@@ -156,8 +200,8 @@ __device__ __forceinline__ void wavefront_strip(Op& op, int NA, int NB, int s, u
                 }
             }
         }
-        op.strip_done(s);
     }
+    op.strip_done(s);
 }
 
 // Completion protocol: the last of `total` workgroups resets the ticket and
