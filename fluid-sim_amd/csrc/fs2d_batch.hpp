@@ -33,6 +33,7 @@ struct InstBufs {
     uint8_t* cell;
     double *phi, *phi2;
     int *t0, *t1;
+    LsCand* lsc;
     float *p, *r, *z, *q, *w, *s0, *s1, *pk, *Adiag, *P, *fx, *fy, *gx_, *gy_;
     uint8_t* cpl;
     PcgScalars* sc;
@@ -342,21 +343,29 @@ __global__ void b_gather_status_kernel(const InstBufs* __restrict__ ins, int n, 
 // ---------------------------------------------------------------------------
 // Wavefront passes over all simulations.
 // ---------------------------------------------------------------------------
-// Level-set sweeps: pass `dir` (bit 0 flips x, bit 1 flips y) reading the
-// parity's input buffer.
+// Level-set sweeps towards -x (FX) / -y (FY), reading the parity's input buffer.
+template <int FX, int FY>
 struct ClosestMaker {
-    int dir, parity;
-    __device__ ClosestOp operator()(const InstBufs& I) const {
-        return ClosestOp{parity ? I.t1 : I.t0, parity ? I.t0 : I.t1, I.px, I.py, I.nx, I.ny, dir & 1, dir >> 1,
-                         I.dx, I.dr};
+    int parity;
+    __device__ ClosestOp<FX, FY> operator()(const InstBufs& I) const {
+        return ClosestOp<FX, FY>{I.lsc, parity ? I.t0 : I.t1, I.nx, I.ny, I.dx, I.dr};
     }
 };
+template <int FX, int FY>
 struct EikonalMaker {
-    int dir, parity;
-    __device__ EikonalOp operator()(const InstBufs& I) const {
-        return EikonalOp{parity ? I.phi2 : I.phi, parity ? I.phi : I.phi2, I.nx, I.ny, dir & 1, dir >> 1, I.dx};
+    int parity;
+    __device__ EikonalOp<FX, FY> operator()(const InstBufs& I) const {
+        return EikonalOp<FX, FY>{parity ? I.phi2 : I.phi, parity ? I.phi : I.phi2, I.nx, I.ny, I.dx};
     }
 };
+// Fixed candidates of closest-particle sweep `dir` from the parity's input.
+__global__ __launch_bounds__(kBlock) void b_ls_cand_kernel(const InstBufs* __restrict__ ins, BlockMap bm, int dir,
+                                                           int parity) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    ls_cand_body(parity ? I.t1 : I.t0, I.px, I.py, I.lsc, I.nx, I.ny, dir & 1, dir >> 1, I.dx, I.dr, vb, nvb);
+}
 template <class Op, class Maker>
 __global__ __launch_bounds__(64) void b_wavefront_kernel(const InstBufs* __restrict__ ins, StripMap sm, WaveCtl c,
                                                          Maker mk) {

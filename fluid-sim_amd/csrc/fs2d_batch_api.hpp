@@ -92,7 +92,7 @@ InstBufs inst_bufs(const fs2d_sim* s, unsigned long long* ls_handoff, unsigned l
     I.pmass = s->k > 0 ? s->rho * s->dx * s->dx / (s->k * s->k) : 0.0;
     I.geom = s->geom;
     I.u = s->u; I.v = s->v; I.u2 = s->u2; I.v2 = s->v2; I.nu = s->nu; I.nv = s->nv; I.dens = s->dens; I.dens2 = s->dens2;
-    I.cell = s->cell; I.phi = s->phi; I.phi2 = s->phi2; I.t0 = s->t0; I.t1 = s->t1;
+    I.cell = s->cell; I.phi = s->phi; I.phi2 = s->phi2; I.t0 = s->t0; I.t1 = s->t1; I.lsc = s->lsc;
     I.p = s->p; I.r = s->r; I.z = s->z; I.q = s->q; I.w = s->w; I.s0 = s->s[0]; I.s1 = s->s[1]; I.pk = s->pk;
     I.Adiag = s->Adiag; I.P = s->P; I.fx = s->fx; I.fy = s->fy; I.gx_ = s->gx_; I.gy_ = s->gy_; I.cpl = s->cpl;
     I.sc = s->sc; I.qq = s->qq; I.ws_part = s->ws_part; I.rmax_part = s->rmax_part;
@@ -262,10 +262,23 @@ void bstage_level_set(fs2d_batch* B) {
             blaunch(B, b_ls_seed_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
         }
         const WaveCtl c{nullptr, B->ls_ctl, B->werr};
+        const int S = B->strips.total;
         for (int k = 0; k < 16; k++) {
+            {
+                Timed t(B, K_LS_OTHER);
+                blaunch(B, b_ls_cand_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells, k & 3, k & 1);
+            }
             Timed t(B, K_LS_SWEEP);
-            blaunch(B, b_wavefront_kernel<ClosestOp, ClosestMaker>, B->strips.total, 64, B->d_ins, B->strips, c,
-                    ClosestMaker{k & 3, k & 1});
+            switch (k & 3) {
+                case 0: blaunch(B, b_wavefront_kernel<ClosestOp<0, 0>, ClosestMaker<0, 0>>, S, 64, B->d_ins,
+                                B->strips, c, ClosestMaker<0, 0>{k & 1}); break;
+                case 1: blaunch(B, b_wavefront_kernel<ClosestOp<1, 0>, ClosestMaker<1, 0>>, S, 64, B->d_ins,
+                                B->strips, c, ClosestMaker<1, 0>{k & 1}); break;
+                case 2: blaunch(B, b_wavefront_kernel<ClosestOp<0, 1>, ClosestMaker<0, 1>>, S, 64, B->d_ins,
+                                B->strips, c, ClosestMaker<0, 1>{k & 1}); break;
+                default: blaunch(B, b_wavefront_kernel<ClosestOp<1, 1>, ClosestMaker<1, 1>>, S, 64, B->d_ins,
+                                 B->strips, c, ClosestMaker<1, 1>{k & 1}); break;
+            }
         }
         {
             Timed t(B, K_LS_OTHER);
@@ -274,8 +287,16 @@ void bstage_level_set(fs2d_batch* B) {
         }
         for (int k = 0; k < 16; k++) {
             Timed t(B, K_LS_SWEEP);
-            blaunch(B, b_wavefront_kernel<EikonalOp, EikonalMaker>, B->strips.total, 64, B->d_ins, B->strips, c,
-                    EikonalMaker{k & 3, k & 1});
+            switch (k & 3) {
+                case 0: blaunch(B, b_wavefront_kernel<EikonalOp<0, 0>, EikonalMaker<0, 0>>, S, 64, B->d_ins,
+                                B->strips, c, EikonalMaker<0, 0>{k & 1}); break;
+                case 1: blaunch(B, b_wavefront_kernel<EikonalOp<1, 0>, EikonalMaker<1, 0>>, S, 64, B->d_ins,
+                                B->strips, c, EikonalMaker<1, 0>{k & 1}); break;
+                case 2: blaunch(B, b_wavefront_kernel<EikonalOp<0, 1>, EikonalMaker<0, 1>>, S, 64, B->d_ins,
+                                B->strips, c, EikonalMaker<0, 1>{k & 1}); break;
+                default: blaunch(B, b_wavefront_kernel<EikonalOp<1, 1>, EikonalMaker<1, 1>>, S, 64, B->d_ins,
+                                 B->strips, c, EikonalMaker<1, 1>{k & 1}); break;
+            }
         }
         {
             Timed t(B, K_LS_OTHER);

@@ -63,6 +63,7 @@ struct fs2d_sim {
     uint8_t* cell = nullptr;
     double *phi = nullptr, *phi2 = nullptr;
     int *t0 = nullptr, *t1 = nullptr;
+    LsCand* lsc = nullptr;  // fixed candidates of the closest-particle sweeps
     // projection: row-major pressure + SK4 solver arrays (fs2d_skew.hpp)
     float* p = nullptr;
     SkGeom geom{};
@@ -308,9 +309,18 @@ void stage_level_set(fs2d_sim* s) {
         int* tout = s->t1;
         for (int rep = 0; rep < 4; rep++)
             for (int d = 0; d < 4; d++) {
+                {
+                    Timed t(s, K_LS_OTHER);
+                    hipLaunchKernelGGL(ls_cand_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, tin, s->px,
+                                       s->py, s->lsc, s->nx, s->ny, d & 1, d >> 1, s->dx, s->dr);
+                }
                 Timed t(s, K_LS_SWEEP);
-                ClosestOp op{tin, tout, s->px, s->py, s->nx, s->ny, d & 1, d >> 1, s->dx, s->dr};
-                launch_wavefront(s, op);
+                switch (d) {
+                    case 0: launch_wavefront(s, ClosestOp<0, 0>{s->lsc, tout, s->nx, s->ny, s->dx, s->dr}); break;
+                    case 1: launch_wavefront(s, ClosestOp<1, 0>{s->lsc, tout, s->nx, s->ny, s->dx, s->dr}); break;
+                    case 2: launch_wavefront(s, ClosestOp<0, 1>{s->lsc, tout, s->nx, s->ny, s->dx, s->dr}); break;
+                    default: launch_wavefront(s, ClosestOp<1, 1>{s->lsc, tout, s->nx, s->ny, s->dx, s->dr}); break;
+                }
                 std::swap(tin, tout);
             }
         double* a = s->phi;
@@ -325,8 +335,12 @@ void stage_level_set(fs2d_sim* s) {
         for (int rep = 0; rep < 4; rep++)
             for (int d = 0; d < 4; d++) {
                 Timed t(s, K_LS_SWEEP);
-                EikonalOp op{a, b, s->nx, s->ny, d & 1, d >> 1, s->dx};
-                launch_wavefront(s, op);
+                switch (d) {
+                    case 0: launch_wavefront(s, EikonalOp<0, 0>{a, b, s->nx, s->ny, s->dx}); break;
+                    case 1: launch_wavefront(s, EikonalOp<1, 0>{a, b, s->nx, s->ny, s->dx}); break;
+                    case 2: launch_wavefront(s, EikonalOp<0, 1>{a, b, s->nx, s->ny, s->dx}); break;
+                    default: launch_wavefront(s, EikonalOp<1, 1>{a, b, s->nx, s->ny, s->dx}); break;
+                }
                 std::swap(a, b);
             }
         {
@@ -679,6 +693,7 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
     if ((st = dalloc(s, &s->p, (n))) != FS2D_OK) return bail(st)
     A(u, nU); A(v, nV); A(u2, nU); A(v2, nV); A(nu, nU); A(nv, nV);
     A(dens, nC); A(dens2, nC); A(cell, nC); A(phi, nC); A(phi2, nC); A(t0, nC); A(t1, nC);
+    if (!(s->flags & FS2D_FLAG_STATIC_CELLS)) A(lsc, nC);
     s->geom = make_geom(s->nx, s->ny);
     const size_t nS = (size_t)s->geom.size();
     A(p, nC);

@@ -817,95 +817,108 @@ __global__ __launch_bounds__(kBlock) void ls_seed_kernel(const float* __restrict
 }
 
 // One directional Gauss-Seidel sweep of the closest-particle propagation
-// (FluidSim2D.cpp:775-792).  Carried state: (index, x, y) of the adopted
-// particle, so the two "new" candidates need no memory access; the current
-// value and the two "old" neighbours come from the previous sweep's array,
-// loaded ahead of the sweep front (indices, then the particles' positions).
+// (FluidSim2D.cpp:775-792).  The cell adopts, among its own entry and its four
+// neighbours' entries taken in the order (cell, i-1, i+1, j-1, j+1), the
+// first one with the strictly smallest corner distance: the lexicographic
+// minimum of (distance, position in that order).  Three of the five do not
+// depend on the sweep front (the cell's own entry and the two neighbours not
+// yet updated in this sweep, all from the previous sweep's array), so
+// ls_cand_body merges them beforehand, in parallel, into one candidate; the
+// sweep compares only the two candidates its front produces.  FX / FY: the
+// sweep runs towards -x / -y.
+struct LsCand {
+    double d;    // corner distance; HUGE_VAL: no candidate
+    int t, pos;  // particle (-1: none), position in the candidate order
+    float x, y;  // the particle's position
+};
+
+__device__ __forceinline__ void ls_cand_body(const int* __restrict__ t_in, const float* __restrict__ px,
+                                             const float* __restrict__ py, LsCand* __restrict__ out, int nx, int ny,
+                                             int fx, int fy, double dx, double dr, int vb, int nvb) {
+    const long n = (long)nx * ny;
+    for (long c = vb * (long)blockDim.x + threadIdx.x; c < n; c += (long)nvb * blockDim.x) {
+        const int i = (int)(c % nx), j = (int)(c / nx);
+        LsCand best{HUGE_VAL, -1, 0, 0.0f, 0.0f};
+        const int tc = t_in[c];
+        if (tc >= 0) {
+            const float x = px[tc], y = py[tc];
+            best = LsCand{corner_dist(i, j, x, y, dx, dr), tc, 0, x, y};
+        }
+        // not yet updated: (i+1, j) sweeping +x (position 2), (i-1, j) sweeping -x (position 1); likewise y
+        const int ii = fx ? i - 1 : i + 1, jj = fy ? j - 1 : j + 1;
+        if (ii >= 0 && ii < nx) {
+            const int tn = t_in[(long)j * nx + ii];
+            if (tn >= 0) {
+                const float x = px[tn], y = py[tn];
+                const double d = corner_dist(i, j, x, y, dx, dr);
+                if (d < best.d) best = LsCand{d, tn, fx ? 1 : 2, x, y};
+            }
+        }
+        if (jj >= 0 && jj < ny) {
+            const int tn = t_in[(long)jj * nx + i];
+            if (tn >= 0) {
+                const float x = px[tn], y = py[tn];
+                const double d = corner_dist(i, j, x, y, dx, dr);
+                if (d < best.d) best = LsCand{d, tn, fy ? 3 : 4, x, y};
+            }
+        }
+        out[c] = best;
+    }
+}
+__global__ __launch_bounds__(kBlock) void ls_cand_kernel(const int* __restrict__ t_in, const float* __restrict__ px,
+                                                         const float* __restrict__ py, LsCand* __restrict__ out,
+                                                         int nx, int ny, int fx, int fy, double dx, double dr) {
+    ls_cand_body(t_in, px, py, out, nx, ny, fx, fy, dx, dr, blockIdx.x, gridDim.x);
+}
+
+// The sweep itself.  Carried state: (index, x, y) of the adopted particle.
+template <int FX, int FY>
 struct ClosestOp {
     static constexpr int NW = 3;
-    const int* __restrict__ t_in;
+    const LsCand* __restrict__ cand;
     int* __restrict__ t_out;
-    const float* __restrict__ px;
-    const float* __restrict__ py;
-    int nx, ny, flipx, flipy;
+    int nx, ny;
     double dx, dr;
-    // the cell's own entry and its "old" x and y neighbours (-1: none or outside)
-    struct Pre {
-        int tc, tx, ty;
-    };
-    struct Rec {
-        int tc, tx, ty;
-        float cx, cy, xx, xy, yx, yy;
-    };
+    using Rec = LsCand;
     __device__ bool skip() const { return false; }
     __device__ void boundary(uint32_t* s) const {
         s[0] = 0xFFFFFFFFu;
         s[1] = s[2] = 0;
     }
-    // Loads are branch-free: out-of-grid addresses are clamped into the array
-    // and their values replaced (wait counts stay exact, see wavefront_strip).
-    __device__ Pre pre(int a, int b, bool ok) const {
-        const int i = clampi(flipx ? nx - 1 - a : a, 0, nx - 1);
-        const int j = clampi(flipy ? ny - 1 - b : b, 0, ny - 1);
-        const long e = (long)j * nx + i;
-        // not yet updated in this sweep: (i+1, j) sweeping +x, (i-1, j) sweeping -x; likewise y
-        const int ii = flipx ? i - 1 : i + 1, jj = flipy ? j - 1 : j + 1;
-        const bool xin = ii >= 0 && ii < nx, yin = jj >= 0 && jj < ny;
-        const int tc = t_in[e];
-        const int tx = t_in[(long)j * nx + clampi(ii, 0, nx - 1)];
-        const int ty = t_in[(long)clampi(jj, 0, ny - 1) * nx + i];
-        return Pre{ok ? tc : -1, ok && xin ? tx : -1, ok && yin ? ty : -1};
-    }
-    __device__ Rec gather(const Pre& p) const {
-        const int c = max(p.tc, 0), x = max(p.tx, 0), y = max(p.ty, 0);
-        const float cx = px[c], cy = py[c], xx = px[x], xy = py[x], yx = px[y], yy = py[y];
-        return Rec{p.tc, p.tx, p.ty, cx, cy, xx, xy, yx, yy};
+    // Branch-free load: an out-of-grid address is clamped into the array and
+    // its value replaced (wait counts stay exact, see wavefront_strip).
+    __device__ Rec load(int a, int b, bool ok) const {
+        const int i = clampi(FX ? nx - 1 - a : a, 0, nx - 1);
+        const int j = clampi(FY ? ny - 1 - b : b, 0, ny - 1);
+        const LsCand c = cand[(long)j * nx + i];
+        return ok ? c : LsCand{HUGE_VAL, -1, 0, 0.0f, 0.0f};
     }
     __device__ void compute(int a, int b, const Rec& r, const uint32_t* left, const uint32_t* below,
                             uint32_t* out) {
-        const int i = flipx ? nx - 1 - a : a;
-        const int j = flipy ? ny - 1 - b : b;
-        const long e = (long)j * nx + i;
-        int tc = r.tc;
-        double cphi = HUGE_VAL;
-        float cx = 0, cy = 0;
-        if (tc >= 0) {
-            cx = r.cx;
-            cy = r.cy;
-            cphi = corner_dist(i, j, cx, cy, dx, dr);
-        }
-        // candidates in the reference's order: (i-1,j), (i+1,j), (i,j-1), (i,j+1)
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            int tn;
-            float qx, qy;
-            const bool xs = k < 2;
-            const bool minus = (k & 1) == 0;
-            // is this neighbour already updated in the current sweep?
-            const bool fresh = xs ? (minus != (bool)flipx) : (minus != (bool)flipy);
-            if (fresh) {
-                const uint32_t* st = xs ? left : below;
-                tn = (int)st[0];
-                qx = u2f(st[1]);
-                qy = u2f(st[2]);
-            } else {
-                tn = xs ? r.tx : r.ty;
-                qx = xs ? r.xx : r.yx;
-                qy = xs ? r.xy : r.yy;
-            }
-            if (tn < 0) continue;
+        const int i = FX ? nx - 1 - a : a;
+        const int j = FY ? ny - 1 - b : b;
+        double bd = r.d;
+        int bt = r.t, bp = r.pos;
+        float bx = r.x, by = r.y;
+        // the updated neighbours: (i-1, j) / (j-1) sweeping +x / +y (positions 1, 3), else (i+1, j) / (j+1);
+        // selects rather than branches (the distance of an absent candidate is computed and discarded)
+        auto consider = [&](const uint32_t* st, int pos) {
+            const int tn = (int)st[0];
+            const float qx = u2f(st[1]), qy = u2f(st[2]);
             const double d = corner_dist(i, j, qx, qy, dx, dr);
-            if (d < cphi) {
-                cphi = d;
-                tc = tn;
-                cx = qx;
-                cy = qy;
-            }
-        }
-        t_out[e] = tc;
-        out[0] = (uint32_t)tc;
-        out[1] = f2u(cx);
-        out[2] = f2u(cy);
+            const bool take = (tn >= 0) & ((d < bd) | ((d == bd) & (pos < bp)));  // no short-circuit branches
+            bd = take ? d : bd;
+            bt = take ? tn : bt;
+            bp = take ? pos : bp;
+            bx = take ? qx : bx;
+            by = take ? qy : by;
+        };
+        consider(left, FX ? 2 : 1);
+        consider(below, FY ? 4 : 3);
+        t_out[(long)j * nx + i] = bt;
+        out[0] = (uint32_t)bt;
+        out[1] = f2u(bx);
+        out[2] = f2u(by);
     }
     __device__ void strip_done(int) {}
     __device__ void last_done(int) {}
@@ -948,30 +961,29 @@ __global__ __launch_bounds__(kBlock) void ls_surface_kernel(const double* __rest
 
 // One directional sweep of the eikonal update (:845-902) on cells with phi < 0,
 // for logical a >= 1 and b >= 1.  Carried state: phi (FP64).
+template <int FX, int FY>
 struct EikonalOp {
     static constexpr int NW = 2;
     const double* __restrict__ in;
     double* __restrict__ outp;
-    int nx, ny, flipx, flipy;
+    int nx, ny;
     double dx;
-    struct Pre {
+    struct Rec {
         double v;
     };
-    using Rec = Pre;
     __device__ bool skip() const { return false; }
     __device__ void boundary(uint32_t* s) const { s[0] = s[1] = 0; }
-    __device__ Pre pre(int a, int b, bool ok) const {
-        const int i = clampi(flipx ? nx - 1 - a : a, 0, nx - 1);
-        const int j = clampi(flipy ? ny - 1 - b : b, 0, ny - 1);
-        const double v = in[(long)j * nx + i];  // branch-free (see ClosestOp::pre)
-        return Pre{ok ? v : 0.0};
+    __device__ Rec load(int a, int b, bool ok) const {
+        const int i = clampi(FX ? nx - 1 - a : a, 0, nx - 1);
+        const int j = clampi(FY ? ny - 1 - b : b, 0, ny - 1);
+        const double v = in[(long)j * nx + i];  // branch-free (see ClosestOp::load)
+        return Rec{ok ? v : 0.0};
     }
-    __device__ Rec gather(const Pre& p) const { return p; }
     __device__ void compute(int a, int b, const Rec& r, const uint32_t* left, const uint32_t* below,
                             uint32_t* out) {
 #pragma clang fp contract(off)
-        const int i = flipx ? nx - 1 - a : a;
-        const int j = flipy ? ny - 1 - b : b;
+        const int i = FX ? nx - 1 - a : a;
+        const int j = FY ? ny - 1 - b : b;
         const long e = (long)j * nx + i;
         double cur = r.v;
         if (a >= 1 && b >= 1 && !(cur >= 0)) {

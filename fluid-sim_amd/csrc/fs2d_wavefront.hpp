@@ -57,18 +57,18 @@ __device__ __forceinline__ unsigned readlane_u32(unsigned v, int lane) {
 //   static constexpr int NW;                       32-bit words of carried state
 //   bool skip() const;                             whole pass is a no-op
 //   void boundary(uint32_t s[NW]) const;           state of out-of-grid neighbours
-//   struct Pre, struct Rec;                        a cell's inputs, in two stages
-//   Pre pre(int a, int b, bool ok) const;          first-stage loads (ok: (a, b) in the grid)
-//   Rec gather(const Pre&) const;                  second-stage loads that need the first
+//   struct Rec;                                    a cell's inputs
+//   Rec load(int a, int b, bool ok) const;         loads of (a, b) (ok: in the grid; else a
+//                                                  branch-free load of a clamped address)
 //   void compute(int a, int b, const Rec& r, const uint32_t left[NW],
 //                const uint32_t below[NW], uint32_t out[NW]);
 //                                                  new state of (a, b); stores results
 //   void strip_done(int strip);                    once per strip, whole wave
 //   void last_done(int nstrips);                   by the last workgroup, whole wave
-// A cell's inputs never depend on the carried state, so the engine issues
-// pre() kWavePrefetch steps before gather() and gather() kWavePrefetch steps
-// before compute(): the loads of a step are in flight while earlier steps
-// compute, and a step waits only for the DPP of its neighbour's result.
+// A cell's inputs never depend on the carried state, so the engine loads them
+// kWavePrefetch steps before compute() needs them, in a register ring whose
+// slot k is loaded and consumed at steps of the same residue (no copies): a
+// step waits only for its own record and the DPP of its neighbour's result.
 constexpr int kWavePrefetch = 8;
 // Workgroup prologue: ticket, epoch and the sticky error flag.
 struct WaveTicket {
@@ -89,7 +89,7 @@ __device__ __forceinline__ WaveTicket wave_ticket(const WaveCtl& c) {
 
 // Strip s of an NA x NB pass; hand-off rows of strip k at handoff + k NA NW.
 // Lane 0 needs the row below's value of column t at step t; it loads that
-// hand-off word 2 kWavePrefetch steps ahead with the cells' inputs (so no wait
+// hand-off word kWavePrefetch steps ahead with the cells' inputs (so no wait
 // ever drains the prefetch stream) and re-polls only a word whose epoch is not
 // yet current.  Strip 0 reads its own (never consumed) row instead: the
 // address stays valid and the value is unused.
@@ -123,19 +123,12 @@ __device__ __forceinline__ void wavefront_strip(Op& op, int NA, int NB, int s, u
     };
 
     const int T = NA + rows - 1;
-    typename Op::Pre pq[D];
     typename Op::Rec rq[D];
-    unsigned long long hp[D][NW], hr[D][NW];
-    // steps 0 .. D-1 fully loaded, steps D .. 2D-1 first stage
+    unsigned long long hq[D][NW];
 #pragma unroll
     for (int k = 0; k < D; k++) {
-        rq[k] = op.gather(op.pre(k - lane, b, in_grid(k - lane)));
-        hand(k, hr[k]);
-    }
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-        pq[k] = op.pre(D + k - lane, b, in_grid(D + k - lane));
-        hand(D + k, hp[k]);
+        rq[k] = op.load(k - lane, b, in_grid(k - lane));
+        hand(k, hq[k]);
     }
     for (int t0 = 0; t0 < T; t0 += D) {
 #pragma unroll
@@ -146,12 +139,9 @@ __device__ __forceinline__ void wavefront_strip(Op& op, int NA, int NB, int s, u
             const typename Op::Rec cur = rq[kq];
             unsigned long long hw[NW];
 #pragma unroll
-            for (int w = 0; w < NW; w++) hw[w] = hr[kq][w];
-            rq[kq] = op.gather(pq[kq]);  // step t + D
-#pragma unroll
-            for (int w = 0; w < NW; w++) hr[kq][w] = hp[kq][w];
-            pq[kq] = op.pre(t + 2 * D - lane, b, in_grid(t + 2 * D - lane));  // step t + 2D
-            hand(t + 2 * D, hp[kq]);
+            for (int w = 0; w < NW; w++) hw[w] = hq[kq][w];
+            rq[kq] = op.load(t + D - lane, b, in_grid(t + D - lane));  // step t + D
+            hand(t + D, hq[kq]);
 
             uint32_t bl0[NW];
             if (!consume || t >= NA) {
