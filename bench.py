@@ -260,6 +260,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=5)
     ap.add_argument("--loop-steps", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--only", choices=["ensemble", "adjoint"],
+                    help="run one leg alone and print its object (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -267,6 +269,12 @@ def main():
         sys.exit(2)
 
     import fs2d
+    if args.only == "ensemble":
+        print(json.dumps(ensemble_leg(args.ensemble_members, args.ensemble_steps, args.warmup, args.loop_steps)))
+        return
+    if args.only == "adjoint":
+        print(json.dumps(adjoint_leg()))
+        return
     n = args.n
     sim, cells, dt = build_gpu_sim(n, args.tol, args.tau, args.max_iters)
     state = [sim.get(f) for f in (fs2d.F_U, fs2d.F_V, fs2d.F_DENSITY)]

@@ -31,6 +31,7 @@ struct InstBufs {
     SkGeom geom;
     float *u, *v, *u2, *v2, *nu, *nv, *dens, *dens2;
     uint8_t* cell;
+    uint8_t* cell2;
     double *phi, *phi2;
     int *t0, *t1;
     LsCand* lsc;
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(kBlock) void b_assemble_kernel(const InstBufs* __re
     const InstBufs& I = ins[b];
     const double sA = I.dt / (I.rho * I.dx * I.dx);
     assemble_sk_body(I.geom, I.cell, I.phi, I.u, I.v, I.Adiag, I.cpl, I.r, I.pk, I.s0, sA, 1.0 / I.dx,
-                     I.rmax_part, vb, nvb);
+                     I.rmax_part, (I.flags & 8) ? 0 : 1 /* FS2D_FLAG_NO_GHOST_PRESSURE */, vb, nvb);
 }
 
 // Iteration h of the PCG loop: s double-buffered by parity, sigma_h and
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void b_update_velocity_kernel(const InstBuf
     if (!bmap(bm, b, vb, nvb)) return;
     const InstBufs& I = ins[b];
     update_velocity_body(I.cell, I.phi, I.p, I.u, I.v, I.nu, I.nv, I.mu, I.mv, I.nx, I.ny, I.dt / (I.rho * I.dx),
-                         vb, nvb);
+                         (I.flags & 8) ? 0 : 1 /* FS2D_FLAG_NO_GHOST_PRESSURE */, vb, nvb);
 }
 
 // Extrapolation of (u, v) (which = 0, after P2G) or (nu, nv) (which = 1).
@@ -358,6 +359,32 @@ struct EikonalMaker {
         return EikonalOp<FX, FY>{parity ? I.phi2 : I.phi, parity ? I.phi : I.phi2, I.nx, I.ny, I.dx};
     }
 };
+// Cells from particles (createWaterLevelSet without USE_LEVEL_SET).
+__global__ __launch_bounds__(kBlock) void b_pc_clear_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    pc_clear_body(I.cell, I.cell2, (long)I.nx * I.ny, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_pc_mark_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    pc_mark_body(I.px, I.py, I.np, I.cell, I.cell2, I.nx, I.ny, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_pc_copy_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    copy_body(I.cell, (const uint8_t*)I.cell2, (size_t)I.nx * I.ny, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_pc_fill_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    pc_fill_body(I.cell, I.cell2, I.nx, I.ny, vb, nvb);
+}
+
 // Fixed candidates of closest-particle sweep `dir` from the parity's input.
 __global__ __launch_bounds__(kBlock) void b_ls_cand_kernel(const InstBufs* __restrict__ ins, BlockMap bm, int dir,
                                                            int parity) {

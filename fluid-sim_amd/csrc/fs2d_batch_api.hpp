@@ -92,7 +92,7 @@ InstBufs inst_bufs(const fs2d_sim* s, unsigned long long* ls_handoff, unsigned l
     I.pmass = s->k > 0 ? s->rho * s->dx * s->dx / (s->k * s->k) : 0.0;
     I.geom = s->geom;
     I.u = s->u; I.v = s->v; I.u2 = s->u2; I.v2 = s->v2; I.nu = s->nu; I.nv = s->nv; I.dens = s->dens; I.dens2 = s->dens2;
-    I.cell = s->cell; I.phi = s->phi; I.phi2 = s->phi2; I.t0 = s->t0; I.t1 = s->t1; I.lsc = s->lsc;
+    I.cell = s->cell; I.cell2 = s->cell2; I.phi = s->phi; I.phi2 = s->phi2; I.t0 = s->t0; I.t1 = s->t1; I.lsc = s->lsc;
     I.p = s->p; I.r = s->r; I.z = s->z; I.q = s->q; I.w = s->w; I.s0 = s->s[0]; I.s1 = s->s[1]; I.pk = s->pk;
     I.Adiag = s->Adiag; I.P = s->P; I.fx = s->fx; I.fy = s->fy; I.gx_ = s->gx_; I.gy_ = s->gy_; I.cpl = s->cpl;
     I.sc = s->sc; I.qq = s->qq; I.ws_part = s->ws_part; I.rmax_part = s->rmax_part;
@@ -255,7 +255,13 @@ void bstage_diagnostics(fs2d_batch* B) {
 }
 
 void bstage_level_set(fs2d_batch* B) {
-    if (!(B->flags & FS2D_FLAG_STATIC_CELLS)) {
+    if (B->flags & FS2D_FLAG_PARTICLE_CELLS) {
+        Timed t(B, K_LS_OTHER);
+        blaunch(B, b_pc_clear_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
+        if (B->any_particles) blaunch(B, b_pc_mark_kernel, B->g_part, kBlock, B->d_ins, B->m_part);
+        blaunch(B, b_pc_fill_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
+        blaunch(B, b_pc_copy_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
+    } else if (!(B->flags & FS2D_FLAG_STATIC_CELLS)) {
         bstage_sort(B);
         {
             Timed t(B, K_LS_OTHER);

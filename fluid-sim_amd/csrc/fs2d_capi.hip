@@ -61,6 +61,7 @@ struct fs2d_sim {
     float *u = nullptr, *v = nullptr, *u2 = nullptr, *v2 = nullptr, *nu = nullptr, *nv = nullptr;
     float *dens = nullptr, *dens2 = nullptr;
     uint8_t* cell = nullptr;
+    uint8_t* cell2 = nullptr;  // flags written by the bubble fill (FS2D_FLAG_PARTICLE_CELLS)
     double *phi = nullptr, *phi2 = nullptr;
     int *t0 = nullptr, *t1 = nullptr;
     LsCand* lsc = nullptr;  // fixed candidates of the closest-particle sweeps
@@ -223,6 +224,9 @@ fs2d_status d2h(fs2d_sim* s, void* dst, const void* src, size_t bytes) {
     return FS2D_OK;
 }
 
+// USE_GHOST_PRESSURE of the reference (src/FluidSim2D.cpp:15), on by default.
+int ghost_pressure(const fs2d_sim* s) { return (s->flags & FS2D_FLAG_NO_GHOST_PRESSURE) ? 0 : 1; }
+
 long n_u(const fs2d_sim* s) { return (long)(s->nx + 1) * s->ny; }
 long n_v(const fs2d_sim* s) { return (long)s->nx * (s->ny + 1); }
 long n_c(const fs2d_sim* s) { return (long)s->nx * s->ny; }
@@ -297,7 +301,18 @@ void diagnostics(fs2d_sim* s) {
 
 // FluidSim2D::createWaterLevelSet (src/FluidSim2D.cpp:653-732).
 void stage_level_set(fs2d_sim* s) {
-    if (!(s->flags & FS2D_FLAG_STATIC_CELLS)) {
+    if (s->flags & FS2D_FLAG_PARTICLE_CELLS) {
+        // createWaterLevelSet without USE_LEVEL_SET (src/FluidSim2D.cpp:665-706)
+        Timed t(s, K_LS_OTHER);
+        const long nc = n_c(s);
+        hipLaunchKernelGGL(pc_clear_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->cell, s->cell2, nc);
+        if (s->np > 0)
+            hipLaunchKernelGGL(pc_mark_kernel, dim3(blocks_for(s->np)), dim3(kBlock), 0, s->stream, s->px, s->py,
+                               s->np, s->cell, s->cell2, s->nx, s->ny);
+        hipLaunchKernelGGL(pc_fill_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->cell, s->cell2,
+                           s->nx, s->ny);
+        dcopy(s, s->cell, (const uint8_t*)s->cell2, (size_t)nc);
+    } else if (!(s->flags & FS2D_FLAG_STATIC_CELLS)) {
         const long nc = n_c(s);
         sort_particles(s);
         {
@@ -409,7 +424,8 @@ void assemble(fs2d_sim* s) {
     Timed t(s, K_ASSEMBLE);
     const double sA = s->dt / (s->rho * s->dx * s->dx);
     hipLaunchKernelGGL(assemble_sk_kernel, dim3(sk_blocks(s)), dim3(kBlock), 0, s->stream, s->geom, s->cell, s->phi,
-                       s->u, s->v, s->Adiag, s->cpl, s->r, s->pk, s->s[0], sA, 1.0 / s->dx, s->rmax_part);
+                       s->u, s->v, s->Adiag, s->cpl, s->r, s->pk, s->s[0], sA, 1.0 / s->dx, s->rmax_part,
+                       ghost_pressure(s));
 }
 
 // MIC(0) factor and PCG (src/FluidSim2D.cpp:364-466) on the assembled matrix
@@ -497,7 +513,7 @@ void stage_update_velocity(fs2d_sim* s) {
         Timed t(s, K_UPDATE_VEL);
         hipLaunchKernelGGL(update_velocity_kernel, dim3(blocks_for(n_u(s) + n_v(s))), dim3(kBlock), 0, s->stream,
                            s->cell, s->phi, s->p, s->u, s->v, s->nu, s->nv, s->mu, s->mv, s->nx, s->ny,
-                           s->dt / (s->rho * s->dx));
+                           s->dt / (s->rho * s->dx), ghost_pressure(s));
     }
     extrapolate_uv(s, s->nu, s->nv);
     if (s->mode == FS2D_SEMILAGRANGIAN) {
@@ -625,6 +641,17 @@ static fs2d_status validate(const fs2d_config* c, char* why, size_t n) {
         snprintf(why, n, "solver parameters");
         return FS2D_ERR_CONFIG;
     }
+    if (c->flags & ~(FS2D_FLAG_STATIC_CELLS | FS2D_FLAG_DENSITY | FS2D_FLAG_PARTICLE_CELLS |
+                     FS2D_FLAG_NO_GHOST_PRESSURE)) {
+        snprintf(why, n, "unknown flags 0x%x", c->flags);
+        return FS2D_ERR_CONFIG;
+    }
+    if ((c->flags & FS2D_FLAG_PARTICLE_CELLS) &&
+        ((c->flags & FS2D_FLAG_STATIC_CELLS) || !(c->flags & FS2D_FLAG_NO_GHOST_PRESSURE))) {
+        // without a level set the ghost-fluid terms would divide phi = 0 by phi = 0
+        snprintf(why, n, "FS2D_FLAG_PARTICLE_CELLS needs FS2D_FLAG_NO_GHOST_PRESSURE and no FS2D_FLAG_STATIC_CELLS");
+        return FS2D_ERR_CONFIG;
+    }
     if (!c->initial_values) {
         snprintf(why, n, "initial_values is NULL");
         return FS2D_ERR_CONFIG;
@@ -693,7 +720,8 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
     if ((st = dalloc(s, &s->p, (n))) != FS2D_OK) return bail(st)
     A(u, nU); A(v, nV); A(u2, nU); A(v2, nV); A(nu, nU); A(nv, nV);
     A(dens, nC); A(dens2, nC); A(cell, nC); A(phi, nC); A(phi2, nC); A(t0, nC); A(t1, nC);
-    if (!(s->flags & FS2D_FLAG_STATIC_CELLS)) A(lsc, nC);
+    if (!(s->flags & (FS2D_FLAG_STATIC_CELLS | FS2D_FLAG_PARTICLE_CELLS))) A(lsc, nC);
+    if (s->flags & FS2D_FLAG_PARTICLE_CELLS) A(cell2, nC);
     s->geom = make_geom(s->nx, s->ny);
     const size_t nS = (size_t)s->geom.size();
     A(p, nC);

@@ -200,7 +200,8 @@ __device__ __forceinline__ bool sk_cell(const SkGeom& g, long e, int& i, int& j)
 __device__ __forceinline__ void assemble_sk_body(
     SkGeom g, const uint8_t* __restrict__ cell, const double* __restrict__ phi, const float* __restrict__ u,
     const float* __restrict__ v, float* __restrict__ Adiag, uint8_t* __restrict__ flags, float* __restrict__ r,
-    float* __restrict__ p, float* __restrict__ s0, double scaleA, double scale, double* partials, int vb, int nvb) {
+    float* __restrict__ p, float* __restrict__ s0, double scaleA, double scale, double* partials, int ghost, int vb,
+    int nvb) {
     const int nx = g.nx;
     double m = 0;
     for (long e = vb * (long)blockDim.x + threadIdx.x; e < g.size(); e += (long)nvb * blockDim.x) {
@@ -211,15 +212,26 @@ __device__ __forceinline__ void assemble_sk_body(
             const long c = (long)j * nx + i;
             if (cell[c] == FLUID) {
                 const uint8_t cl = cell[c - 1], cr = cell[c + 1], cb = cell[c - nx], ct = cell[c + nx];
-                const double ph = phi[c];
-                if (cl == FLUID) { ad += scaleA; f |= CPL_L; }
-                else if (cl == EMPTY) ad -= scaleA * fmax(phi[c - 1] / ph, -1e3);
-                if (cr == FLUID) { ad += scaleA; f |= CPL_R; }
-                else if (cr == EMPTY) ad += scaleA * (1 - fmax(phi[c + 1] / ph, -1e3));
-                if (cb == FLUID) { ad += scaleA; f |= CPL_D; }
-                else if (cb == EMPTY) ad -= scaleA * fmax(phi[c - nx] / ph, -1e3);
-                if (ct == FLUID) { ad += scaleA; f |= CPL_U; }
-                else if (ct == EMPTY) ad += scaleA * (1 - fmax(phi[c + nx] / ph, -1e3));
+                if (ghost) {
+                    const double ph = phi[c];
+                    if (cl == FLUID) { ad += scaleA; f |= CPL_L; }
+                    else if (cl == EMPTY) ad -= scaleA * fmax(phi[c - 1] / ph, -1e3);
+                    if (cr == FLUID) { ad += scaleA; f |= CPL_R; }
+                    else if (cr == EMPTY) ad += scaleA * (1 - fmax(phi[c + 1] / ph, -1e3));
+                    if (cb == FLUID) { ad += scaleA; f |= CPL_D; }
+                    else if (cb == EMPTY) ad -= scaleA * fmax(phi[c - nx] / ph, -1e3);
+                    if (ct == FLUID) { ad += scaleA; f |= CPL_U; }
+                    else if (ct == EMPTY) ad += scaleA * (1 - fmax(phi[c + nx] / ph, -1e3));
+                } else {
+                    // without USE_GHOST_PRESSURE (:304-331): only the +x and +y EMPTY
+                    // neighbours add to the diagonal
+                    if (cl == FLUID) { ad += scaleA; f |= CPL_L; }
+                    if (cr == FLUID) { ad += scaleA; f |= CPL_R; }
+                    else if (cr == EMPTY) ad += scaleA;
+                    if (cb == FLUID) { ad += scaleA; f |= CPL_D; }
+                    if (ct == FLUID) { ad += scaleA; f |= CPL_U; }
+                    else if (ct == EMPTY) ad += scaleA;
+                }
                 const long iu = (long)j * (nx + 1) + i;
                 const double u0 = u[iu], u1 = u[iu + 1], v0 = v[c], v1 = v[c + nx];
                 rhs = -scale * (u1 - u0 + v1 - v0);
@@ -243,8 +255,9 @@ __device__ __forceinline__ void assemble_sk_body(
 __global__ __launch_bounds__(kBlock) void assemble_sk_kernel(
     SkGeom g, const uint8_t* __restrict__ cell, const double* __restrict__ phi, const float* __restrict__ u,
     const float* __restrict__ v, float* __restrict__ Adiag, uint8_t* __restrict__ flags, float* __restrict__ r,
-    float* __restrict__ p, float* __restrict__ s0, double scaleA, double scale, double* partials) {
-    assemble_sk_body(g, cell, phi, u, v, Adiag, flags, r, p, s0, scaleA, scale, partials, blockIdx.x, gridDim.x);
+    float* __restrict__ p, float* __restrict__ s0, double scaleA, double scale, double* partials, int ghost) {
+    assemble_sk_body(g, cell, phi, u, v, Adiag, flags, r, p, s0, scaleA, scale, partials, ghost, blockIdx.x,
+                     gridDim.x);
 }
 
 // Sum of a small partial array by one wave, broadcast to every thread of the
@@ -357,7 +370,7 @@ __global__ __launch_bounds__(kBlock) void sk_to_rowmajor_kernel(SkGeom g, const 
 __device__ __forceinline__ void update_velocity_body(
     const uint8_t* __restrict__ cell, const double* __restrict__ phi, const float* __restrict__ p,
     const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ nu, float* __restrict__ nv,
-    uint32_t* __restrict__ mu, uint32_t* __restrict__ mv, int nx, int ny, double scale, int vb, int nvb) {
+    uint32_t* __restrict__ mu, uint32_t* __restrict__ mv, int nx, int ny, double scale, int ghost, int vb, int nvb) {
     const long nU = (long)(nx + 1) * ny, nV = (long)nx * (ny + 1);
     for (long e = vb * (long)blockDim.x + threadIdx.x; e < nU + nV; e += (long)nvb * blockDim.x) {
         if (e < nU) {
@@ -370,6 +383,7 @@ __device__ __forceinline__ void update_velocity_body(
                 const uint8_t cl = i > 0 ? cell[c - 1] : SOLID;
                 if ((i > 0 && cl == FLUID) || cc == FLUID) {
                     if ((i == 0 || cl == SOLID) || cc == SOLID) val = 0;
+                    else if (!ghost) val -= scale * ((double)p[c] - (double)p[c - 1]);  // :486-505 without ghosts
                     else if (i > 0 && cl == EMPTY)
                         val -= scale * (1 - fmax(phi[c - 1] / phi[c], -1e3)) * (double)p[c];
                     else if (cc == EMPTY)
@@ -393,6 +407,7 @@ __device__ __forceinline__ void update_velocity_body(
                 const uint8_t cb = j > 0 ? cell[c - nx] : SOLID;
                 if ((j > 0 && cb == FLUID) || cc == FLUID) {
                     if ((j == 0 || cb == SOLID) || cc == SOLID) val = 0;
+                    else if (!ghost) val -= scale * ((double)p[c] - (double)p[c - nx]);
                     else if (j > 0 && cb == EMPTY)
                         val -= scale * (1 - fmax(phi[c - nx] / phi[c], -1e3)) * (double)p[c];
                     else if (cc == EMPTY)
@@ -411,8 +426,8 @@ __device__ __forceinline__ void update_velocity_body(
 __global__ __launch_bounds__(kBlock) void update_velocity_kernel(
     const uint8_t* __restrict__ cell, const double* __restrict__ phi, const float* __restrict__ p,
     const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ nu, float* __restrict__ nv,
-    uint32_t* __restrict__ mu, uint32_t* __restrict__ mv, int nx, int ny, double scale) {
-    update_velocity_body(cell, phi, p, u, v, nu, nv, mu, mv, nx, ny, scale, blockIdx.x, gridDim.x);
+    uint32_t* __restrict__ mu, uint32_t* __restrict__ mv, int nx, int ny, double scale, int ghost) {
+    update_velocity_body(cell, phi, p, u, v, nu, nv, mu, mv, nx, ny, scale, ghost, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1028,6 +1043,72 @@ __device__ __forceinline__ void classify_body(uint8_t* __restrict__ cell, const 
 }
 __global__ void classify_kernel(uint8_t* __restrict__ cell, const double* __restrict__ phi, long n) {
     classify_body(cell, phi, n, blockIdx.x, gridDim.x);
+}
+
+// ---------------------------------------------------------------------------
+// createWaterLevelSet without USE_LEVEL_SET (src/FluidSim2D.cpp:665-706).
+// ---------------------------------------------------------------------------
+// FLUID cells become EMPTY (:667-674) and the flags are copied to `out`, the
+// array the bubble fill writes.
+__device__ __forceinline__ void pc_clear_body(uint8_t* __restrict__ cell, uint8_t* __restrict__ out, long n, int vb,
+                                              int nvb) {
+    for (long c = vb * (long)blockDim.x + threadIdx.x; c < n; c += (long)nvb * blockDim.x) {
+        const uint8_t f = cell[c] == FLUID ? EMPTY : cell[c];
+        cell[c] = f;
+        out[c] = f;
+    }
+}
+__global__ __launch_bounds__(kBlock) void pc_clear_kernel(uint8_t* __restrict__ cell, uint8_t* __restrict__ out,
+                                                          long n) {
+    pc_clear_body(cell, out, n, blockIdx.x, gridDim.x);
+}
+// Each particle marks cell ((int)x, (int)y) of its position in metres (:676-683:
+// the reference does not divide by dx) in both flag arrays, if EMPTY.  Racing
+// particles store the same value.  Marks outside the grid (a domain larger
+// than size metres) are skipped, where the reference writes out of bounds.
+__device__ __forceinline__ void pc_mark_body(const float* __restrict__ px, const float* __restrict__ py, long np,
+                                             uint8_t* __restrict__ cell, uint8_t* __restrict__ out, int nx, int ny,
+                                             int vb, int nvb) {
+    for (long e = vb * (long)blockDim.x + threadIdx.x; e < np; e += (long)nvb * blockDim.x) {
+        const int x = (int)(double)px[e], y = (int)(double)py[e];
+        if (x < 0 || x >= nx || y < 0 || y >= ny) continue;
+        const long c = (long)y * nx + x;
+        if (cell[c] == EMPTY) {
+            cell[c] = FLUID;
+            out[c] = FLUID;
+        }
+    }
+}
+__global__ __launch_bounds__(kBlock) void pc_mark_kernel(const float* __restrict__ px, const float* __restrict__ py,
+                                                         long np, uint8_t* __restrict__ cell,
+                                                         uint8_t* __restrict__ out, int nx, int ny) {
+    pc_mark_body(px, py, np, cell, out, nx, ny, blockIdx.x, gridDim.x);
+}
+// Bubble fill (:685-704): i = 1..nx-2 outer, j = 1..ny-2 inner, in place: an
+// EMPTY cell with 4 FLUID neighbours, or 3 FLUID and 1 SOLID, becomes FLUID.
+// Fills never cascade: a cell fills only if none of its neighbours is EMPTY
+// when it is visited, and an EMPTY neighbour visited earlier could not have
+// filled either (this cell was still EMPTY then).  So the serial in-place pass
+// equals one parallel pass over the flags before it: read `cell`, write `out`.
+__device__ __forceinline__ void pc_fill_body(const uint8_t* __restrict__ cell, uint8_t* __restrict__ out, int nx,
+                                             int ny, int vb, int nvb) {
+    const long n = (long)nx * ny;
+    for (long c = vb * (long)blockDim.x + threadIdx.x; c < n; c += (long)nvb * blockDim.x) {
+        const int i = (int)(c % nx), j = (int)(c / nx);
+        if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2 || cell[c] != EMPTY) continue;
+        const uint8_t nb[4] = {cell[c - 1], cell[c + 1], cell[c - nx], cell[c + nx]};
+        int fluid = 0, solid = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            fluid += nb[k] == FLUID;
+            solid += nb[k] == SOLID;
+        }
+        if (fluid == 4 || (fluid == 3 && solid == 1)) out[c] = FLUID;
+    }
+}
+__global__ __launch_bounds__(kBlock) void pc_fill_kernel(const uint8_t* __restrict__ cell, uint8_t* __restrict__ out,
+                                                         int nx, int ny) {
+    pc_fill_body(cell, out, nx, ny, blockIdx.x, gridDim.x);
 }
 
 // Diagnostics (:709-731): volume, grid energy (Catmull-Rom velocity at cell

@@ -24,7 +24,7 @@ EMPTY, FLUID, SOLID = 0, 1, 2
 (F_U, F_V, F_NEWU, F_NEWV, F_P, F_PHI, F_DENSITY, F_CELL,
  F_PARTICLE_POS, F_PARTICLE_VEL) = range(10)
 F32, F64, U8 = 0, 1, 2
-FLAG_STATIC_CELLS, FLAG_DENSITY = 1, 2
+FLAG_STATIC_CELLS, FLAG_DENSITY, FLAG_PARTICLE_CELLS, FLAG_NO_GHOST_PRESSURE = 1, 2, 4, 8
 
 
 class Config(ctypes.Structure):
@@ -148,7 +148,7 @@ _SHAPES = {
 
 def make_config(cells, k=2, dt=0.005, dx=0.01, rho=997.0, gravity=(0.0, -9.81), mode=PICFLIP,
                 alpha=1.0, tau=0.999, sigma=0.25, tol=1e-12, max_iters=200, seed=1,
-                static_cells=False, density=False):
+                static_cells=False, density=False, particle_cells=False, ghost_pressure=True):
     """A Config for `cells` (uint8, row-major [j, i]); the caller keeps `cells`
     alive until the handle is created."""
     c = default_config()
@@ -160,7 +160,8 @@ def make_config(cells, k=2, dt=0.005, dx=0.01, rho=997.0, gravity=(0.0, -9.81), 
     c.initial_values = cells.ctypes.data
     c.tau, c.sigma, c.pcg_tol, c.pcg_max_iters = tau, sigma, tol, max_iters
     c.seed = seed
-    c.flags = (FLAG_STATIC_CELLS if static_cells else 0) | (FLAG_DENSITY if density else 0)
+    c.flags = ((FLAG_STATIC_CELLS if static_cells else 0) | (FLAG_DENSITY if density else 0) |
+               (FLAG_PARTICLE_CELLS if particle_cells else 0) | (0 if ghost_pressure else FLAG_NO_GHOST_PRESSURE))
     return c
 
 

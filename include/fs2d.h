@@ -84,10 +84,15 @@ typedef enum {
 
 typedef enum { FS2D_F32 = 0, FS2D_F64 = 1, FS2D_U8 = 2 } fs2d_dtype;
 
-/* Build-only switches (no reference counterpart). */
+/* Switches.  STATIC_CELLS and DENSITY have no reference counterpart; the
+ * other two select the reference's compile-time variants
+ * (src/FluidSim2D.cpp:15-16). */
 enum {
-    FS2D_FLAG_STATIC_CELLS = 1,  /* keep the created cell flags; no level set ("all-fluid" mode) */
-    FS2D_FLAG_DENSITY = 2        /* advect a passive cell-centred density in the SL stage */
+    FS2D_FLAG_STATIC_CELLS = 1,     /* keep the created cell flags; no level set ("all-fluid" mode) */
+    FS2D_FLAG_DENSITY = 2,          /* advect a passive cell-centred density in the SL stage */
+    FS2D_FLAG_PARTICLE_CELLS = 4,   /* USE_LEVEL_SET undefined: cells from particle positions and a
+                                       bubble fill (FluidSim2D.cpp:665-706); needs NO_GHOST_PRESSURE */
+    FS2D_FLAG_NO_GHOST_PRESSURE = 8 /* USE_GHOST_PRESSURE undefined (FluidSim2D.cpp:304-331, 486-530) */
 };
 
 /*

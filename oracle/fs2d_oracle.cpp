@@ -226,6 +226,11 @@ struct Sim {
     double tau = 0.999, sigma = 0.25, tol = 1e-12;
     int maxIters = 200;
     int slInplace = 1, levelSet = 1, useDensity = 0;
+    // Compile-time variants of the reference (src/FluidSim2D.cpp:15-16): with
+    // particleCells the cells come from the particles instead of the level
+    // set (USE_LEVEL_SET undefined); ghost = 0 drops the ghost-fluid pressure
+    // terms (USE_GHOST_PRESSURE undefined).
+    int particleCells = 0, ghost = 1;
     ArrD dens;
     // diagnostics of the last projection
     int lastIters = 0, hitMax = 0;
@@ -410,7 +415,9 @@ struct Sim {
 
     // ---- FluidSim2D::createWaterLevelSet (src/FluidSim2D.cpp:653-732) ----
     void createWaterLevelSet() {
-        if (levelSet) {
+        if (levelSet && particleCells) {
+            markCellsFromParticles();
+        } else if (levelSet) {
             constructFromParticles();
             redistance();
             for (int j = 0; j < sizeY; j++)
@@ -418,6 +425,33 @@ struct Sim {
                     if (cell(i, j) != SOLID) cell(i, j) = phi(i, j) < 0.0 ? FLUID : EMPTY;
         }
         computeDiagnostics();
+    }
+
+    // createWaterLevelSet without USE_LEVEL_SET (src/FluidSim2D.cpp:665-706):
+    // FLUID cells become EMPTY; every particle marks the cell (int)x, (int)y of
+    // its position *in metres* (the reference does not divide by dx); then, in
+    // serial order (i outer, j inner, cells updated in place), an EMPTY cell
+    // with 4 FLUID neighbours, or 3 FLUID and 1 SOLID, becomes FLUID.  Marks
+    // outside the grid (possible only when the domain exceeds size metres) are
+    // skipped here, where the reference would write out of bounds.
+    void markCellsFromParticles() {
+        for (auto& c : cell.d) if (c == FLUID) c = EMPTY;
+        for (auto& ps : pos) {
+            const int x = (int)ps.x, y = (int)ps.y;
+            if (x < 0 || x >= sizeX || y < 0 || y >= sizeY) continue;
+            if (cell(x, y) == EMPTY) cell(x, y) = FLUID;
+        }
+        for (int i = 1; i < sizeX - 1; i++)
+            for (int j = 1; j < sizeY - 1; j++) {
+                if (cell(i, j) != EMPTY) continue;
+                int fluid = 0, solid = 0;
+                const uint8_t nb[4] = {cell(i - 1, j), cell(i + 1, j), cell(i, j - 1), cell(i, j + 1)};
+                for (uint8_t c : nb) {
+                    if (c == FLUID) fluid++;
+                    else if (c == SOLID) solid++;
+                }
+                if (fluid == 4 || (fluid == 3 && solid == 1)) cell(i, j) = FLUID;
+            }
     }
 
     // Diagnostics (src/FluidSim2D.cpp:709-731).
@@ -531,6 +565,17 @@ struct Sim {
         for (int j = 0; j < sizeY; j++)
             for (int i = 0; i < sizeX; i++) {
                 if (cell(i, j) != FLUID) continue;
+                if (!ghost) {
+                    // without USE_GHOST_PRESSURE (:304-331): only the +x and +y
+                    // EMPTY neighbours add to the diagonal
+                    if (cell(i - 1, j) == FLUID) Adiag(i, j) += scaleA;
+                    if (cell(i + 1, j) == FLUID) { Adiag(i, j) += scaleA; Ax(i, j) = -scaleA; }
+                    else if (cell(i + 1, j) == EMPTY) Adiag(i, j) += scaleA;
+                    if (cell(i, j - 1) == FLUID) Adiag(i, j) += scaleA;
+                    if (cell(i, j + 1) == FLUID) { Adiag(i, j) += scaleA; Ay(i, j) = -scaleA; }
+                    else if (cell(i, j + 1) == EMPTY) Adiag(i, j) += scaleA;
+                    continue;
+                }
                 if (cell(i - 1, j) == FLUID) Adiag(i, j) += scaleA;
                 else if (cell(i - 1, j) == EMPTY) Adiag(i, j) -= scaleA * maxd(phi(i - 1, j) / phi(i, j), -1e3);
                 if (cell(i + 1, j) == FLUID) { Adiag(i, j) += scaleA; Ax(i, j) = -scaleA; }
@@ -649,6 +694,7 @@ struct Sim {
             for (int i = 0; i < sizeX; i++) {
                 if ((i > 0 && cell(i - 1, j) == FLUID) || cell(i, j) == FLUID) {
                     if ((i == 0 || cell(i - 1, j) == SOLID) || cell(i, j) == SOLID) nu(i, j) = 0;
+                    else if (!ghost) nu(i, j) -= scale * (p(i, j) - p(i - 1, j));  // i > 0 here
                     else if (i > 0 && cell(i - 1, j) == EMPTY)
                         nu(i, j) -= scale * (1 - maxd(phi(i - 1, j) / phi(i, j), -1e3)) * p(i, j);
                     else if (cell(i, j) == EMPTY)
@@ -658,6 +704,7 @@ struct Sim {
                 } else du(i, j) = UINT32_MAX;
                 if ((j > 0 && cell(i, j - 1) == FLUID) || cell(i, j) == FLUID) {
                     if ((j == 0 || cell(i, j - 1) == SOLID) || cell(i, j) == SOLID) nv(i, j) = 0;
+                    else if (!ghost) nv(i, j) -= scale * (p(i, j) - p(i, j - 1));  // j > 0 here
                     else if (j > 0 && cell(i, j - 1) == EMPTY)
                         nv(i, j) -= scale * (1 - maxd(phi(i, j - 1) / phi(i, j), -1e3)) * p(i, j);
                     else if (cell(i, j) == EMPTY)
@@ -1019,6 +1066,13 @@ void orc_set_options(void* h, double tau, double sigma, double tol, int max_iter
     Sim* s = (Sim*)h;
     s->tau = tau; s->sigma = sigma; s->tol = tol; s->maxIters = max_iters;
     s->slInplace = sl_inplace; s->useDensity = use_density;
+}
+
+// The reference's compile-time variants (src/FluidSim2D.cpp:15-16).
+void orc_set_variants(void* h, int particle_cells, int ghost) {
+    Sim* s = (Sim*)h;
+    s->particleCells = particle_cells;
+    s->ghost = ghost;
 }
 
 void orc_set_params(void* h, double dt, double gx, double gy, double alpha) {

@@ -66,6 +66,7 @@ def lib():
         L.orc_tricubic.argtypes = [vp, i, i, d, d]
         L.orc_extrapolate.argtypes = [vp, vp, i, i]
         L.orc_rand_stream.argtypes = [ctypes.c_uint, vp, i]
+        L.orc_set_variants.argtypes = [vp, i, i]
         L.orc_adjoint_step.restype = i
         L.orc_adjoint_step.argtypes = [vp] * 7
         L.orc_set_threads.argtypes = [i]
@@ -95,6 +96,12 @@ class Oracle:
     def set_options(self, tau=0.999, sigma=0.25, tol=1e-12, max_iters=200, sl_inplace=True,
                     density=False):
         self.L.orc_set_options(self.h, tau, sigma, tol, max_iters, int(sl_inplace), int(density))
+
+    def set_variants(self, particle_cells=False, ghost=True):
+        """The reference's compile-time variants: cells from particles instead
+        of the level set (USE_LEVEL_SET off), no ghost-fluid pressure
+        (USE_GHOST_PRESSURE off)."""
+        self.L.orc_set_variants(self.h, int(particle_cells), int(ghost))
 
     def set_params(self, dt, gravity, alpha):
         self.L.orc_set_params(self.h, dt, gravity[0], gravity[1], alpha)

@@ -78,3 +78,14 @@ def test_status_strings_and_kernel_names():
     names = [L.fs2d_kernel_name(k).decode() for k in range(L.fs2d_kernel_count())]
     assert "pcg_forward_solve" in names and "sl_advect" in names
     assert L.fs2d_kernel_name(-1) is None
+
+
+def test_variant_flag_combinations_rejected_without_device():
+    cells = O.dam_break(16, 16)
+    # the particle-cell variant without the no-ghost variant would divide phi = 0 by phi = 0
+    st, _ = _create(cells, flags=fs2d.FLAG_PARTICLE_CELLS)
+    assert st == fs2d.ERR_CONFIG
+    st, _ = _create(cells, flags=fs2d.FLAG_PARTICLE_CELLS | fs2d.FLAG_NO_GHOST_PRESSURE | fs2d.FLAG_STATIC_CELLS)
+    assert st == fs2d.ERR_CONFIG
+    st, _ = _create(cells, flags=64)
+    assert st == fs2d.ERR_CONFIG
