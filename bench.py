@@ -172,22 +172,24 @@ def ensemble_leg(count, steps, warmup, loop_steps):
     kt = batch.kernel_times()
     its = [s.stats()["pcg_iters"] for s in batch.sims]
     batch.close()
-    singles = [fs2d.Sim(c, **kw) for c, kw in members]
-    for _ in range(warmup):
+    tl = None
+    if loop_steps > 0:  # 0: batch only (profiling runs)
+        singles = [fs2d.Sim(c, **kw) for c, kw in members]
+        for _ in range(warmup):
+            for s in singles:
+                s.step()
+        t0 = time.perf_counter()
+        for _ in range(loop_steps):
+            for s in singles:
+                s.step()
+        tl = (time.perf_counter() - t0) / loop_steps
         for s in singles:
-            s.step()
-    t0 = time.perf_counter()
-    for _ in range(loop_steps):
-        for s in singles:
-            s.step()
-    tl = (time.perf_counter() - t0) / loop_steps
-    for s in singles:
-        s.close()
+            s.close()
     return {
         "metric": "cell-steps/s of the batched ensemble", "value": cells / tb, "unit": "cell-steps/s",
         "ms_per_step": tb * 1e3, "steps": steps, "warmup": warmup,
-        "loop_value": cells / tl, "loop_ms_per_step": tl * 1e3, "loop_steps": loop_steps,
-        "speedup_vs_loop": tl / tb,
+        "loop_value": cells / tl if tl else None, "loop_ms_per_step": tl * 1e3 if tl else None,
+        "loop_steps": loop_steps, "speedup_vs_loop": tl / tb if tl else None,
         "config": {"workload": "%d PIC/FLIP dam breaks, sides 64..512 (seed 2), block fractions U[0.2,0.7], "
                                "k=2, FLIP 0.95, dt 0.002, PCG 1e-6 masked per member" % count,
                    "members": count, "cells": cells, "pcg_iters_max": int(max(its)),
