@@ -1185,3 +1185,4 @@ const char* fs2d_status_string(int st) {
 
 #include "fs2d_adjoint_api.hpp"
 #include "fs2d_batch_api.hpp"
+#include "fs3d_api.hpp"

@@ -117,6 +117,27 @@ def load():
         "fs2d_adjoint_set_field": (i, [vp, i, i, vp, sz]),
         "fs2d_adjoint_device_field": (i, [vp, i, ctypes.POINTER(vp)]),
         "fs2d_status_string": (ctypes.c_char_p, [i]),
+        # include/fs3d.h
+        "fs3d_create": (i, [i, i, i, ctypes.POINTER(vp)]),
+        "fs3d_destroy": (None, [vp]),
+        "fs3d_last_error": (ctypes.c_char_p, [vp]),
+        "fs3d_stream": (vp, [vp]),
+        "fs3d_field_count": (sz, [vp, i]),
+        "fs3d_set_field": (i, [vp, i, i, vp, sz]),
+        "fs3d_get_field": (i, [vp, i, i, vp, sz]),
+        "fs3d_device_field": (vp, [vp, i]),
+        "fs3d_vel_interp": (i, [vp, vp, sz, vp, i]),
+        "fs3d_vel_interp_device": (i, [vp, vp, sz, vp, i]),
+        "fs3d_range_errors": (ctypes.c_long, [vp]),
+        "fs3d_vel_div": (i, [vp, i, vp, sz]),
+        "fs3d_vel_cell": (i, [vp, i, vp, sz]),
+        "fs3d_vel_face": (i, [vp, i, i, vp, sz]),
+        "fs3d_vel_div_device": (i, [vp, vp]),
+        "fs3d_vel_cell_device": (i, [vp, vp]),
+        "fs3d_vel_face_device": (i, [vp, i, vp]),
+        "fs3d_array_inner_product": (i, [vp, vp, vp, sz, ctypes.POINTER(d)]),
+        "fs3d_array_infinite_norm": (i, [vp, vp, sz, ctypes.POINTER(d)]),
+        "fs3d_array_multiply_add": (i, [vp, vp, vp, d, vp, sz]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -406,8 +427,10 @@ class Adjoint:
 
 
 def header_functions():
-    """Names of the functions include/fs2d.h declares."""
+    """Names of the functions include/fs2d.h and include/fs3d.h declare."""
     import re
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(fs2d_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for h in (HEADER, os.path.join(ROOT, "include", "fs3d.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names.update(re.findall(r"\b(fs[23]d_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)

@@ -21,7 +21,8 @@ def test_library_exports_every_header_function():
     exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
     assert set(names) <= exported
     # nothing else leaks out of the C ABI
-    assert {n for n in exported if n.startswith("fs2d_")} == set(names)
+    assert {n for n in exported if n.startswith(("fs2d_", "fs3d_"))} == set(names)
+    assert any(n.startswith("fs3d_") for n in names)
 
 
 def test_config_defaults_are_the_reference_constants():
