@@ -4,8 +4,7 @@ Workload (the single-grid configuration the headline metric is quoted on):
 a 2048 x 2048 MAC grid, all interior cells FLUID inside one-cell SOLID walls,
 semi-Lagrangian mode.  One step = semi-Lagrangian advection of u, v and a
 passive density, then the MIC(0)-preconditioned CG projection to a relative
-infinity-norm residual of 1e-6 (tau 0.999, the reference's value) and the
-velocity update with extrapolation.  Initial velocity: i.i.d. N(0,1) from
+infinity-norm residual of 1e-6 and the velocity update with extrapolation.  Initial velocity: i.i.d. N(0,1) from
 numpy.random.default_rng(1) (u then v), Gaussian-smoothed with sigma = 8 cells
 (periodic padding), projected divergence-free on the GPU; density: 16 Gaussian
 blobs drawn from the same generator; dt = 1.0 * dx / max|u| (CFL 1) fixed
@@ -34,6 +33,11 @@ dt 0.01, velocity seeded like the 2048^2 workload (seed 3), loss = L2 of the
 final density against 8 blobs (seed 4): forward rollout with a checkpoint
 every 8 steps, then the reverse pass (recomputing each segment); ratio =
 reverse time / forward time, with checkpoints and with every state kept.
+
+`grid3d`: the 3D MAC grid (include/MACGrid3D.h) at 256^3 with N(0,1) FP32
+u, v, w: velInterp (Catmull-Rom in the velocity tap order) at 2^24 uniform
+random points, once in random order and once sorted by cell, and velDiv at
+every cell (16 algorithmic bytes per cell); HIP events on the grid's stream.
 
 `cpu_baseline` times the oracle (FP64 CPU restatement of the reference step,
 OpenMP) on the same starting state with PCG capped at two iteration counts,
@@ -246,6 +250,73 @@ def adjoint_leg(n=256, steps=64, every=8):
     }
 
 
+def grid3d_leg(n=256, npts=1 << 24, reps=10):
+    """The 3D MAC grid (MACGrid3D.h): velInterp at npts points of an n^3 grid,
+    random and cell-sorted, and velDiv over every cell; kernel times from HIP
+    events recorded on the grid's stream."""
+    import ctypes
+    import fs2d
+    from fs2d import grid3d as G
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime libfs2d.so is linked to
+    vp = ctypes.c_void_p
+    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [vp]
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+    hip.hipEventRecord.argtypes = [vp, vp]
+    hip.hipEventSynchronize.argtypes = [vp]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+    hip.hipEventDestroy.argtypes = [vp]
+
+    def dmalloc(nbytes):
+        p = vp()
+        assert hip.hipMalloc(ctypes.byref(p), nbytes) == 0
+        return p
+
+    g = G.MACGrid3D(n, n, n)
+    r = np.random.default_rng(11)
+    for c in (G.U, G.V, G.W):
+        g.set(c, r.standard_normal(g.shape(c), dtype=np.float32))
+    pos = r.uniform(0.0, n, (npts, 3)).astype(np.float32)
+    key = (pos[:, 2].astype(np.int64) * n + pos[:, 1].astype(np.int64)) * n + pos[:, 0].astype(np.int64)
+    sorted_pos = np.ascontiguousarray(pos[np.argsort(key, kind="stable")])
+    dpos, dout, ddiv = dmalloc(pos.nbytes), dmalloc(pos.nbytes), dmalloc(4 * n ** 3)
+    stream = g.L.fs3d_stream(g.h)
+    ev = [vp(), vp()]
+    for e in ev:
+        assert hip.hipEventCreate(ctypes.byref(e)) == 0
+
+    def timed(launch):
+        launch()  # warm
+        assert hip.hipEventRecord(ev[0], stream) == 0
+        for _ in range(reps):
+            launch()
+        assert hip.hipEventRecord(ev[1], stream) == 0
+        assert hip.hipEventSynchronize(ev[1]) == 0
+        ms = ctypes.c_float()
+        assert hip.hipEventElapsedTime(ctypes.byref(ms), ev[0], ev[1]) == 0
+        return ms.value / reps * 1e-3
+
+    res = {}
+    for name, pts in (("random", pos), ("cell_sorted", sorted_pos)):
+        assert hip.hipMemcpy(dpos, pts.ctypes.data, pts.nbytes, 1) == 0
+        t = timed(lambda: g.L.fs3d_vel_interp_device(g.h, dpos, npts, dout, G.SAMPLE_VELOCITY))
+        res["vel_interp_%s" % name] = {"samples_per_s": npts / t, "us_per_launch": t * 1e6}
+    assert g.L.fs3d_range_errors(g.h) == 0
+    t = timed(lambda: g.L.fs3d_vel_div_device(g.h, ddiv))
+    div_bytes = 16 * n ** 3  # u, v, w in and the divergence out, 4 bytes each per cell
+    res["vel_div"] = {"cells_per_s": n ** 3 / t, "us_per_launch": t * 1e6, "GB_per_s": div_bytes / t / 1e9,
+                      "hbm_frac": div_bytes / t / 1e9 / HBM_PEAK_GBS}
+    for e in ev:
+        hip.hipEventDestroy(e)
+    for p_ in (dpos, dout, ddiv):
+        hip.hipFree(p_)
+    g.close()
+    res["config"] = {"workload": "%d^3 MAC grid, FP32 N(0,1) u, v, w; velInterp (velocity taps) at %d uniform "
+                                 "random points, in random and in cell order; velDiv at every cell" % (n, npts)}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,7 +333,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=5)
     ap.add_argument("--loop-steps", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
-    ap.add_argument("--only", choices=["ensemble", "adjoint"],
+    ap.add_argument("--no-grid3d", action="store_true", help="skip the 3D grid leg")
+    ap.add_argument("--only", choices=["ensemble", "adjoint", "grid3d"],
                     help="run one leg alone and print its object (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -276,6 +348,9 @@ def main():
         return
     if args.only == "adjoint":
         print(json.dumps(adjoint_leg()))
+        return
+    if args.only == "grid3d":
+        print(json.dumps(grid3d_leg()))
         return
     n = args.n
     sim, cells, dt = build_gpu_sim(n, args.tol, args.tau, args.max_iters)
@@ -337,6 +412,8 @@ def main():
         out["ensemble"] = ensemble_leg(args.ensemble_members, args.ensemble_steps, 2, args.loop_steps)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()
+    if not args.no_grid3d:
+        out["grid3d"] = grid3d_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(state, cells, n, dt, args.tol, args.tau, args.cpu_threads,
                                            float(np.mean(iters)))
