@@ -431,10 +431,14 @@ struct BwdMaker {
         return BwdOp{{I.q, I.P, I.gx_, I.gy_}, nullptr, {I.z}, I.sc, qq, h + 1};
     }
 };
+// The batched passes use a 4-supergroup ring: 50 KB of LDS instead of 98 KB,
+// so two workgroups fit a CU and a launch over a thousand strips needs half
+// the rounds of residency.  The ring depth changes no arithmetic.
+constexpr int kBatchRing = 4;
 template <class Op, bool BACKWARD, class Maker>
 __global__ __launch_bounds__(kSweepThreads) void b_sweep_kernel(const InstBufs* __restrict__ ins, StripMap sm,
                                                                 unsigned long long* ticket, int* err, Maker mk) {
-    __shared__ __align__(16) SweepLds<Op> L;
+    __shared__ __align__(16) SweepLds<Op, kBatchRing> L;
     const unsigned long long t = sweep_ticket(L, ticket);
     const int gs = (int)(t % (unsigned long long)sm.total);
     const unsigned epoch = (unsigned)(t / (unsigned long long)sm.total) + 1u;

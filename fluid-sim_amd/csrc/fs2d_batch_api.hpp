@@ -159,10 +159,20 @@ fs2d_status build_maps(fs2d_batch* B) {
     B->g_sk = add_map(H, c_sk, o_sk);
     B->g_tiles = add_map(H, c_tiles, o_tiles);
     B->g_one = add_map(H, c_one, o_one);
+    // Strip list of the wavefront passes: each member's strips contiguous and
+    // in order (a strip's upstream neighbour is taken earlier), members with
+    // the longest strip chains first, so that their pipelines start in the
+    // first round of residency and the short members fill in behind them.
+    // Which workgroup runs a strip does not change any result.
     const size_t strip_inst = H.size();
     int total = 0;
-    std::vector<int> sfirst(n);
-    for (int b = 0; b < n; b++) {
+    std::vector<int> sfirst(n), order(n);
+    for (int b = 0; b < n; b++) order[b] = b;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const fs2d_sim *sa = B->sims[a], *sb = B->sims[b];
+        return nstrips[a] != nstrips[b] ? nstrips[a] > nstrips[b] : sa->nx > sb->nx;
+    });
+    for (int b : order) {
         sfirst[b] = total;
         for (int k = 0; k < nstrips[b]; k++) H.push_back(b);
         total += nstrips[b];

@@ -86,13 +86,17 @@ enum : uint32_t { CPL_L = 1, CPL_R = 2, CPL_D = 4, CPL_U = 8 };
 constexpr int kSub = 4;         // SK4 groups per supergroup (16 steps)
 constexpr int kLoaders = 3;
 constexpr int kBatch = 2;       // supergroups per loader batch
-constexpr int kRing = 8;        // supergroups in the LDS ring (a power of two)
+constexpr int kRing = 8;        // supergroups in the LDS ring of a single launch (a power of two)
 constexpr int kHandRing = 256;  // steps in the hand-off ring
 constexpr int kPollers = 2;     // poller waves: their polls interleave in time
 constexpr int kSweepThreads = 64 * (1 + kLoaders + kPollers);
 constexpr int kLdsSpin = 1 << 26;  // bounded LDS waits (~seconds)
+// A ring of kLoaders * kBatch + 2 supergroups lets every loader keep its batch
+// in flight.  A smaller ring (batched launches, where more workgroups per CU
+// matter more) only makes loaders wait for free slots: a loader waits for the
+// consumption of supergroups below its own, which other loaders fill without
+// waiting on it, so any power of two >= 2 is deadlock-free.
 static_assert(kRing >= kLoaders * kBatch + 2, "ring too small for the loaders");
-static_assert((kRing & (kRing - 1)) == 0, "ring slots are taken modulo kRing");
 
 __device__ __forceinline__ float f4get(const float4& v, int q) {
     return q == 0 ? v.x : (q == 1 ? v.y : (q == 2 ? v.z : v.w));
@@ -126,13 +130,14 @@ __device__ __forceinline__ uint32_t nbr(uint32_t edge, uint32_t v) {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, ctl, 0xf, 0xf, true);
 }
 
-template <class Op>
+template <class Op, int RING = kRing>
 struct SweepLds {
-    float4 ring_f[kRing][kSub][Op::NF][64];
-    uint32_t ring_b[Op::NB ? kRing : 1][kSub][64];
+    static_assert(RING >= 2 && (RING & (RING - 1)) == 0, "ring slots are taken modulo RING");
+    float4 ring_f[RING][kSub][Op::NF][64];
+    uint32_t ring_b[Op::NB ? RING : 1][kSub][64];
     uint32_t hand[Op::NW][kHandRing];
     uint32_t outbox[2][Op::NW][16];
-    unsigned slot_tag[kRing];
+    unsigned slot_tag[RING];
     unsigned consumed_sg, hand_ready, abort_flag;
 };
 
@@ -152,8 +157,8 @@ struct SweepLds {
 //         lane's state after the previous step, updated; o[n][q] output n;
 //         prod[w][q] state word w after step q
 //   void strip_done(int s)     wave 0, after the last step
-template <class Op, bool B, bool C, bool P>
-__device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<Op>& L, int s, unsigned epoch,
+template <class Op, bool B, bool C, bool P, int RING>
+__device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<Op, RING>& L, int s, unsigned epoch,
                                               unsigned long long* dst, int* err) {
     constexpr int NW = Op::NW, NF = Op::NF, NB = Op::NB, NOUT = Op::NOUT;
     const int lane = threadIdx.x & 63;
@@ -174,7 +179,7 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
     auto x_ready = [&](int sg, unsigned tag) { return tag == (unsigned)sg + 1u; };
     auto h_ready = [&](int sg, unsigned hr) { return hr >= 16u * (unsigned)(sg + 1); };
     auto fetch_x = [&](int sg, Buf& f) {
-        const int slot = sg & (kRing - 1);
+        const int slot = sg & (RING - 1);
 #pragma unroll
         for (int j = 0; j < kSub; j++) {
 #pragma unroll
@@ -201,7 +206,7 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
         __builtin_amdgcn_s_sleep(0);
     };
     auto wait_x = [&](int sg) {
-        for (int spin = 0; !bad && !x_ready(sg, uniform(lds_ld(&L.slot_tag[sg & (kRing - 1)]))); spin++) give_up(spin);
+        for (int spin = 0; !bad && !x_ready(sg, uniform(lds_ld(&L.slot_tag[sg & (RING - 1)]))); spin++) give_up(spin);
     };
     auto wait_h = [&](int sg) {
         for (int spin = 0; !bad && !h_ready(sg, uniform(lds_ld(&L.hand_ready))); spin++) give_up(spin);
@@ -224,7 +229,7 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
     auto step = [&](int sg, Buf& cur, Buf& nxt) {
         const int t0 = sg_t0(sg);
         const int sn = min(sg + 1, NSG - 1);
-        const unsigned tagv = lds_ld(&L.slot_tag[sn & (kRing - 1)]);
+        const unsigned tagv = lds_ld(&L.slot_tag[sn & (RING - 1)]);
         const unsigned hrv = C ? lds_ld(&L.hand_ready) : 0u;
         fetch_h(sn, nxt);
         fetch_x(sn, nxt);
@@ -294,8 +299,8 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
 }
 
 // Workgroup prologue: take a ticket and clear the LDS counters.
-template <class Op>
-__device__ __forceinline__ unsigned long long sweep_ticket(SweepLds<Op>& L, unsigned long long* ticket) {
+template <class Op, int RING>
+__device__ __forceinline__ unsigned long long sweep_ticket(SweepLds<Op, RING>& L, unsigned long long* ticket) {
     __shared__ unsigned long long sh_ticket;
     if (threadIdx.x == 0) {
         sh_ticket = atomicAdd(ticket, 1ull);
@@ -303,7 +308,7 @@ __device__ __forceinline__ unsigned long long sweep_ticket(SweepLds<Op>& L, unsi
         L.hand_ready = 0;
         L.abort_flag = 0;
     }
-    if (threadIdx.x < kRing) L.slot_tag[threadIdx.x] = 0;
+    if (threadIdx.x < RING) L.slot_tag[threadIdx.x] = 0;
     __syncthreads();
     return sh_ticket;
 }
@@ -311,9 +316,9 @@ __device__ __forceinline__ unsigned long long sweep_ticket(SweepLds<Op>& L, unsi
 // One strip of a pass: `order` is the strip's position in the pass (strip
 // order for forward passes, reversed for backward ones), `epoch` tags its
 // hand-off words.
-template <class Op, bool BACKWARD>
+template <class Op, bool BACKWARD, int RING>
 __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& c, int order, unsigned epoch,
-                                          SweepLds<Op>& L) {
+                                          SweepLds<Op, RING>& L) {
     constexpr int NW = Op::NW, NIN = Op::NIN, NF = Op::NF, NB = Op::NB;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     __shared__ int sh_skip;
@@ -357,8 +362,8 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
         double acc = 0.0;
         for (int sb = me * kBatch; sb < NSG; sb += kLoaders * kBatch) {
             const int last = min(sb + kBatch, NSG) - 1;
-            // slot of supergroup sg is free once sg - kRing has been consumed
-            for (int spin = 0; !bad && (int)lds_ld(&L.consumed_sg) < last - kRing + 1; spin++) {
+            // slot of supergroup sg is free once sg - RING has been consumed
+            for (int spin = 0; !bad && (int)lds_ld(&L.consumed_sg) < last - RING + 1; spin++) {
                 if (spin >= kLdsSpin || lds_ld(&L.abort_flag)) {
                     bad = true;
                     raise();
@@ -382,7 +387,7 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
 #pragma unroll
             for (int k = 0; k < kBatch; k++) {
                 if (sb + k < NSG) {
-                    const int slot = (sb + k) & (kRing - 1);
+                    const int slot = (sb + k) & (RING - 1);
 #pragma unroll
                     for (int j = 0; j < kSub; j++) {
                         float4 x[NF];
@@ -394,7 +399,7 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
                 }
             }
             if (lane == 0)
-                for (int sg = sb; sg <= last; sg++) lds_st(&L.slot_tag[sg & (kRing - 1)], (unsigned)sg + 1u);
+                for (int sg = sb; sg <= last; sg++) lds_st(&L.slot_tag[sg & (RING - 1)], (unsigned)sg + 1u);
         }
         op.loader_done(s, me, acc);
     } else if (consume) {
