@@ -40,7 +40,9 @@ struct InstBufs {
     PcgScalars* sc;
     double *qq, *ws_part, *rmax_part;
     uint32_t *mu, *mv;
-    int *fu0, *fu1, *fv0, *fv1, *fcount;
+    int *fu0, *fu1, *fv0, *fv1, *fcount, *fpos;
+    int4* fsrc;
+    float4* fkv;
     float *px, *py, *pvx, *pvy;
     int *pkey, *counts, *starts, *cursor, *sorted, *tile_sums;
     Diag* dg;
@@ -192,24 +194,46 @@ __global__ __launch_bounds__(kBlock) void b_update_velocity_kernel(const InstBuf
 
 // Extrapolation of (u, v) (which = 0, after P2G) or (nu, nv) (which = 1).
 __device__ __forceinline__ void b_extra_arrays(const InstBufs& I, int which, ExtraArr& A, ExtraArr& B) {
-    A = ExtraArr{which ? I.nu : I.u, I.mu, I.fu0, I.fu1, I.fcount + 0, I.nx + 1, I.ny};
-    B = ExtraArr{which ? I.nv : I.v, I.mv, I.fv0, I.fv1, I.fcount + 1, I.nx, I.ny + 1};
+    const long nU = (long)(I.nx + 1) * I.ny;
+    A = ExtraArr{G(which ? I.nu : I.u), G(I.mu),  G(I.fu0), G(I.fpos), G((i4v*)I.fsrc), G((f4v*)I.fkv), G(I.fu1),
+                 G(I.fcount + 0),       I.nx + 1, I.ny};
+    B = ExtraArr{G(which ? I.nv : I.v), G(I.mv),        G(I.fv0),         G(I.fpos + nU), G((i4v*)(I.fsrc + nU)),
+                 G((f4v*)(I.fkv + nU)),         G(I.fv1),       G(I.fcount + 1),  I.nx,           I.ny + 1};
 }
-__global__ __launch_bounds__(kBlock) void b_extrap_layer1_kernel(const InstBufs* __restrict__ ins, BlockMap bm,
-                                                                 int which) {
+enum BatchExtra { BX_ROWS, BX_HIST, BX_SCATTER, BX_LINK };
+template <int WHAT>
+__global__ __launch_bounds__(kBlock) void b_extrap_kernel(const InstBufs* __restrict__ ins, BlockMap bm, int which) {
     int b, vb, nvb;
     if (!bmap(bm, b, vb, nvb)) return;
     ExtraArr A, B;
     b_extra_arrays(ins[b], which, A, B);
-    extrap_layer1_body(A, B, vb, nvb);
+    if (WHAT == BX_ROWS) extrap_rows_body(A, B, vb, nvb);
+    else if (WHAT == BX_HIST) extrap_hist_body(A, B, vb, nvb);
+    else if (WHAT == BX_SCATTER) extrap_scatter_body(A, B, vb, nvb);
+    else extrap_link_body(A, B, vb, nvb);
+}
+__global__ __launch_bounds__(kExtraColThreads) void b_extrap_cols_kernel(const InstBufs* __restrict__ ins, BlockMap bm,
+                                                                         int which) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    ExtraArr A, B;
+    b_extra_arrays(ins[b], which, A, B);
+    extrap_cols_body(A, B, vb, nvb);
 }
 // Two workgroups per simulation: u and v.
-__global__ __launch_bounds__(1024) void b_extrap_layers_kernel(const InstBufs* __restrict__ ins, int n, int which) {
+__global__ __launch_bounds__(kBlock) void b_extrap_scan_kernel(const InstBufs* __restrict__ ins, int n, int which) {
     const int b = blockIdx.x >> 1;
     if (b >= n) return;
     ExtraArr A, B;
     b_extra_arrays(ins[b], which, A, B);
-    extrap_layers_body(A, B, blockIdx.x & 1, 2);
+    extrap_scan_body(A, B, blockIdx.x & 1);
+}
+__global__ __launch_bounds__(1024) void b_extrap_values_kernel(const InstBufs* __restrict__ ins, int n, int which) {
+    const int b = blockIdx.x >> 1;
+    if (b >= n) return;
+    ExtraArr A, B;
+    b_extra_arrays(ins[b], which, A, B);
+    extrap_values_body(A, B, blockIdx.x & 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -349,14 +373,14 @@ template <int FX, int FY>
 struct ClosestMaker {
     int parity;
     __device__ ClosestOp<FX, FY> operator()(const InstBufs& I) const {
-        return ClosestOp<FX, FY>{I.lsc, parity ? I.t0 : I.t1, I.nx, I.ny, I.dx, I.dr};
+        return ClosestOp<FX, FY>{G(I.lsc), G(parity ? I.t0 : I.t1), I.nx, I.ny, I.dx, I.dr};
     }
 };
 template <int FX, int FY>
 struct EikonalMaker {
     int parity;
     __device__ EikonalOp<FX, FY> operator()(const InstBufs& I) const {
-        return EikonalOp<FX, FY>{parity ? I.phi2 : I.phi, parity ? I.phi : I.phi2, I.nx, I.ny, I.dx};
+        return EikonalOp<FX, FY>{G(parity ? I.phi2 : I.phi), G(parity ? I.phi : I.phi2), I.nx, I.ny, I.dx};
     }
 };
 // Cells from particles (createWaterLevelSet without USE_LEVEL_SET).

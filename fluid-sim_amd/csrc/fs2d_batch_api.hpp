@@ -20,8 +20,9 @@ struct fs2d_batch {
     std::vector<InstBufs> h_ins;
     InstBufs* d_ins = nullptr;
     int* d_maps = nullptr;
-    BlockMap m_cells{}, m_faces{}, m_sl{}, m_part{}, m_diag{}, m_sk{}, m_tiles{}, m_one{};
-    int g_cells = 0, g_faces = 0, g_sl = 0, g_part = 0, g_diag = 0, g_sk = 0, g_tiles = 0, g_one = 0;
+    BlockMap m_cells{}, m_faces{}, m_sl{}, m_part{}, m_diag{}, m_sk{}, m_tiles{}, m_one{}, m_xrows{}, m_xcols{};
+    int g_cells = 0, g_faces = 0, g_sl = 0, g_part = 0, g_diag = 0, g_sk = 0, g_tiles = 0, g_one = 0, g_xrows = 0,
+        g_xcols = 0;
     StripMap strips{};
     unsigned* ls_ctl = nullptr;
     int* werr = nullptr;
@@ -97,6 +98,7 @@ InstBufs inst_bufs(const fs2d_sim* s, unsigned long long* ls_handoff, unsigned l
     I.Adiag = s->Adiag; I.P = s->P; I.fx = s->fx; I.fy = s->fy; I.gx_ = s->gx_; I.gy_ = s->gy_; I.cpl = s->cpl;
     I.sc = s->sc; I.qq = s->qq; I.ws_part = s->ws_part; I.rmax_part = s->rmax_part;
     I.mu = s->mu; I.mv = s->mv; I.fu0 = s->fu0; I.fu1 = s->fu1; I.fv0 = s->fv0; I.fv1 = s->fv1; I.fcount = s->fcount;
+    I.fpos = s->fpos; I.fsrc = s->fsrc; I.fkv = s->fkv;
     I.px = s->px; I.py = s->py; I.pvx = s->pvx; I.pvy = s->pvy;
     I.pkey = s->pkey; I.counts = s->counts; I.starts = s->starts; I.cursor = s->cursor; I.sorted = s->sorted;
     I.tile_sums = s->tile_sums; I.dg = s->dg; I.partials = s->partials;
@@ -134,6 +136,7 @@ int add_map(std::vector<int>& H, const std::vector<int>& cnt, MapOff& off) {
 fs2d_status build_maps(fs2d_batch* B) {
     const int n = B->n;
     std::vector<int> c_cells(n), c_faces(n), c_sl(n), c_part(n), c_diag(n), c_sk(n), c_tiles(n), c_one(n, 1);
+    std::vector<int> c_xrows(n), c_xcols(n);
     std::vector<int> nstrips(n);
     B->any_particles = false;
     for (int b = 0; b < n; b++) {
@@ -146,11 +149,13 @@ fs2d_status build_maps(fs2d_batch* B) {
         c_diag[b] = blocks_for(nc + s->np);
         c_sk[b] = sk_blocks(s);
         c_tiles[b] = (int)((nc + 1023) / 1024);
+        c_xrows[b] = extrap_row_blocks(s->ny);
+        c_xcols[b] = extrap_col_blocks(s->nx);
         nstrips[b] = (s->ny + 63) / 64;
         B->any_particles = B->any_particles || s->np > 0;
     }
     std::vector<int> H;
-    MapOff o_cells, o_faces, o_sl, o_part, o_diag, o_sk, o_tiles, o_one;
+    MapOff o_cells, o_faces, o_sl, o_part, o_diag, o_sk, o_tiles, o_one, o_xrows, o_xcols;
     B->g_cells = add_map(H, c_cells, o_cells);
     B->g_faces = add_map(H, c_faces, o_faces);
     B->g_sl = add_map(H, c_sl, o_sl);
@@ -159,6 +164,8 @@ fs2d_status build_maps(fs2d_batch* B) {
     B->g_sk = add_map(H, c_sk, o_sk);
     B->g_tiles = add_map(H, c_tiles, o_tiles);
     B->g_one = add_map(H, c_one, o_one);
+    B->g_xrows = add_map(H, c_xrows, o_xrows);
+    B->g_xcols = add_map(H, c_xcols, o_xcols);
     // Strip list of the wavefront passes: each member's strips contiguous and
     // in order (a strip's upstream neighbour is taken earlier), members with
     // the longest strip chains first, so that their pipelines start in the
@@ -187,6 +194,7 @@ fs2d_status build_maps(fs2d_batch* B) {
     auto mk = [&](const MapOff& o) { return BlockMap{maps + o.inst, maps + o.first, maps + o.count}; };
     B->m_cells = mk(o_cells); B->m_faces = mk(o_faces); B->m_sl = mk(o_sl); B->m_part = mk(o_part);
     B->m_diag = mk(o_diag); B->m_sk = mk(o_sk); B->m_tiles = mk(o_tiles); B->m_one = mk(o_one);
+    B->m_xrows = mk(o_xrows); B->m_xcols = mk(o_xcols);
     B->strips = StripMap{maps + strip_inst, maps + strip_first, total};
     return FS2D_OK;
 }
@@ -326,9 +334,13 @@ void bstage_level_set(fs2d_batch* B) {
 
 void bstage_extrapolate(fs2d_batch* B, int which) {
     Timed t(B, K_EXTRAPOLATE);
-    blaunch(B, b_fill_kernel, B->g_one, kBlock, B->d_ins, B->m_one, (int)BF_FCOUNT);
-    blaunch(B, b_extrap_layer1_kernel, B->g_faces, kBlock, B->d_ins, B->m_faces, which);
-    blaunch(B, b_extrap_layers_kernel, 2 * B->n, 1024, B->d_ins, B->n, which);
+    blaunch(B, b_extrap_kernel<BX_ROWS>, B->g_xrows, kBlock, B->d_ins, B->m_xrows, which);
+    blaunch(B, b_extrap_cols_kernel, B->g_xcols, kExtraColThreads, B->d_ins, B->m_xcols, which);
+    blaunch(B, b_extrap_kernel<BX_HIST>, B->g_faces, kBlock, B->d_ins, B->m_faces, which);
+    blaunch(B, b_extrap_scan_kernel, 2 * B->n, kBlock, B->d_ins, B->n, which);
+    blaunch(B, b_extrap_kernel<BX_SCATTER>, B->g_faces, kBlock, B->d_ins, B->m_faces, which);
+    blaunch(B, b_extrap_kernel<BX_LINK>, B->g_faces, kBlock, B->d_ins, B->m_faces, which);
+    blaunch(B, b_extrap_values_kernel, 2 * B->n, 1024, B->d_ins, B->n, which);
 }
 
 void bstage_p2g(fs2d_batch* B) {

@@ -78,7 +78,12 @@ struct fs2d_sim {
     unsigned long long* sk_handoff = nullptr;  // separate from the level-set engine's: epochs differ
     // extrapolation
     uint32_t *mu = nullptr, *mv = nullptr;
-    int *fu0 = nullptr, *fu1 = nullptr, *fv0 = nullptr, *fv1 = nullptr, *fcount = nullptr;
+    int *fu0 = nullptr, *fv0 = nullptr;    // entries in layer order
+    int *fu1 = nullptr, *fv1 = nullptr;    // per-layer counts, offsets, cursors
+    int* fcount = nullptr;                 // layers of u, v
+    int* fpos = nullptr;                   // u entries then v entries: position in layer order
+    int4* fsrc = nullptr;                  // sources of each ordered entry (u then v)
+    float4* fkv = nullptr;                 // known sources' values (u then v)
     // particles
     float *px = nullptr, *py = nullptr, *pvx = nullptr, *pvy = nullptr;
     long np = 0, np_cap = 0;
@@ -280,13 +285,27 @@ void sort_particles(fs2d_sim* s) {
     s->sorted_valid = true;
 }
 
+int extrap_row_blocks(int ny) { return std::max(1, std::min((2 * ny + 1 + 3) / 4, kMaxBlocks)); }
+int extrap_col_blocks(int nx) { return (nx + 1 + 63) / 64 + (nx + 63) / 64; }
+
+// Array2D::extrapolate of u and v (include/Array2D.h:552-591), from masks that
+// mark the known faces 0 and the others ~0.
 void extrapolate_uv(fs2d_sim* s, float* uval, float* vval) {
     Timed t(s, K_EXTRAPOLATE);
-    ExtraArr A{uval, s->mu, s->fu0, s->fu1, s->fcount + 0, s->nx + 1, s->ny};
-    ExtraArr B{vval, s->mv, s->fv0, s->fv1, s->fcount + 1, s->nx, s->ny + 1};
-    fill(s, s->fcount, 0, 2);
-    hipLaunchKernelGGL(extrap_layer1_kernel, dim3(blocks_for(n_u(s) + n_v(s))), dim3(kBlock), 0, s->stream, A, B);
-    hipLaunchKernelGGL(extrap_layers_kernel, dim3(2), dim3(1024), 0, s->stream, A, B);
+    const size_t nU = n_u(s);
+    ExtraArr A{G(uval),     G(s->mu),  G(s->fu0), G(s->fpos), G((i4v*)s->fsrc), G((f4v*)s->fkv), G(s->fu1), G(s->fcount + 0),
+               s->nx + 1, s->ny};
+    ExtraArr B{G(vval),    G(s->mv),        G(s->fv0),      G(s->fpos + nU), G((i4v*)(s->fsrc + nU)),
+               G((f4v*)(s->fkv + nU)), G(s->fv1), G(s->fcount + 1), s->nx,         s->ny + 1};
+    const int nb = blocks_for(n_u(s) + n_v(s));
+    hipLaunchKernelGGL(extrap_rows_kernel, dim3(extrap_row_blocks(s->ny)), dim3(kBlock), 0, s->stream, A, B);
+    hipLaunchKernelGGL(extrap_cols_kernel, dim3(extrap_col_blocks(s->nx)), dim3(kExtraColThreads), 0, s->stream, A,
+                       B);
+    hipLaunchKernelGGL(extrap_hist_kernel, dim3(nb), dim3(kBlock), 0, s->stream, A, B);
+    hipLaunchKernelGGL(extrap_scan_kernel, dim3(2), dim3(kBlock), 0, s->stream, A, B);
+    hipLaunchKernelGGL(extrap_scatter_kernel, dim3(nb), dim3(kBlock), 0, s->stream, A, B);
+    hipLaunchKernelGGL(extrap_link_kernel, dim3(nb), dim3(kBlock), 0, s->stream, A, B);
+    hipLaunchKernelGGL(extrap_values_kernel, dim3(2), dim3(1024), 0, s->stream, A, B);
 }
 
 void diagnostics(fs2d_sim* s) {
@@ -331,10 +350,10 @@ void stage_level_set(fs2d_sim* s) {
                 }
                 Timed t(s, K_LS_SWEEP);
                 switch (d) {
-                    case 0: launch_wavefront(s, ClosestOp<0, 0>{s->lsc, tout, s->nx, s->ny, s->dx, s->dr}); break;
-                    case 1: launch_wavefront(s, ClosestOp<1, 0>{s->lsc, tout, s->nx, s->ny, s->dx, s->dr}); break;
-                    case 2: launch_wavefront(s, ClosestOp<0, 1>{s->lsc, tout, s->nx, s->ny, s->dx, s->dr}); break;
-                    default: launch_wavefront(s, ClosestOp<1, 1>{s->lsc, tout, s->nx, s->ny, s->dx, s->dr}); break;
+                    case 0: launch_wavefront(s, ClosestOp<0, 0>{G(s->lsc), G(tout), s->nx, s->ny, s->dx, s->dr}); break;
+                    case 1: launch_wavefront(s, ClosestOp<1, 0>{G(s->lsc), G(tout), s->nx, s->ny, s->dx, s->dr}); break;
+                    case 2: launch_wavefront(s, ClosestOp<0, 1>{G(s->lsc), G(tout), s->nx, s->ny, s->dx, s->dr}); break;
+                    default: launch_wavefront(s, ClosestOp<1, 1>{G(s->lsc), G(tout), s->nx, s->ny, s->dx, s->dr}); break;
                 }
                 std::swap(tin, tout);
             }
@@ -351,10 +370,10 @@ void stage_level_set(fs2d_sim* s) {
             for (int d = 0; d < 4; d++) {
                 Timed t(s, K_LS_SWEEP);
                 switch (d) {
-                    case 0: launch_wavefront(s, EikonalOp<0, 0>{a, b, s->nx, s->ny, s->dx}); break;
-                    case 1: launch_wavefront(s, EikonalOp<1, 0>{a, b, s->nx, s->ny, s->dx}); break;
-                    case 2: launch_wavefront(s, EikonalOp<0, 1>{a, b, s->nx, s->ny, s->dx}); break;
-                    default: launch_wavefront(s, EikonalOp<1, 1>{a, b, s->nx, s->ny, s->dx}); break;
+                    case 0: launch_wavefront(s, EikonalOp<0, 0>{G(a), G(b), s->nx, s->ny, s->dx}); break;
+                    case 1: launch_wavefront(s, EikonalOp<1, 0>{G(a), G(b), s->nx, s->ny, s->dx}); break;
+                    case 2: launch_wavefront(s, EikonalOp<0, 1>{G(a), G(b), s->nx, s->ny, s->dx}); break;
+                    default: launch_wavefront(s, EikonalOp<1, 1>{G(a), G(b), s->nx, s->ny, s->dx}); break;
                 }
                 std::swap(a, b);
             }
@@ -728,7 +747,9 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
     A(r, nS); A(z, nS); A(q, nS); A(w, nS); A(s[0], nS); A(s[1], nS); A(pk, nS);
     A(Adiag, nS); A(P, nS); A(fx, nS); A(fy, nS); A(gx_, nS); A(gy_, nS); A(cpl, nS);
     A(sc, 1); A(qq, 2 * (size_t)s->geom.S * kLoaders); A(ws_part, kMaxBlocks); A(rmax_part, kMaxBlocks); A(sk_ticket, 1);
-    A(mu, nU); A(mv, nV); A(fu0, nU); A(fu1, nU); A(fv0, nV); A(fv1, nV); A(fcount, 2);
+    A(mu, nU); A(mv, nV); A(fu0, nU); A(fv0, nV); A(fcount, 2);
+    A(fu1, extra_aux_len(s->nx + 1, s->ny)); A(fv1, extra_aux_len(s->nx, s->ny + 1));
+    A(fpos, nU + nV); A(fsrc, nU + nV); A(fkv, nU + nV);
     A(counts, nC); A(starts, nC); A(cursor, nC); A(tile_sums, (nC + 1023) / 1024 + 1);
     A(dg, 1); A(partials, 3 * kMaxBlocks);
     A(wctl, 4); A(werr, 1);

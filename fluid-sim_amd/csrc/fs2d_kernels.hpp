@@ -277,6 +277,14 @@ __device__ __forceinline__ double block_sum_bcast(const double* a, int n) {
 // w.s' (:450).  sigma_k = z.r comes from the forward solves' per-strip q.q
 // partials; beta = sigma_k / sigma_{k-1} (:457-458).  s is double-buffered
 // because every cell also needs s' of its four neighbours.
+//
+// Neighbour offsets in SK4 (element e: step q = e & 3, lane l = (e >> 2) & 63):
+// (i-1, j) is step t-1 of the lane, (i+1, j) step t+1; (i, j-1) is lane l-1 at
+// step t-1 and (i, j+1) lane l+1 at step t+1, both inside the strip except on
+// its edge lanes, which take the general index.  FLUID cells are interior, so
+// every neighbour exists.
+__device__ __forceinline__ long sk_step_back(long e) { return (e & 3) ? e - 1 : e - 253; }    // t-1, same lane
+__device__ __forceinline__ long sk_step_fwd(long e) { return (e & 3) != 3 ? e + 1 : e + 253; }  // t+1, same lane
 __device__ __forceinline__ void pcg_spmv_sk_body(
     SkGeom g, const float* __restrict__ Adiag, const uint8_t* __restrict__ flags, const float* __restrict__ z,
     const float* __restrict__ s_old, float* __restrict__ s_new, float* __restrict__ w, float sA,
@@ -295,14 +303,23 @@ __device__ __forceinline__ void pcg_spmv_sk_body(
         const float ad = Adiag[e];
         float wv = 0.0f;
         if (ad != 0.0f) {  // FLUID: interior, all four neighbours exist
-            int i, j;
-            sk_cell(g, e, i, j);
             const uint32_t f = flags[e];
+            const int l = (int)((e >> 2) & 63);
+            long nd, nu;
+            if (l > 0 && l < 63) {
+                nd = sk_step_back(e) - 4;
+                nu = sk_step_fwd(e) + 4;
+            } else {
+                int i, j;
+                sk_cell(g, e, i, j);
+                nd = g.idx(i, j - 1);
+                nu = g.idx(i, j + 1);
+            }
             float nsum = 0.0f;
-            if (f & CPL_L) { const long n = g.idx(i - 1, j); nsum += fmaf(beta, s_old[n], z[n]); }
-            if (f & CPL_R) { const long n = g.idx(i + 1, j); nsum += fmaf(beta, s_old[n], z[n]); }
-            if (f & CPL_D) { const long n = g.idx(i, j - 1); nsum += fmaf(beta, s_old[n], z[n]); }
-            if (f & CPL_U) { const long n = g.idx(i, j + 1); nsum += fmaf(beta, s_old[n], z[n]); }
+            if (f & CPL_L) { const long n = sk_step_back(e); nsum += fmaf(beta, s_old[n], z[n]); }
+            if (f & CPL_R) { const long n = sk_step_fwd(e); nsum += fmaf(beta, s_old[n], z[n]); }
+            if (f & CPL_D) nsum += fmaf(beta, s_old[nd], z[nd]);
+            if (f & CPL_U) nsum += fmaf(beta, s_old[nu], z[nu]);
             wv = fmaf(-sA, nsum, ad * sc_);
         }
         w[e] = wv;
@@ -330,12 +347,27 @@ __device__ __forceinline__ void pcg_axpy_sk_body(
     const double ws = n_ws < 0 ? sc->ws : block_sum_bcast(ws_partials, n_ws);
     const float a = (float)(sig / ws);
     float m = 0.0f;
-    const XcdLoop it = xcd_loop(g.size(), vb, nvb);
+    // one float4 of each array per thread and iteration (the SK4 arrays hold
+    // a multiple of 256 entries)
+    float4* p4 = (float4*)p;
+    float4* r4 = (float4*)r;
+    const float4* s4 = (const float4*)s;
+    const float4* w4 = (const float4*)w;
+    const XcdLoop it = xcd_loop(g.size() / 4, vb, nvb);
     for (long e = it.e; e < it.end; e += it.stride) {
-        p[e] = fmaf(a, s[e], p[e]);
-        const float rv = fmaf(-a, w[e], r[e]);
-        r[e] = rv;
-        m = fmaxf(m, fabsf(rv));
+        const float4 sv = s4[e], wv = w4[e];
+        float4 pv = p4[e], rv = r4[e];
+        pv.x = fmaf(a, sv.x, pv.x);
+        pv.y = fmaf(a, sv.y, pv.y);
+        pv.z = fmaf(a, sv.z, pv.z);
+        pv.w = fmaf(a, sv.w, pv.w);
+        rv.x = fmaf(-a, wv.x, rv.x);
+        rv.y = fmaf(-a, wv.y, rv.y);
+        rv.z = fmaf(-a, wv.z, rv.z);
+        rv.w = fmaf(-a, wv.w, rv.w);
+        p4[e] = pv;
+        r4[e] = rv;
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(rv.x), fabsf(rv.y)), fmaxf(fabsf(rv.z), fabsf(rv.w))));
     }
     double bm = block_max((double)m);
     if (threadIdx.x == 0) partials[vb] = bm;
@@ -432,91 +464,437 @@ __global__ __launch_bounds__(kBlock) void update_velocity_kernel(
 
 // ---------------------------------------------------------------------------
 // Extrapolation (Array2D<double>::extrapolate, include/Array2D.h:552-591).
-// The reference's FIFO BFS is layer-synchronous: an entry of layer L takes the
-// mean of its neighbours of layer < L, all final before layer L starts.  Pass 1
-// marks layer 1 in parallel and compacts it; pass 2 runs the layers in one
-// workgroup per array, with a barrier between layers.
+// The reference's FIFO BFS from the known entries (mask 0) is layer-
+// synchronous: an entry of layer L takes the mean of its neighbours of layer
+// < L, taken in the order -x, +x, -y, +y (:570-590), all final before layer L
+// starts.  On a full rectangle without obstacles the BFS layer of an entry is
+// its L1 distance to the known set, so the layers come from a separable
+// distance transform instead of a frontier:
+//   rows:    d_row(i) = distance to the nearest known entry of the row (a
+//            wave-wide prefix max / suffix min of known indices);
+//   columns: d(j) = min_j' d_row(j') + |j - j'| (a prefix min of d_row - j,
+//            then a suffix min of d + j over the result, in chunks per wave);
+//   sort:    entries counted per layer, offsets scanned, scattered into layer
+//            order; each entry's sources (neighbours of lower layer) are then
+//            linked: the known ones by value, the others by their position in
+//            the previous layer;
+//   values:  one workgroup per array runs the layers in order, with the
+//            previous layer's values in LDS: a layer costs one LDS round trip
+//            and a barrier, and the global stores never wait.
+// Masks end as the layer numbers (0 known, ~0 where the array has no known
+// entry), as the reference leaves them.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kFar = 1u << 29;  // distance of an entry with no known entry in its array
+constexpr int kExtraColThreads = 1024;
+
+typedef int i4v __attribute__((ext_vector_type(4)));    // int4 and float4 layouts, as builtin
+typedef float f4v __attribute__((ext_vector_type(4)));  // vectors (loadable through gptr)
 struct ExtraArr {
-    float* val;
-    uint32_t* mask;
-    int* front0;
-    int* front1;
-    int* count;  // [0] size of layer-1 frontier
+    gptr<float> val;
+    gptr<uint32_t> mask;
+    gptr<int> order;   // entries in layer order
+    gptr<int> pos;     // position of each entry in that order
+    gptr<i4v> src;     // per ordered entry and neighbour (-x, +x, -y, +y): -1 none, -2 known, else position
+    gptr<f4v> kv;      // per ordered entry: the known neighbours' values
+    gptr<int> aux;     // layer d: count at [32 d], cursor at [32 d + 1] (one 128-byte line per layer,
+                       // so that the atomics of different layers do not share a line); offsets after
+    gptr<int> count;   // number of layers
     int w, h;
 };
+constexpr int kExtraLds = 4096;  // entries of a layer whose values stay in LDS
+__host__ __device__ inline int extra_layers(int w, int h) { return w + h + 2; }
+__host__ __device__ inline int extra_aux_len(int w, int h) { return 33 * extra_layers(w, h); }
+__device__ __forceinline__ gptr<int> extra_off(const ExtraArr& X) { return X.aux + 32 * extra_layers(X.w, X.h); }
 
-__device__ __forceinline__ void extrap_layer1_body(ExtraArr A, ExtraArr B, int vb, int nvb) {
+// Row pass; also clears the per-layer counts.
+__device__ __forceinline__ void extrap_rows_body(ExtraArr A, ExtraArr B, int vb, int nvb) {
+    const int lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
+    const int gt = vb * blockDim.x + threadIdx.x, nt = nvb * blockDim.x;
+    for (int k = gt; k < extra_layers(A.w, A.h); k += nt) A.aux[32 * k] = 0;
+    for (int k = gt; k < extra_layers(B.w, B.h); k += nt) B.aux[32 * k] = 0;
+    const long nrows = (long)A.h + B.h;
+    for (long r = (long)vb * wpb + (threadIdx.x >> 6); r < nrows; r += (long)nvb * wpb) {
+        const ExtraArr& X = r < A.h ? A : B;
+        const int j = (int)(r < A.h ? r : r - A.h);
+        const gptr<uint32_t> m = X.mask + (long)j * X.w;
+        // forward: index of the last known entry at or before i
+        int carry = -(int)kFar;
+        for (int i0 = 0; i0 < X.w; i0 += 64) {
+            const int i = i0 + lane;
+            int c = (i < X.w && m[i] == 0) ? i : -(int)kFar;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(c, o, 64);
+                if (lane >= o) c = c > t ? c : t;
+            }
+            c = c > carry ? c : carry;
+            carry = __shfl(c, 63, 64);
+            if (i < X.w) m[i] = c < 0 ? kFar : (uint32_t)(i - c);  // known entries: 0
+        }
+        // backward: index of the next known entry at or after i
+        carry = 2 * (int)kFar;
+        for (int i0 = (X.w - 1) & ~63; i0 >= 0; i0 -= 64) {
+            const int i = i0 + lane;
+            const uint32_t df = i < X.w ? m[i] : kFar;
+            int c = (i < X.w && df == 0) ? i : 2 * (int)kFar;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_down(c, o, 64);
+                if (lane + o < 64) c = c < t ? c : t;
+            }
+            c = c < carry ? c : carry;
+            carry = __shfl(c, 0, 64);
+            if (i < X.w) {
+                const uint32_t db = c >= (int)kFar ? kFar : (uint32_t)(c - i);
+                m[i] = df < db ? df : db;
+            }
+        }
+    }
+}
+
+// Column pass: a workgroup per 64 columns; each wave takes one contiguous
+// chunk of rows.  Forward F(j) = j + min_{j' <= j} (d_row(j') - j'), then
+// backward d(j) = min_{j' >= j} (F(j') + j') - j, each as chunk totals, an
+// exclusive combination across waves in LDS, and a rescan.
+__device__ __forceinline__ void extrap_cols_body(ExtraArr A, ExtraArr B, int vb, int nvb) {
+    constexpr int NW = kExtraColThreads / 64;
+    __shared__ int totl[NW][64];  // per wave and column: the chunk's total
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nbA = (A.w + 63) / 64, nbB = (B.w + 63) / 64;
+    for (int blk = vb; blk < nbA + nbB; blk += nvb) {
+        const ExtraArr& X = blk < nbA ? A : B;
+        const int i = (blk < nbA ? blk : blk - nbA) * 64 + lane;
+        const bool col = i < X.w;
+        const int chunk = (X.h + NW - 1) / NW;
+        const int j0 = wave * chunk, j1 = j0 + chunk < X.h ? j0 + chunk : X.h;
+        const gptr<uint32_t> m = X.mask + (col ? i : 0);
+        const long W = X.w;
+        // forward totals
+        int t = (int)kFar;
+        if (col)
+            for (int j = j0; j < j1; j++) {
+                const int a = (int)m[j * W] - j;
+                t = a < t ? a : t;
+            }
+        __syncthreads();  // the previous block's readers of totl are done
+        totl[wave][lane] = t;
+        __syncthreads();
+        int run = (int)kFar;
+        for (int v = 0; v < wave; v++) run = totl[v][lane] < run ? totl[v][lane] : run;
+        if (col)
+            for (int j = j0; j < j1; j++) {
+                const int a = (int)m[j * W] - j;
+                run = a < run ? a : run;
+                const int F = j + run;
+                m[j * W] = (uint32_t)(F < (int)kFar ? F : (int)kFar);
+            }
+        __syncthreads();
+        // backward totals over F
+        t = 3 * (int)kFar;
+        if (col)
+            for (int j = j1 - 1; j >= j0; j--) {
+                const int b = (int)m[j * W] + j;
+                t = b < t ? b : t;
+            }
+        totl[wave][lane] = t;
+        __syncthreads();
+        run = 3 * (int)kFar;
+        for (int v = NW - 1; v > wave; v--) run = totl[v][lane] < run ? totl[v][lane] : run;
+        if (col)
+            for (int j = j1 - 1; j >= j0; j--) {
+                const int b = (int)m[j * W] + j;
+                run = b < run ? b : run;
+                const int d = run - j;
+                m[j * W] = (uint32_t)(d < (int)kFar ? d : (int)kFar);
+            }
+        __syncthreads();
+    }
+}
+
+// Entries per layer, counted per 16 x 16 tile of faces: a tile's distances
+// span at most 31 layers, so the workgroup adds its LDS counts with about 30
+// global atomics instead of 256.  tile_pass(..., SCATTER) reserves each
+// layer's range for the tile and writes the entries' places in layer order.
+template <bool SCATTER>
+__device__ __forceinline__ void extrap_tile_pass(ExtraArr A, ExtraArr B, int vb, int nvb) {
+    __shared__ int cnt[32], base[32];
+    __shared__ uint32_t dmin_s;
+    const int tid = threadIdx.x, lx = tid & 15, ly = tid >> 4;
+    const int txA = (A.w + 15) >> 4, tyA = (A.h + 15) >> 4, txB = (B.w + 15) >> 4, tyB = (B.h + 15) >> 4;
+    const int ntA = txA * tyA, nt = ntA + txB * tyB;
+    for (int t = vb; t < nt; t += nvb) {
+        const bool inA = t < ntA;
+        const ExtraArr& X = inA ? A : B;
+        const int tt = inA ? t : t - ntA, tx = inA ? txA : txB;
+        const int i = (tt % tx) * 16 + lx, j = (tt / tx) * 16 + ly;
+        const bool in = i < X.w && j < X.h;
+        const long f = (long)j * X.w + i;
+        const uint32_t d = in ? X.mask[f] : 0u;
+        const bool valid = d >= 1 && d < kFar;
+        if (tid < 32) cnt[tid] = 0;
+        if (tid == 0) dmin_s = kFar;
+        __syncthreads();
+        if (valid) atomicMin(&dmin_s, d);
+        __syncthreads();
+        const uint32_t dm = dmin_s;
+        const int bin = (int)(d - dm);  // < 31 for a valid entry
+        const int rank = valid ? atomicAdd(&cnt[bin], 1) : 0;
+        __syncthreads();
+        if (tid < 32 && cnt[tid] > 0) {
+            const gptr<int> slot = X.aux + 32 * (dm + tid) + (SCATTER ? 1 : 0);
+            const int b = __hip_atomic_fetch_add(slot, cnt[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (SCATTER) base[tid] = b;
+        }
+        if (SCATTER) {
+            __syncthreads();
+            if (valid) {
+                const int k = base[bin] + rank;
+                X.order[k] = (int)f;
+                X.pos[f] = k;
+            } else if (in && d >= kFar) {
+                X.mask[f] = 0xFFFFFFFFu;  // no known entry reaches it
+            }
+        }
+        __syncthreads();  // cnt and base are reused by the next tile
+    }
+}
+__device__ __forceinline__ void extrap_hist_body(ExtraArr A, ExtraArr B, int vb, int nvb) {
+    extrap_tile_pass<false>(A, B, vb, nvb);
+}
+
+// Exclusive offsets of the layers (workgroup 0: A, 1: B), the cursors, and
+// the number of layers.
+__device__ __forceinline__ void extrap_scan_body(ExtraArr A, ExtraArr B, int vb) {
+    ExtraArr X = vb == 0 ? A : B;
+    const int n = extra_layers(X.w, X.h);
+    const gptr<int> off = extra_off(X);
+    __shared__ int part[kBlock];
+    __shared__ int last[kBlock];
+    const int per = (n + kBlock - 1) / kBlock;
+    const int k0 = threadIdx.x * per, k1 = k0 + per < n ? k0 + per : n;
+    int sum = 0, hi = 0;
+    for (int k = k0; k < k1; k++) {
+        const int c = X.aux[32 * k];
+        sum += c;
+        if (c > 0) hi = k;
+    }
+    part[threadIdx.x] = sum;
+    last[threadIdx.x] = hi;
+    __syncthreads();
+    if (threadIdx.x < 64) {  // one wave: 4 partials per lane, then a lane scan
+        const int l = threadIdx.x;
+        int v[4], tsum = 0, top = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            v[q] = part[4 * l + q];
+            tsum += v[q];
+            top = last[4 * l + q] > top ? last[4 * l + q] : top;
+        }
+        int inc = tsum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(inc, o, 64);
+            if (l >= o) inc += t;
+            const int m = __shfl_xor(top, o, 64);
+            top = m > top ? m : top;
+        }
+        int run = inc - tsum;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            part[4 * l + q] = run;
+            run += v[q];
+        }
+        if (l == 0) *X.count = top;
+    }
+    __syncthreads();
+    int run = part[threadIdx.x];
+    for (int k = k0; k < k1; k++) {
+        const int c = X.aux[32 * k];
+        off[k] = run;
+        X.aux[32 * k + 1] = run;
+        run += c;
+    }
+}
+
+// Entries in layer order.  Entries no known entry reaches keep mask ~0.
+__device__ __forceinline__ void extrap_scatter_body(ExtraArr A, ExtraArr B, int vb, int nvb) {
+    extrap_tile_pass<true>(A, B, vb, nvb);
+}
+
+// Sources of each ordered entry: the neighbours of lower layer, in the order
+// -x, +x, -y, +y (:575-588); known ones carry their value (constant while the
+// layers run), the others their position in the order.
+__device__ __forceinline__ void extrap_link_body(ExtraArr A, ExtraArr B, int vb, int nvb) {
     const long nA = (long)A.w * A.h, nB = (long)B.w * B.h;
     for (long e = vb * (long)blockDim.x + threadIdx.x; e < nA + nB; e += (long)nvb * blockDim.x) {
         const ExtraArr& X = e < nA ? A : B;
-        const long f = e < nA ? e : e - nA;
-        const int i = (int)(f % X.w), j = (int)(f / X.w);
-        if (X.mask[f] == 0) continue;
-        // a neighbour flipping from ~0 to 1 concurrently reads as non-zero either way
-        bool k = (i > 0 && X.mask[f - 1] == 0) || (i < X.w - 1 && X.mask[f + 1] == 0) ||
-                 (j > 0 && X.mask[f - X.w] == 0) || (j < X.h - 1 && X.mask[f + X.w] == 0);
-        if (k) {
-            X.mask[f] = 1;
-            int slot = atomicAdd(X.count, 1);
-            X.front0[slot] = (int)f;
+        const long k = e < nA ? e : e - nA;
+        const int n = extra_off(X)[extra_layers(X.w, X.h) - 1];  // the last offset: the number of ordered entries
+        if (k >= n) continue;
+        const int f = X.order[k];
+        const uint32_t d = X.mask[f];
+        const int i = f % X.w, j = f / X.w;
+        const int nb[4] = {i > 0 ? f - 1 : -1, i < X.w - 1 ? f + 1 : -1, j > 0 ? f - X.w : -1,
+                           j < X.h - 1 ? f + X.w : -1};
+        int c[4];
+        float v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            c[q] = -1;
+            v[q] = 0.0f;
+            if (nb[q] < 0) continue;
+            const uint32_t dq = X.mask[nb[q]];
+            if (dq == 0) {
+                c[q] = -2;
+                v[q] = X.val[nb[q]];
+            } else if (dq < d) {
+                c[q] = X.pos[nb[q]];
+            }
         }
+        X.src[k] = i4v{c[0], c[1], c[2], c[3]};
+        X.kv[k] = f4v{v[0], v[1], v[2], v[3]};
     }
-}
-__global__ __launch_bounds__(kBlock) void extrap_layer1_kernel(ExtraArr A, ExtraArr B) {
-    extrap_layer1_body(A, B, blockIdx.x, gridDim.x);
 }
 
-__device__ __forceinline__ void extrap_layers_body(ExtraArr A, ExtraArr B, int vb, int nvb) {
-    ExtraArr X = vb == 0 ? A : B;
-    __shared__ int n_cur, n_next;
-    if (threadIdx.x == 0) {
-        n_cur = *X.count;
-        n_next = 0;
+// Values, layer by layer (workgroup 0: A, 1: B), sums in the reference's
+// neighbour order.  A layer of at most kExtraLds entries keeps its values in
+// LDS for the next layer; a larger one is read back from global memory.  A
+// workgroup barrier orders the layers (on gfx950 it waits for LDS only; the
+// workgroup's global accesses stay ordered on its CU), so the records of each
+// thread's first entry are loaded kExtraPf layers ahead into a register ring,
+// and the layer offsets sit in LDS: a layer then costs an LDS round trip and
+// a barrier.
+constexpr int kExtraOffLds = 8192;
+struct ExtraRec {
+    i4v c;
+    f4v kv;
+    int f;
+};
+__device__ __forceinline__ float extrap_entry(const ExtraArr& X, const ExtraRec& r, bool prev_lds,
+                                              const float* prevb, int pb) {
+    const int cq[4] = {r.c.x, r.c.y, r.c.z, r.c.w};
+    const float vq[4] = {r.kv.x, r.kv.y, r.kv.z, r.kv.w};
+    float sum = 0.0f;
+    int cnt = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (cq[q] == -1) continue;
+        float x;
+        if (cq[q] == -2) x = vq[q];
+        else if (prev_lds) x = prevb[cq[q] - pb];
+        else x = X.val[X.order[cq[q]]];
+        sum += x;
+        cnt++;
     }
-    __syncthreads();
-    int* cur = X.front0;
-    int* nxt = X.front1;
-    for (uint32_t L = 1; n_cur > 0; ++L) {
-        for (int k = threadIdx.x; k < n_cur; k += blockDim.x) {
-            const int f = cur[k];
-            const int i = f % X.w, j = f / X.w;
-            const int nb[4] = {i > 0 ? f - 1 : -1, i < X.w - 1 ? f + 1 : -1, j > 0 ? f - X.w : -1,
-                               j < X.h - 1 ? f + X.w : -1};
-            float sum = 0.0f;
-            int cnt = 0;
+    return cnt == 0 ? 0.0f : sum / (float)cnt;
+}
+// Fast path: every layer of at most 2 x 1024 entries and the offsets in LDS.
+// Each thread owns entries kb + tid and kb + tid + 1024 of a layer; their
+// records are fetched kExtraRing layers ahead from clamped (always valid)
+// positions, so no global load is ever waited for early.
+constexpr int kExtraRing = 2;
+__device__ __forceinline__ void extrap_values_fast(const ExtraArr& X, const int* offl, int nl, float (*buf)[kExtraLds]) {
+    const int tid = threadIdx.x;
+    const int last = offl[nl + 1] - 1;  // the last ordered entry
+    auto fetch = [&](int L, ExtraRec (&r)[2]) {
+        const int kb = offl[L <= nl ? L : nl];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                if (nb[q] < 0) continue;
-                const uint32_t mq = X.mask[nb[q]];
-                if (mq < L) {
-                    sum += X.val[nb[q]];
-                    cnt++;
+        for (int e = 0; e < 2; e++) {
+            const int k = min(kb + tid + 1024 * e, last);
+            r[e].c = X.src[k];
+            r[e].kv = X.kv[k];
+            r[e].f = X.order[k];
+        }
+    };
+    ExtraRec ring[kExtraRing][2];
+#pragma unroll
+    for (int u = 0; u < kExtraRing; u++) fetch(1 + u, ring[u]);
+    for (int L0 = 1; L0 <= nl; L0 += kExtraRing) {
+#pragma unroll
+        for (int u = 0; u < kExtraRing; u++) {
+            const int L = L0 + u;
+            if (L > nl) break;
+            const int pb = offl[L - 1], kb = offl[L], ke = offl[L + 1];
+            const float* prevb = buf[(L - 1) & 1];
+            float* curb = buf[L & 1];
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const int k = kb + tid + 1024 * e;
+                if (k < ke) {
+                    const float val = extrap_entry(X, ring[u][e], L > 1, prevb, pb);
+                    X.val[ring[u][e].f] = val;
+                    curb[k - kb] = val;
                 }
             }
-            X.val[f] = cnt == 0 ? 0.0f : sum / (float)cnt;
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                if (nb[q] < 0) continue;
-                if (atomicCAS(&X.mask[nb[q]], 0xFFFFFFFFu, L + 1) == 0xFFFFFFFFu) {
-                    int slot = atomicAdd(&n_next, 1);
-                    nxt[slot] = nb[q];
-                }
-            }
+            fetch(L + kExtraRing, ring[u]);
+            __syncthreads();
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            n_cur = n_next;
-            n_next = 0;
-        }
-        __syncthreads();
-        int* t = cur;
-        cur = nxt;
-        nxt = t;
     }
 }
-__global__ __launch_bounds__(1024) void extrap_layers_kernel(ExtraArr A, ExtraArr B) {
-    extrap_layers_body(A, B, blockIdx.x, gridDim.x);
+__device__ __forceinline__ void extrap_values_body(ExtraArr A, ExtraArr B, int vb) {
+    ExtraArr X = vb == 0 ? A : B;
+    __shared__ float buf[2][kExtraLds];
+    __shared__ int offl[kExtraOffLds];
+    __shared__ int widest;
+    const int nl = *X.count;
+    if (nl == 0) return;
+    const gptr<int> offg = extra_off(X);
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const bool off_lds = nl + 2 <= kExtraOffLds;
+    if (tid == 0) widest = 0;
+    __syncthreads();
+    if (off_lds) {
+        int wmax = 0;
+        for (int k = tid; k < nl + 2; k += bd) {
+            offl[k] = offg[k];
+            if (k >= 1 && k <= nl) wmax = max(wmax, offg[k + 1] - offg[k]);
+        }
+        atomicMax(&widest, wmax);
+    }
+    __syncthreads();
+    if (off_lds && bd == 1024 && widest <= 2048) {
+        extrap_values_fast(X, offl, nl, buf);
+        return;
+    }
+    // general path: offsets and large layers from global memory
+    for (int L = 1; L <= nl; L++) {
+        const int pb = offg[L - 1], kb = offg[L], ke = offg[L + 1];
+        const bool prev_lds = L > 1 && kb - pb <= kExtraLds;
+        const bool cur_lds = ke - kb <= kExtraLds;
+        const float* prevb = buf[(L - 1) & 1];
+        float* curb = buf[L & 1];
+        for (int k = kb + tid; k < ke; k += bd) {
+            ExtraRec r;
+            r.c = X.src[k];
+            r.kv = X.kv[k];
+            r.f = X.order[k];
+            const float val = extrap_entry(X, r, prev_lds, prevb, pb);
+            X.val[r.f] = val;
+            if (cur_lds) curb[k - kb] = val;
+        }
+        __syncthreads();
+    }
+}
+__global__ __launch_bounds__(kBlock) void extrap_rows_kernel(ExtraArr A, ExtraArr B) {
+    extrap_rows_body(A, B, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kExtraColThreads) void extrap_cols_kernel(ExtraArr A, ExtraArr B) {
+    extrap_cols_body(A, B, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kBlock) void extrap_hist_kernel(ExtraArr A, ExtraArr B) {
+    extrap_hist_body(A, B, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kBlock) void extrap_scan_kernel(ExtraArr A, ExtraArr B) {
+    extrap_scan_body(A, B, blockIdx.x);
+}
+__global__ __launch_bounds__(kBlock) void extrap_scatter_kernel(ExtraArr A, ExtraArr B) {
+    extrap_scatter_body(A, B, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(kBlock) void extrap_link_kernel(ExtraArr A, ExtraArr B) {
+    extrap_link_body(A, B, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(1024) void extrap_values_kernel(ExtraArr A, ExtraArr B) {
+    extrap_values_body(A, B, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -890,8 +1268,8 @@ __global__ __launch_bounds__(kBlock) void ls_cand_kernel(const int* __restrict__
 template <int FX, int FY>
 struct ClosestOp {
     static constexpr int NW = 3;
-    const LsCand* __restrict__ cand;
-    int* __restrict__ t_out;
+    gptr<const LsCand> cand;
+    gptr<int> t_out;
     int nx, ny;
     double dx, dr;
     using Rec = LsCand;
@@ -905,7 +1283,13 @@ struct ClosestOp {
     __device__ Rec load(int a, int b, bool ok) const {
         const int i = clampi(FX ? nx - 1 - a : a, 0, nx - 1);
         const int j = clampi(FY ? ny - 1 - b : b, 0, ny - 1);
-        const LsCand c = cand[(long)j * nx + i];
+        const gptr<const LsCand> p = cand + ((long)j * nx + i);
+        LsCand c;  // field by field: a struct copy cannot read through a global-typed pointer
+        c.d = p->d;
+        c.t = p->t;
+        c.pos = p->pos;
+        c.x = p->x;
+        c.y = p->y;
         return ok ? c : LsCand{HUGE_VAL, -1, 0, 0.0f, 0.0f};
     }
     __device__ void compute(int a, int b, const Rec& r, const uint32_t* left, const uint32_t* below,
@@ -979,8 +1363,8 @@ __global__ __launch_bounds__(kBlock) void ls_surface_kernel(const double* __rest
 template <int FX, int FY>
 struct EikonalOp {
     static constexpr int NW = 2;
-    const double* __restrict__ in;
-    double* __restrict__ outp;
+    gptr<const double> in;
+    gptr<double> outp;
     int nx, ny;
     double dx;
     struct Rec {

@@ -35,6 +35,17 @@ namespace fs2d {
 
 constexpr int kSpinLimit = 1 << 16;  // polls of one memory round trip each: well under 1 s
 
+// Pointers typed as global memory.  Accesses through them compile to GLOBAL
+// instructions, counted by vmcnt alone.  A pointer read from memory (the
+// batched kernels' descriptors) is otherwise generic, and FLAT accesses also
+// count in lgkmcnt, so every LDS or scalar-load wait would wait for them too.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__host__ __device__ inline gptr<T> G(T* p) {
+    return (gptr<T>)p;
+}
+
 // ctl[0] ticket, ctl[1] finished workgroups, ctl[2] epoch (starts at 1).
 struct WaveCtl {
     unsigned long long* handoff;  // [strips][NA][NW]
@@ -47,6 +58,12 @@ __device__ __forceinline__ unsigned long long atomic_ld64(const unsigned long lo
                              __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void atomic_st64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long atomic_ld64(gptr<const unsigned long long> p) {
+    return __hip_atomic_load(const_cast<gptr<unsigned long long>>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_st64(gptr<unsigned long long> p, unsigned long long v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ unsigned readlane_u32(unsigned v, int lane) {
@@ -108,8 +125,8 @@ __device__ __forceinline__ void wavefront_strip(Op& op, int NA, int NB, int s, u
 #pragma unroll
     for (int w = 0; w < NW; w++) left[w] = out[w] = bnd[w];
 
-    const unsigned long long* src = handoff + (size_t)(s > 0 ? s - 1 : 0) * NA * NW;
-    unsigned long long* dst = handoff + (size_t)s * NA * NW;
+    const gptr<const unsigned long long> src = G(handoff) + (size_t)(s > 0 ? s - 1 : 0) * NA * NW;
+    const gptr<unsigned long long> dst = G(handoff) + (size_t)s * NA * NW;
     const bool produce = (s + 1 < nstrips);
     const bool consume = s > 0;
     bool failed = false;
