@@ -487,8 +487,6 @@ __global__ __launch_bounds__(kBlock) void update_velocity_kernel(
 constexpr uint32_t kFar = 1u << 29;  // distance of an entry with no known entry in its array
 constexpr int kExtraColThreads = 1024;
 
-typedef int i4v __attribute__((ext_vector_type(4)));    // int4 and float4 layouts, as builtin
-typedef float f4v __attribute__((ext_vector_type(4)));  // vectors (loadable through gptr)
 struct ExtraArr {
     gptr<float> val;
     gptr<uint32_t> mask;

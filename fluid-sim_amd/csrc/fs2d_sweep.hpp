@@ -85,7 +85,8 @@ enum : uint32_t { CPL_L = 1, CPL_R = 2, CPL_D = 4, CPL_U = 8 };
 
 constexpr int kSub = 4;         // SK4 groups per supergroup (16 steps)
 constexpr int kLoaders = 3;
-constexpr int kBatch = 2;       // supergroups per loader batch
+constexpr int kBatch = 1;       // supergroups per loader batch (2 queues more loads ahead of the
+                                // pollers' and costs 3% at 2048^2)
 constexpr int kRing = 8;        // supergroups in the LDS ring of a single launch (a power of two)
 constexpr int kHandRing = 256;  // steps in the hand-off ring
 constexpr int kPollers = 2;     // poller waves: their polls interleave in time
@@ -341,6 +342,9 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
     };
 
     if (wave == 0) {
+        // the compute wave is the critical path: issue priority over the
+        // loaders and pollers that share its SIMD
+        __builtin_amdgcn_s_setprio(3);
         unsigned long long* dst = c.handoff + (size_t)s * sk_handoff_stride(nx, NW);
         if (consume && produce)
             sweep_compute<Op, BACKWARD, true, true>(op, g, L, s, epoch, dst, c.err);

@@ -45,6 +45,10 @@ template <class T>
 __host__ __device__ inline gptr<T> G(T* p) {
     return (gptr<T>)p;
 }
+// int4 / float4 layouts as builtin vectors, which load and store through gptr
+// (the HIP vector classes' copy operators expect generic pointers)
+typedef int i4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
 
 // ctl[0] ticket, ctl[1] finished workgroups, ctl[2] epoch (starts at 1).
 struct WaveCtl {
