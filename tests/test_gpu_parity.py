@@ -292,6 +292,35 @@ def test_golden_semilag_rows_on_gpu():
     np.testing.assert_allclose(np.array(got)[:, 1:], np.array(ref[:2])[:, 1:], rtol=1e-6)
 
 
+@pytest.mark.parametrize("nx,ny,maxh", [(3000, 24, 5), (4200, 24, 5), (40, 8300, 5), (48, 48, 0)])
+def test_extrapolation_wide_and_deep_layers(nx, ny, maxh):
+    """updateVelocity + extrapolation (Array2D.h:552-591) where the layer-order
+    pass leaves its fast path: layers wider than 2048 faces (3000 x 24), wider
+    than the 4096 kept in LDS (4200 x 24), more than 8190 layers (40 x 8300),
+    and no known face at all (48 x 48 without fluid).  Fluid fills the bottom
+    rows to a random height per column, so the first layers are ragged."""
+    rng = np.random.default_rng(nx * 7 + ny)
+    cells = O.walls(nx, ny)
+    if maxh:
+        h = rng.integers(1, maxh + 1, size=nx)
+        for i in range(1, nx - 1):
+            cells[1:1 + h[i], i] = O.FLUID
+    g, o = pair(cells, k=0, dt=0.005, dx=1.0 / max(nx, ny), mode=fs2d.SEMILAGRANGIAN, static_cells=True,
+                gravity=(0.0, 0.0))
+    g.set(fs2d.F_U, rng.standard_normal((ny, nx + 1)).astype(np.float32))
+    g.set(fs2d.F_V, rng.standard_normal((ny + 1, nx)).astype(np.float32))
+    g.set(fs2d.F_P, rng.standard_normal((ny, nx)).astype(np.float32))
+    push(g, o, [fs2d.F_CELL, fs2d.F_PHI, fs2d.F_U, fs2d.F_V, fs2d.F_P])
+    g.run_stage(fs2d.STAGE_UPDATE_VELOCITY)
+    o.run_stage(O.ST_UPDATEVEL)
+    for fg, fo in ((fs2d.F_NEWU, O.F_NEWU), (fs2d.F_NEWV, O.F_NEWV)):
+        a, b = g.get(fg), o.get(fo)
+        e = rel_l2(a, b)
+        print("%dx%d field %d: rel-L2 %.3e, max abs %.3e" % (nx, ny, fg, e, np.abs(a - b).max()))
+        assert np.isfinite(a).all()
+        assert e < REL
+
+
 def test_all_fluid_box_divergence_free_2048():
     """Size-independent property at the benchmark size: after the projection
     the discrete divergence of every FLUID cell is at the solver tolerance."""
