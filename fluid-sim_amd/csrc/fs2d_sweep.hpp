@@ -28,7 +28,8 @@
 //                   by step; steps whose edge column is outside the grid get
 //                   the boundary value.
 // The producing lane's 16 boundary values per supergroup go through a 2-entry
-// LDS outbox and are stored as tagged words by 16 lanes one supergroup later.
+// LDS outbox and are stored as tagged words by 16 lanes after the first
+// sub-group of the next supergroup, into a padded row (no range tests).
 //
 // Waves signal each other through LDS counters without s_waitcnt: LDS executes
 // one wave's operations in order, so a flag written after data (or after the
@@ -74,10 +75,15 @@ __host__ inline SkGeom make_geom(int nx, int ny) {
 
 struct SkCtl {
     unsigned long long* ticket;
-    unsigned long long* handoff;  // per strip: [nx][NW] tagged words + 64 scratch words
+    unsigned long long* handoff;  // per strip: [kHandPadLo + nx + kHandPadHi][NW] tagged words + 64 scratch words
     int* err;
 };
-__host__ __device__ inline size_t sk_handoff_stride(int nx, int nw) { return (size_t)nx * nw + 64; }
+// A strip's hand-off row holds columns -kHandPadLo .. nx + kHandPadHi - 1:
+// every column a producing lane can name (-79 .. nx + 110, since 4 T4 <=
+// nx + 95), so the stores need no range test; only columns 0 .. nx-1 are read.
+constexpr int kHandPadLo = 80, kHandPadHi = 112;
+__host__ __device__ inline size_t sk_hand_cols(int nx) { return (size_t)nx + kHandPadLo + kHandPadHi; }
+__host__ __device__ inline size_t sk_handoff_stride(int nx, int nw) { return sk_hand_cols(nx) * nw + 64; }
 
 // Coupling bits of a FLUID cell: the off-diagonal entries of row (i, j) are
 // -scaleA towards each coupled neighbour (FluidSim2D.cpp:266-301).
@@ -188,6 +194,12 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
             f.b[j] = NB ? L.ring_b[NB ? slot : 0][j][lane] : 0u;
         }
     };
+    // LDS bases in a VGPR (mbcnt_lo(0, 0) is 0 in every lane but not known to
+    // be uniform), so that the hand ring's and tags' constant offsets fold
+    // into the ds_read offset fields instead of one address move per read
+    const unsigned zero_v = __builtin_amdgcn_mbcnt_lo(0u, 0u);
+    const uint32_t* const hand_v = &L.hand[0][0] + zero_v;
+    unsigned* const tag_v = &L.slot_tag[0] + zero_v;
     // a supergroup's 16 steps sit at (t0 & 255) .. +15 of the hand ring (no wrap)
     auto fetch_h = [&](int sg, Buf& f) {
         const int h0 = sg_t0(sg) & (kHandRing - 1);
@@ -195,7 +207,7 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
         for (int j = 0; j < kSub; j++)
 #pragma unroll
             for (int w = 0; w < NW; w++)
-                f.h[j][w] = C ? *(const uint4*)&L.hand[w][h0 + 4 * j] : make_uint4(0, 0, 0, 0);
+                f.h[j][w] = C ? *(const uint4*)(hand_v + w * kHandRing + h0 + 4 * j) : make_uint4(0, 0, 0, 0);
     };
     bool bad = false;
     auto give_up = [&](int spin) {
@@ -216,10 +228,16 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
     uint32_t st[NW];
 #pragma unroll
     for (int w = 0; w < NW; w++) st[w] = 0u;
-    // hand-off stores: lanes 16 w + q carry word w of step q; lanes that have
-    // nothing to store (past 16 NW, or a column outside the grid) write to a
-    // per-lane scratch slot after the strip's row, so the store needs no branch
-    unsigned long long* const dump = dst + (size_t)nx * NW + lane;
+    // Hand-off stores: at supergroup sg, lane 16 w + q stores word w of step q
+    // of supergroup sg-1 (at sg = 0 a column left of the grid, never read).
+    // Each such lane walks 16 columns per supergroup through the padded row;
+    // lanes past 16 NW keep one scratch word each.  No per-supergroup address
+    // arithmetic, no range test.
+    const bool hreal = lane < 16 * NW;
+    const int hcol0 = sg_t0(-1) + (lane & 15) - prod_lane;
+    unsigned long long* hp = hreal ? dst + (size_t)(hcol0 + kHandPadLo) * NW + (lane >> 4)
+                                   : dst + sk_hand_cols(nx) * NW + lane;
+    const long hinc = hreal ? (B ? -16L : 16L) * NW : 0L;
 
     // One supergroup: compute `cur` while the LDS reads of the next one go
     // into `nxt` speculatively: the readiness flags are read just before the
@@ -227,15 +245,26 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
     // the compute; only a failed check branches (wait, then read again).  The
     // prefetch index is clamped, so the last supergroup re-reads its own slot
     // and every iteration has the same shape.
+    // LDS order per supergroup: an explicit wait at the top for all but the
+    // previous supergroup's outbox and consumed_sg writes (the data of `cur`
+    // arrived a supergroup ago, so it costs nothing), the outbox read, the
+    // first sub-group, then all reads of the next supergroup.  The reads then
+    // never exceed lgkmcnt's range (15) ahead of a use, so no wait for a
+    // young read ever lands on the FMA chain.
+    // s_waitcnt immediates (gfx9 encoding: lgkmcnt in bits 8-11, vmcnt and
+    // expcnt at their maxima): lgkmcnt(young writes) and lgkmcnt(0)
+    constexpr unsigned kYoung = P ? NW * kSub + 1 : 1;
+    static_assert(kYoung <= 15, "outbox writes exceed lgkmcnt's range");
+    constexpr unsigned kWaitOld = 0xC07Fu | (kYoung << 8);
+    constexpr unsigned kWaitAll = 0xC07Fu;
     auto step = [&](int sg, Buf& cur, Buf& nxt) {
         const int t0 = sg_t0(sg);
         const int sn = min(sg + 1, NSG - 1);
-        const unsigned tagv = lds_ld(&L.slot_tag[sn & (RING - 1)]);
-        const unsigned hrv = C ? lds_ld(&L.hand_ready) : 0u;
-        fetch_h(sn, nxt);
-        fetch_x(sn, nxt);
+        __builtin_amdgcn_s_waitcnt(kWaitOld);
         uint32_t ob = 0;
         if (P) ob = L.outbox[(sg - 1) & 1][min(lane >> 4, NW - 1)][lane & 15];
+        __builtin_amdgcn_sched_barrier(0);
+        unsigned tagv = 0, hrv = 0;
 
         uint32_t prod[kSub][NW][4];
 #pragma unroll
@@ -248,35 +277,45 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
                 float* base = op.out[n] + strip_base + (long)t0 * 64;  // uniform
                 *(float4*)(base + j * 256 + lane * 4) = make_float4(o[n][0], o[n][1], o[n][2], o[n][3]);
             }
-            // the previous supergroup's boundary values leave once the outbox
-            // read has had a sub-group's time to return
-            if (P && k == 0) {
-                const int col = sg_t0(sg - 1) + (lane & 15) - prod_lane;
-                const bool real = sg > 0 && lane < 16 * NW && col >= 0 && col < nx;
-                atomic_st64(real ? dst + (size_t)col * NW + (lane >> 4) : dump,
-                            (unsigned long long)ob | ((unsigned long long)epoch << 32));
+            if (k == 0) {
+                // the previous supergroup's boundary values leave once the
+                // outbox read has had a sub-group's time to return
+                if (P) {
+                    atomic_st64(hp, (unsigned long long)ob | ((unsigned long long)epoch << 32));
+                    hp += hinc;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                tagv = lds_ld(tag_v + (sn & (RING - 1)));
+                if (C) hrv = lds_ld(&L.hand_ready);
+                fetch_h(sn, nxt);
+                fetch_x(sn, nxt);
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
-        if (P) {
-            if (lane == prod_lane)
+        if (lane == prod_lane) {
+            if (P)
 #pragma unroll
                 for (int w = 0; w < NW; w++)
 #pragma unroll
                     for (int j = 0; j < kSub; j++)
                         *(uint4*)&L.outbox[sg & 1][w][4 * j] =
                             make_uint4(prod[j][w][0], prod[j][w][1], prod[j][w][2], prod[j][w][3]);
+            lds_st(&L.consumed_sg, (unsigned)sg + 1u);
         }
-        if (lane == 0) lds_st(&L.consumed_sg, (unsigned)sg + 1u);
-        const bool okx = x_ready(sn, uniform(tagv)), okh = !C || h_ready(sn, uniform(hrv));
-        if (!(okx && okh)) {
-            if (!okx) {
+        const unsigned sn1 = (unsigned)sn + 1u;
+        const unsigned tv = uniform(tagv), hv = C ? uniform(hrv) : 0u;
+        // nonzero unless the slot holds supergroup sn and its hand values are in
+        // (hand_ready < 2^31, so the difference's sign bit is hv < 16 sn1)
+        if ((tv ^ sn1) | (C ? (hv - 16u * sn1) >> 31 : 0u)) {
+            if (tv != sn1) {
                 wait_x(sn);
                 fetch_x(sn, nxt);
             }
-            if (!okh) {
+            if (C && hv < 16u * sn1) {
                 wait_h(sn);
                 fetch_h(sn, nxt);
             }
+            __builtin_amdgcn_s_waitcnt(kWaitAll);  // re-read data in registers, as on the fast path
         }
     };
 
@@ -285,16 +324,14 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
     if (C) wait_h(0);
     fetch_h(0, b0);
     fetch_x(0, b0);
+    __builtin_amdgcn_s_waitcnt(kWaitAll);  // the first supergroup's data in registers
     for (int sg = 0; sg < NSG; sg += 2) {  // NSG is even
         step(sg, b0, b1);
         step(sg + 1, b1, b0);
     }
-    if (P && lane < 16 * NW) {
+    if (P && hreal) {
         const uint32_t ob = L.outbox[(NSG - 1) & 1][min(lane >> 4, NW - 1)][lane & 15];
-        const int col = sg_t0(NSG - 1) + (lane & 15) - prod_lane;
-        if (col >= 0 && col < nx)
-            atomic_st64(dst + (size_t)col * NW + (lane >> 4),
-                        (unsigned long long)ob | ((unsigned long long)epoch << 32));
+        atomic_st64(hp, (unsigned long long)ob | ((unsigned long long)epoch << 32));
     }
     op.strip_done(s);
 }
@@ -442,7 +479,8 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
                     bool ok = true;
 #pragma unroll
                     for (int w = 0; w < NW; w++) {
-                        if ((unsigned)(hv[w] >> 32) != epoch) hv[w] = atomic_ld64(src + (size_t)col * NW + w);
+                        if ((unsigned)(hv[w] >> 32) != epoch)
+                            hv[w] = atomic_ld64(src + (size_t)(col + kHandPadLo) * NW + w);
                         ok = ok && (unsigned)(hv[w] >> 32) == epoch;
                     }
                     if (ok) {
