@@ -292,6 +292,37 @@ def test_golden_semilag_rows_on_gpu():
     np.testing.assert_allclose(np.array(got)[:, 1:], np.array(ref[:2])[:, 1:], rtol=1e-6)
 
 
+@pytest.mark.parametrize("nx,ny", [(5, 130), (40, 200), (1000, 65), (77, 64), (130, 5)])
+def test_projection_odd_shapes(nx, ny):
+    """Matched-iteration PCG on shapes whose strips and columns do not fill the
+    solver's tiles: a partial last strip (ny mod 64 != 0), a single strip,
+    widths far from a multiple of 32 and a 5-wide grid (the hand-off rows then
+    carry mostly padding columns).  The narrow grid's fluid is kept short (16
+    rows of 3 cells): a long thin column is a near-1D Laplacian whose condition
+    number grows as its length squared (85 rows of one cell put the FP32 error
+    at ~2e-4), and on which MIC(0) is nearly exact, so after a few iterations
+    the FP32 residual is exhausted; 3 iterations run every sweep."""
+    rng = np.random.default_rng(nx * 1000 + ny)
+    cells = O.walls(nx, ny)
+    top = 17 if nx < 8 else max(2, (2 * ny) // 3)
+    cells[1:top, 1:nx - 1] = O.FLUID  # fluid below, air above (ghost terms)
+    g, o = pair(cells, k=0, dt=0.005, dx=1.0 / max(nx, ny), mode=fs2d.SEMILAGRANGIAN, static_cells=True,
+                gravity=(0.0, 0.0))
+    g.set(fs2d.F_U, rng.standard_normal((ny, nx + 1)).astype(np.float32))
+    g.set(fs2d.F_V, rng.standard_normal((ny + 1, nx)).astype(np.float32))
+    push(g, o, [fs2d.F_CELL, fs2d.F_PHI, fs2d.F_U, fs2d.F_V])
+    for iters in ((1, 3) if nx < 8 else (1, 7)):
+        g.set_solver(tau=0.97, tol=0.0, max_iters=iters)
+        o.set_options(tau=0.97, tol=0.0, max_iters=iters, sl_inplace=False)
+        g.run_stage(fs2d.STAGE_PROJECT, allow=(fs2d.OK, fs2d.WARN_MAXITER))
+        o.run_stage(O.ST_PROJECT)
+        assert g.stats()["pcg_iters"] == o.stats()["last_iters"] == iters
+        e = rel_l2(g.get(fs2d.F_P), o.get(O.F_P))
+        print("%dx%d iters %d p rel-L2 %.3e" % (nx, ny, iters, e))
+        assert np.isfinite(g.get(fs2d.F_P)).all()
+        assert e < REL
+
+
 @pytest.mark.parametrize("nx,ny,maxh", [(3000, 24, 5), (4200, 24, 5), (40, 8300, 5), (48, 48, 0)])
 def test_extrapolation_wide_and_deep_layers(nx, ny, maxh):
     """updateVelocity + extrapolation (Array2D.h:552-591) where the layer-order
