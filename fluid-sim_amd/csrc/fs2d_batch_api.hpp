@@ -44,7 +44,7 @@ struct fs2d_batch {
     std::vector<void*> allocs;
     std::string err;
 
-    bool timing = false;
+    unsigned long long timing = 0;  // kernel classes timed (bit k: class k)
     std::vector<TimedLaunch> pending;
     std::vector<hipEvent_t> event_pool;
     double ktime[K_COUNT] = {};
@@ -663,7 +663,7 @@ fs2d_status fs2d_batch_set_timing(fs2d_batch* B, int enable) {
     if (!B) return FS2D_ERR_ARG;
     BHIPCHK(B, hipStreamSynchronize(B->stream));
     resolve_timing(B);
-    B->timing = enable != 0;
+    B->timing = enable ? ((1ull << K_COUNT) - 1ull) : 0ull;
     return FS2D_OK;
 }
 

@@ -105,7 +105,7 @@ struct fs2d_sim {
     int last_iters_hint = 8;
     std::string err;
 
-    bool timing = false;
+    unsigned long long timing = 0;  // bit k: launches of kernel class k are timed
     std::vector<TimedLaunch> pending;
     std::vector<hipEvent_t> event_pool;
     double ktime[K_COUNT] = {};
@@ -135,7 +135,8 @@ fs2d_status fail(fs2d_sim* sim, fs2d_status st, const char* fmt, ...) {
     } while (0)
 
 // Per-kernel timing state lives in the handle (fs2d_sim) or the batch
-// (fs2d_batch): members timing, stream, pending, event_pool, ktime, kcount.
+// (fs2d_batch): members timing (class mask), stream, pending, event_pool,
+// ktime, kcount.
 template <class H>
 hipEvent_t get_event(H* s) {
     if (!s->event_pool.empty()) {
@@ -148,14 +149,14 @@ hipEvent_t get_event(H* s) {
     return e;
 }
 
-// Times one launch on the owner's stream when timing is enabled.
+// Times one launch on the owner's stream when its class is in the timing mask.
 template <class H>
 struct Timed {
     H* s;
     int cls;
     hipEvent_t a = nullptr;
     Timed(H* s_, int c) : s(s_), cls(c) {
-        if (s->timing) {
+        if ((s->timing >> c) & 1ull) {
             a = get_event(s);
             if (a) (void)hipEventRecord(a, s->stream);
         }
@@ -1158,10 +1159,13 @@ fs2d_status fs2d_max_velocity(fs2d_sim* s, double* out) {
 }
 
 fs2d_status fs2d_set_timing(fs2d_sim* s, int enable) {
+    return fs2d_set_timing_classes(s, enable ? ~0ull : 0ull);
+}
+fs2d_status fs2d_set_timing_classes(fs2d_sim* s, unsigned long long class_mask) {
     if (!s) return FS2D_ERR_ARG;
     HIPCHK(s, hipStreamSynchronize(s->stream));
     resolve_timing(s);
-    s->timing = enable != 0;
+    s->timing = class_mask & ((1ull << K_COUNT) - 1ull);
     return FS2D_OK;
 }
 int fs2d_kernel_count(void) { return K_COUNT; }

@@ -93,6 +93,7 @@ def load():
         "fs2d_avg_pressure": (i, [vp, i, ctypes.POINTER(d)]),
         "fs2d_max_velocity": (i, [vp, ctypes.POINTER(d)]),
         "fs2d_set_timing": (i, [vp, i]),
+        "fs2d_set_timing_classes": (i, [vp, ctypes.c_ulonglong]),
         "fs2d_kernel_count": (i, []),
         "fs2d_kernel_name": (ctypes.c_char_p, [i]),
         "fs2d_get_kernel_time": (i, [vp, i, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_longlong)]),
@@ -301,8 +302,17 @@ class Sim:
         self._check(self.L.fs2d_max_velocity(self.h, ctypes.byref(d)))
         return d.value
 
-    def set_timing(self, on):
-        self._check(self.L.fs2d_set_timing(self.h, int(on)))
+    def set_timing(self, on, only=None):
+        """Per-kernel HIP-event timing; `only` names the kernel classes to time
+        (each timed launch adds two events to the stream)."""
+        if only is None or not on:
+            self._check(self.L.fs2d_set_timing(self.h, int(on)))
+            return
+        names = [self.L.fs2d_kernel_name(k).decode() for k in range(self.L.fs2d_kernel_count())]
+        mask = 0
+        for n in only:
+            mask |= 1 << names.index(n)
+        self._check(self.L.fs2d_set_timing_classes(self.h, mask))
 
     def kernel_times(self):
         out = {}

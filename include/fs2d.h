@@ -193,6 +193,11 @@ fs2d_status fs2d_max_velocity(fs2d_sim* sim, double* out);
  * counterpart; PerformanceCounter, src/PerformanceCounter.cpp:9-74, timed
  * stages with a host clock).  Kernel classes are listed by fs2d_kernel_name. */
 fs2d_status fs2d_set_timing(fs2d_sim* sim, int enable);
+/* Times only the kernel classes whose bit is set in class_mask (bit k: class k
+ * of fs2d_kernel_name; 0 turns timing off).  Each timed launch records two
+ * events on the stream, which costs the step time: timing one class measures
+ * that class while leaving the others untouched.  No reference counterpart. */
+fs2d_status fs2d_set_timing_classes(fs2d_sim* sim, unsigned long long class_mask);
 int fs2d_kernel_count(void);
 const char* fs2d_kernel_name(int kernel);
 fs2d_status fs2d_get_kernel_time(fs2d_sim* sim, int kernel, double* total_ms, long long* launches);
