@@ -377,6 +377,38 @@ def test_all_fluid_box_divergence_free_2048():
     assert np.abs(div[fl]).max() < 1e-4 * np.abs(div0[fl]).max()
 
 
+def test_timing_classes():
+    """fs2d_set_timing_classes times exactly the classes in its mask; timing
+    leaves the results unchanged (bit for bit)."""
+    n = 64
+    cells = O.dam_break(n, n)
+    runs = {}
+    for mode in ("off", "fwd", "all"):
+        g = fs2d.Sim(cells, k=2, dt=0.005, dx=1.0 / n, mode=fs2d.PICFLIP)
+        g.step()
+        g.reset_kernel_times()
+        if mode == "fwd":
+            g.set_timing(True, only=["pcg_forward_solve", "p2g"])
+        elif mode == "all":
+            g.set_timing(True)
+        for _ in range(2):
+            g.step()
+        kt = g.kernel_times()
+        timed = {k for k, (ms, launches) in kt.items() if launches > 0}
+        if mode == "off":
+            assert timed == set()
+        elif mode == "fwd":
+            assert timed == {"pcg_forward_solve", "p2g"}
+            assert kt["pcg_forward_solve"][0] > 0
+        else:
+            assert {"pcg_forward_solve", "pcg_backward_solve", "pcg_spmv_dot", "p2g", "levelset_sweep"} <= timed
+        runs[mode] = (g.get(fs2d.F_U), g.get(fs2d.F_P), g.get(fs2d.F_PARTICLE_POS))
+        g.close()
+    for mode in ("fwd", "all"):
+        for a, b in zip(runs["off"], runs[mode]):
+            np.testing.assert_array_equal(a, b)
+
+
 def test_error_paths():
     n = 32
     g = fs2d.Sim(O.dam_break(n, n), k=2, dt=0.005, dx=1.0 / n, mode=fs2d.SEMILAGRANGIAN)
