@@ -440,6 +440,28 @@ struct FactorMaker {
                         nparts[b], I.max_iters, I.sc};
     }
 };
+// The eikonal passes on the sweep engine (EikSweepOp): conversions between a
+// member's phi (row-major, ping-pong by parity) and the SK4 order of a
+// direction's flipped coordinates, in the member's solver scratch.
+__global__ __launch_bounds__(kBlock) void b_eik_to_sk_kernel(const InstBufs* __restrict__ ins, BlockMap bm,
+                                                             int parity, int fx, int fy) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    eik_to_sk_body(I.geom, parity ? I.phi2 : I.phi, I.z, I.q, I.cpl, fx, fy, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_eik_from_sk_kernel(const InstBufs* __restrict__ ins, BlockMap bm,
+                                                               int parity, int fx, int fy) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    eik_from_sk_body(I.geom, I.w, I.s1, parity ? I.phi : I.phi2, fx, fy, vb, nvb);
+}
+struct EikSweepMaker {
+    __device__ EikSweepOp operator()(const InstBufs& I, int) const {
+        return EikSweepOp{{I.z, I.q}, I.cpl, {I.w, I.s1}, I.dx};
+    }
+};
 struct FwdMaker {
     const int* nparts;
     int check;

@@ -309,17 +309,22 @@ void bstage_level_set(fs2d_batch* B) {
             blaunch(B, b_ls_phi_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
             blaunch(B, b_ls_surface_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
         }
+        // eikonal passes on the sweep engine (the single path's order: pass k
+        // reads phi of parity k & 1, direction fx = k & 1, fy = (k >> 1) & 1)
         for (int k = 0; k < 16; k++) {
-            Timed t(B, K_LS_SWEEP);
-            switch (k & 3) {
-                case 0: blaunch(B, b_wavefront_kernel<EikonalOp<0, 0>, EikonalMaker<0, 0>>, S, 64, B->d_ins,
-                                B->strips, c, EikonalMaker<0, 0>{k & 1}); break;
-                case 1: blaunch(B, b_wavefront_kernel<EikonalOp<1, 0>, EikonalMaker<1, 0>>, S, 64, B->d_ins,
-                                B->strips, c, EikonalMaker<1, 0>{k & 1}); break;
-                case 2: blaunch(B, b_wavefront_kernel<EikonalOp<0, 1>, EikonalMaker<0, 1>>, S, 64, B->d_ins,
-                                B->strips, c, EikonalMaker<0, 1>{k & 1}); break;
-                default: blaunch(B, b_wavefront_kernel<EikonalOp<1, 1>, EikonalMaker<1, 1>>, S, 64, B->d_ins,
-                                 B->strips, c, EikonalMaker<1, 1>{k & 1}); break;
+            const int fx = k & 1, fy = (k >> 1) & 1;
+            {
+                Timed t(B, K_LS_OTHER);
+                blaunch(B, b_eik_to_sk_kernel, B->g_sk, kBlock, B->d_ins, B->m_sk, k & 1, fx, fy);
+            }
+            {
+                Timed t(B, K_LS_SWEEP);
+                blaunch(B, b_sweep_kernel<EikSweepOp, false, EikSweepMaker>, S, kSweepThreads, B->d_ins, B->strips,
+                        B->sk_ticket, B->werr, EikSweepMaker{});
+            }
+            {
+                Timed t(B, K_LS_OTHER);
+                blaunch(B, b_eik_from_sk_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells, k & 1, fx, fy);
             }
         }
         {

@@ -319,6 +319,10 @@ void diagnostics(fs2d_sim* s) {
     hipLaunchKernelGGL(diag_finalize_kernel, dim3(1), dim3(kBlock), 0, s->stream, s->partials, nb, s->dx, s->dg);
 }
 
+template <class Op, bool B>
+void launch_sweep(fs2d_sim* s, const Op& op);
+int sk_blocks(const fs2d_sim* s);
+
 // FluidSim2D::createWaterLevelSet (src/FluidSim2D.cpp:653-732).
 void stage_level_set(fs2d_sim* s) {
     if (s->flags & FS2D_FLAG_PARTICLE_CELLS) {
@@ -367,14 +371,24 @@ void stage_level_set(fs2d_sim* s) {
             hipLaunchKernelGGL(ls_surface_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, b, a, s->nx,
                                s->ny);
         }
+        // eikonal passes on the sweep engine, in SK4 order of each direction's
+        // flipped coordinates (the solver's SK4 scratch: z, q, cpl in; w, s1 out)
         for (int rep = 0; rep < 4; rep++)
             for (int d = 0; d < 4; d++) {
-                Timed t(s, K_LS_SWEEP);
-                switch (d) {
-                    case 0: launch_wavefront(s, EikonalOp<0, 0>{G(a), G(b), s->nx, s->ny, s->dx}); break;
-                    case 1: launch_wavefront(s, EikonalOp<1, 0>{G(a), G(b), s->nx, s->ny, s->dx}); break;
-                    case 2: launch_wavefront(s, EikonalOp<0, 1>{G(a), G(b), s->nx, s->ny, s->dx}); break;
-                    default: launch_wavefront(s, EikonalOp<1, 1>{G(a), G(b), s->nx, s->ny, s->dx}); break;
+                const int fx = d & 1, fy = d >> 1;
+                {
+                    Timed t(s, K_LS_OTHER);
+                    hipLaunchKernelGGL(eik_to_sk_kernel, dim3(sk_blocks(s)), dim3(kBlock), 0, s->stream, s->geom, a,
+                                       s->z, s->q, s->cpl, fx, fy);
+                }
+                {
+                    Timed t(s, K_LS_SWEEP);
+                    launch_sweep<EikSweepOp, false>(s, EikSweepOp{{s->z, s->q}, s->cpl, {s->w, s->s[1]}, s->dx});
+                }
+                {
+                    Timed t(s, K_LS_OTHER);
+                    hipLaunchKernelGGL(eik_from_sk_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->geom,
+                                       s->w, s->s[1], b, fx, fy);
                 }
                 std::swap(a, b);
             }

@@ -1356,6 +1356,22 @@ __global__ __launch_bounds__(kBlock) void ls_surface_kernel(const double* __rest
     ls_surface_body(phi, out, nx, ny, blockIdx.x, gridDim.x);
 }
 
+// The eikonal update of one cell (:845-902) from its two already-updated
+// neighbours: only a cell with phi < 0 changes.  FP64, no contraction, as the
+// reference compiles it.
+__device__ __forceinline__ double eik_update(double cur, double left, double below, double dx) {
+#pragma clang fp contract(off)
+    if (!(cur >= 0)) {
+        const double A = fabs(left), B = fabs(below);
+        const double phi0 = A < B ? A : B;
+        const double phi1 = A > B ? A : B;
+        double d = phi0 + dx;
+        if (d > phi1) d = 0.5 * (phi0 + phi1 + sqrt(2 * dx * dx - (phi1 - phi0) * (phi1 - phi0)));
+        if (d < -cur) cur = -d;
+    }
+    return cur;
+}
+
 // One directional sweep of the eikonal update (:845-902) on cells with phi < 0,
 // for logical a >= 1 and b >= 1.  Carried state: phi (FP64).
 template <int FX, int FY>
@@ -1383,19 +1399,98 @@ struct EikonalOp {
         const int j = FY ? ny - 1 - b : b;
         const long e = (long)j * nx + i;
         double cur = r.v;
-        if (a >= 1 && b >= 1 && !(cur >= 0)) {
-            const double A = fabs(u2d(left)), B = fabs(u2d(below));
-            const double phi0 = A < B ? A : B;
-            const double phi1 = A > B ? A : B;
-            double d = phi0 + dx;
-            if (d > phi1) d = 0.5 * (phi0 + phi1 + sqrt(2 * dx * dx - (phi1 - phi0) * (phi1 - phi0)));
-            if (d < -cur) cur = -d;
-        }
+        if (a >= 1 && b >= 1) cur = eik_update(cur, u2d(left), u2d(below), dx);
         outp[e] = cur;
         d2u(cur, out);
     }
     __device__ void strip_done(int) {}
     __device__ void last_done(int) {}
+};
+
+// ---------------------------------------------------------------------------
+// The eikonal passes on the solver's sweep engine (fs2d_sweep.hpp), whose
+// loader and poller waves take the loads and hand-offs off the compute wave.
+// A pass in direction (FX, FY) is a forward sweep in the flipped coordinates
+// (a, b) = (FX ? nx-1-i : i, FY ? ny-1-j : j): its distances are laid out in
+// SK4 order of (a, b) as low and high 32-bit words (bit moves only, no float
+// arithmetic), with a flag byte per entry (a >= 1 and b >= 1: the cells the
+// update may change; 0 on padding).  The SK4 arrays are the solver's scratch
+// (z, q in; w, s1 out; cpl flags), free between projections.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void eik_to_sk_body(SkGeom g, const double* __restrict__ phi, float* __restrict__ lo,
+                                               float* __restrict__ hi, uint8_t* __restrict__ flag, int fx, int fy,
+                                               int vb, int nvb) {
+    const long n = g.size();
+    for (long e = vb * (long)blockDim.x + threadIdx.x; e < n; e += (long)nvb * blockDim.x) {
+        int a, b;
+        const bool in = sk_cell(g, e, a, b);  // flipped coordinates
+        double v = 0.0;
+        if (in) v = phi[(long)(fy ? g.ny - 1 - b : b) * g.nx + (fx ? g.nx - 1 - a : a)];
+        uint32_t w[2];
+        d2u(v, w);
+        lo[e] = __uint_as_float(w[0]);
+        hi[e] = __uint_as_float(w[1]);
+        flag[e] = (uint8_t)(in && a >= 1 && b >= 1);
+    }
+}
+__global__ __launch_bounds__(kBlock) void eik_to_sk_kernel(SkGeom g, const double* __restrict__ phi,
+                                                           float* __restrict__ lo, float* __restrict__ hi,
+                                                           uint8_t* __restrict__ flag, int fx, int fy) {
+    eik_to_sk_body(g, phi, lo, hi, flag, fx, fy, blockIdx.x, gridDim.x);
+}
+__device__ __forceinline__ void eik_from_sk_body(SkGeom g, const float* __restrict__ lo, const float* __restrict__ hi,
+                                                 double* __restrict__ phi, int fx, int fy, int vb, int nvb) {
+    const long n = (long)g.nx * g.ny;
+    for (long c = vb * (long)blockDim.x + threadIdx.x; c < n; c += (long)nvb * blockDim.x) {
+        const int i = (int)(c % g.nx), j = (int)(c / g.nx);
+        const long e = g.idx(fx ? g.nx - 1 - i : i, fy ? g.ny - 1 - j : j);
+        const uint32_t w[2] = {__float_as_uint(lo[e]), __float_as_uint(hi[e])};
+        phi[c] = u2d(w);
+    }
+}
+__global__ __launch_bounds__(kBlock) void eik_from_sk_kernel(SkGeom g, const float* __restrict__ lo,
+                                                             const float* __restrict__ hi, double* __restrict__ phi,
+                                                             int fx, int fy) {
+    eik_from_sk_body(g, lo, hi, phi, fx, fy, blockIdx.x, gridDim.x);
+}
+
+// The sweep operator: carried state phi (FP64) in two words; the row
+// neighbour is the lane's own previous result, the column neighbour lane
+// l-1's, moved by DPP (the edge lane's from the strip below).
+struct EikSweepOp {
+    static constexpr int NW = 2, NIN = 2, NF = 2, NB = 1, NOUT = 2;
+    const float* in[NIN];  // low, high words of phi
+    const uint8_t* bytes;  // 1 where the update may change the cell
+    float* out[NOUT];      // low, high words of the result
+    double dx;
+    __device__ bool skip(bool) const { return false; }
+    __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double&) const {
+        x[0] = in[0];
+        x[1] = in[1];
+    }
+    __device__ void loader_done(int, int, double) const {}
+    template <bool B, bool C>
+    __device__ void group(const float4 (&x)[NF], uint32_t b4, uint32_t (&st)[NW], const uint4 (&hb)[NW],
+                          float (&o)[NOUT][4], uint32_t (&prod)[NW][4]) const {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uint32_t nb[NW];
+#pragma unroll
+            for (int w = 0; w < NW; w++) nb[w] = nbr<B, C>(u4get(hb[w], q), st[w]);
+            const uint32_t cw[2] = {__float_as_uint(f4get(x[0], q)), __float_as_uint(f4get(x[1], q))};
+            double cur = u2d(cw);
+            if ((b4 >> (8 * q)) & 0xFFu) cur = eik_update(cur, u2d(st), u2d(nb), dx);
+            uint32_t r[2];
+            d2u(cur, r);
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                st[w] = r[w];
+                o[w][q] = __uint_as_float(r[w]);
+                prod[w][q] = r[w];
+            }
+        }
+    }
+    __device__ void strip_done(int) const {}
 };
 
 // Smoothing pass phi = min(phi, mean of 4 neighbours) on 1..n-2 (:904-919).
