@@ -457,6 +457,13 @@ __global__ __launch_bounds__(kBlock) void b_eik_from_sk_kernel(const InstBufs* _
     const InstBufs& I = ins[b];
     eik_from_sk_body(I.geom, I.w, I.s1, parity ? I.phi : I.phi2, fx, fy, vb, nvb);
 }
+__global__ __launch_bounds__(kBlock) void b_eik_sk_to_sk_kernel(const InstBufs* __restrict__ ins, BlockMap bm,
+                                                                int fx, int fy, int fx2, int fy2) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    eik_sk_to_sk_body(I.geom, I.w, I.s1, I.z, I.q, I.cpl, fx, fy, fx2, fy2, vb, nvb);
+}
 struct EikSweepMaker {
     __device__ EikSweepOp operator()(const InstBufs& I, int) const {
         return EikSweepOp{{I.z, I.q}, I.cpl, {I.w, I.s1}, I.dx};

@@ -309,23 +309,26 @@ void bstage_level_set(fs2d_batch* B) {
             blaunch(B, b_ls_phi_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
             blaunch(B, b_ls_surface_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
         }
-        // eikonal passes on the sweep engine (the single path's order: pass k
-        // reads phi of parity k & 1, direction fx = k & 1, fy = (k >> 1) & 1)
+        // eikonal passes on the sweep engine, as in the single path: phi (parity
+        // 0) into SK4 order of direction 0, 16 passes (direction d = k mod 4:
+        // fx = d & 1, fy = d >> 1) with each output moved to the next pass's
+        // order, and the last one back to phi
+        {
+            Timed t(B, K_LS_OTHER);
+            blaunch(B, b_eik_to_sk_kernel, B->g_sk, kBlock, B->d_ins, B->m_sk, 0, 0, 0);
+        }
         for (int k = 0; k < 16; k++) {
-            const int fx = k & 1, fy = (k >> 1) & 1;
-            {
-                Timed t(B, K_LS_OTHER);
-                blaunch(B, b_eik_to_sk_kernel, B->g_sk, kBlock, B->d_ins, B->m_sk, k & 1, fx, fy);
-            }
+            const int d = k & 3, d2 = (k + 1) & 3;
             {
                 Timed t(B, K_LS_SWEEP);
                 blaunch(B, b_sweep_kernel<EikSweepOp, false, EikSweepMaker>, S, kSweepThreads, B->d_ins, B->strips,
                         B->sk_ticket, B->werr, EikSweepMaker{});
             }
-            {
-                Timed t(B, K_LS_OTHER);
-                blaunch(B, b_eik_from_sk_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells, k & 1, fx, fy);
-            }
+            Timed t(B, K_LS_OTHER);
+            if (k < 15)
+                blaunch(B, b_eik_sk_to_sk_kernel, B->g_sk, kBlock, B->d_ins, B->m_sk, d & 1, d >> 1, d2 & 1, d2 >> 1);
+            else
+                blaunch(B, b_eik_from_sk_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells, 1, d & 1, d >> 1);
         }
         {
             Timed t(B, K_LS_OTHER);

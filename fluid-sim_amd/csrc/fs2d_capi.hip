@@ -371,27 +371,32 @@ void stage_level_set(fs2d_sim* s) {
             hipLaunchKernelGGL(ls_surface_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, b, a, s->nx,
                                s->ny);
         }
-        // eikonal passes on the sweep engine, in SK4 order of each direction's
-        // flipped coordinates (the solver's SK4 scratch: z, q, cpl in; w, s1 out)
-        for (int rep = 0; rep < 4; rep++)
-            for (int d = 0; d < 4; d++) {
-                const int fx = d & 1, fy = d >> 1;
-                {
-                    Timed t(s, K_LS_OTHER);
-                    hipLaunchKernelGGL(eik_to_sk_kernel, dim3(sk_blocks(s)), dim3(kBlock), 0, s->stream, s->geom, a,
-                                       s->z, s->q, s->cpl, fx, fy);
-                }
-                {
-                    Timed t(s, K_LS_SWEEP);
-                    launch_sweep<EikSweepOp, false>(s, EikSweepOp{{s->z, s->q}, s->cpl, {s->w, s->s[1]}, s->dx});
-                }
-                {
-                    Timed t(s, K_LS_OTHER);
-                    hipLaunchKernelGGL(eik_from_sk_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->geom,
-                                       s->w, s->s[1], b, fx, fy);
-                }
-                std::swap(a, b);
+        // 16 eikonal passes (directions d = k mod 4: fx = d & 1, fy = d >> 1) on
+        // the sweep engine, in SK4 order of each direction's flipped
+        // coordinates, in the solver's SK4 scratch (z, q, cpl in; w, s1 out).
+        // Each pass's output moves straight to the next pass's order; phi is
+        // row-major only before the first pass (in a) and after the last (in
+        // a again: the 16 passes of the wavefront form read a, wrote b and
+        // swapped, an even number of times).
+        {
+            Timed t(s, K_LS_OTHER);
+            hipLaunchKernelGGL(eik_to_sk_kernel, dim3(sk_blocks(s)), dim3(kBlock), 0, s->stream, s->geom, a, s->z,
+                               s->q, s->cpl, 0, 0);
+        }
+        for (int k = 0; k < 16; k++) {
+            const int d = k & 3, d2 = (k + 1) & 3;
+            {
+                Timed t(s, K_LS_SWEEP);
+                launch_sweep<EikSweepOp, false>(s, EikSweepOp{{s->z, s->q}, s->cpl, {s->w, s->s[1]}, s->dx});
             }
+            Timed t(s, K_LS_OTHER);
+            if (k < 15)
+                hipLaunchKernelGGL(eik_sk_to_sk_kernel, dim3(sk_blocks(s)), dim3(kBlock), 0, s->stream, s->geom,
+                                   s->w, s->s[1], s->z, s->q, s->cpl, d & 1, d >> 1, d2 & 1, d2 >> 1);
+            else
+                hipLaunchKernelGGL(eik_from_sk_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->geom,
+                                   s->w, s->s[1], a, d & 1, d >> 1);
+        }
         {
             Timed t(s, K_LS_OTHER);
             for (int rep = 0; rep < 2; rep++) {

@@ -1454,6 +1454,35 @@ __global__ __launch_bounds__(kBlock) void eik_from_sk_kernel(SkGeom g, const flo
     eik_from_sk_body(g, lo, hi, phi, fx, fy, blockIdx.x, gridDim.x);
 }
 
+// Between passes: the previous pass's output (SK4 order of (fx, fy)) moved to
+// the next pass's input order (fx2, fy2), with its flags.
+__device__ __forceinline__ void eik_sk_to_sk_body(SkGeom g, const float* __restrict__ lo, const float* __restrict__ hi,
+                                                  float* __restrict__ lo2, float* __restrict__ hi2,
+                                                  uint8_t* __restrict__ flag, int fx, int fy, int fx2, int fy2,
+                                                  int vb, int nvb) {
+    const long n = g.size();
+    for (long e2 = vb * (long)blockDim.x + threadIdx.x; e2 < n; e2 += (long)nvb * blockDim.x) {
+        int a2, b2;
+        const bool in = sk_cell(g, e2, a2, b2);
+        float vl = 0.0f, vh = 0.0f;
+        if (in) {
+            const int i = fx2 ? g.nx - 1 - a2 : a2, j = fy2 ? g.ny - 1 - b2 : b2;
+            const long e = g.idx(fx ? g.nx - 1 - i : i, fy ? g.ny - 1 - j : j);
+            vl = lo[e];
+            vh = hi[e];
+        }
+        lo2[e2] = vl;
+        hi2[e2] = vh;
+        flag[e2] = (uint8_t)(in && a2 >= 1 && b2 >= 1);
+    }
+}
+__global__ __launch_bounds__(kBlock) void eik_sk_to_sk_kernel(SkGeom g, const float* __restrict__ lo,
+                                                              const float* __restrict__ hi, float* __restrict__ lo2,
+                                                              float* __restrict__ hi2, uint8_t* __restrict__ flag,
+                                                              int fx, int fy, int fx2, int fy2) {
+    eik_sk_to_sk_body(g, lo, hi, lo2, hi2, flag, fx, fy, fx2, fy2, blockIdx.x, gridDim.x);
+}
+
 // The sweep operator: carried state phi (FP64) in two words; the row
 // neighbour is the lane's own previous result, the column neighbour lane
 // l-1's, moved by DPP (the edge lane's from the strip below).
