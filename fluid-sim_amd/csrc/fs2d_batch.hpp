@@ -464,6 +464,29 @@ __global__ __launch_bounds__(kBlock) void b_eik_sk_to_sk_kernel(const InstBufs* 
     const InstBufs& I = ins[b];
     eik_sk_to_sk_body(I.geom, I.w, I.s1, I.z, I.q, I.cpl, fx, fy, fx2, fy2, vb, nvb);
 }
+// The closest-particle passes on the sweep engine (ClosestSweepOp), as in the
+// single path: candidates from t0 (first pass) or from the previous pass's
+// output (w, SK4 order of direction pdir); the last output back to t0.
+__global__ __launch_bounds__(kBlock) void b_ls_cand_sk_kernel(const InstBufs* __restrict__ ins, BlockMap bm, int dir,
+                                                              int first, int pdir) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    ls_cand_sk_body(I.geom, first ? I.t0 : nullptr, I.w, pdir & 1, pdir >> 1, I.px, I.py, I.r, I.z, I.q, dir & 1,
+                    dir >> 1, I.dx, I.dr, vb, nvb);
+}
+__global__ __launch_bounds__(kBlock) void b_closest_t_kernel(const InstBufs* __restrict__ ins, BlockMap bm, int dir) {
+    int b, vb, nvb;
+    if (!bmap(bm, b, vb, nvb)) return;
+    const InstBufs& I = ins[b];
+    closest_t_to_rm_body(I.geom, I.w, I.t0, dir & 1, dir >> 1, vb, nvb);
+}
+template <int FX, int FY>
+struct ClosestSweepMaker {
+    __device__ ClosestSweepOp<FX, FY> operator()(const InstBufs& I, int) const {
+        return ClosestSweepOp<FX, FY>{{I.r, I.z, I.q}, nullptr, {I.w}, I.nx, I.ny, I.dx, I.dr};
+    }
+};
 struct EikSweepMaker {
     __device__ EikSweepOp operator()(const InstBufs& I, int) const {
         return EikSweepOp{{I.z, I.q}, I.cpl, {I.w, I.s1}, I.dx};

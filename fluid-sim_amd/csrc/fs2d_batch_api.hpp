@@ -285,24 +285,38 @@ void bstage_level_set(fs2d_batch* B) {
             Timed t(B, K_LS_OTHER);
             blaunch(B, b_ls_seed_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
         }
-        const WaveCtl c{nullptr, B->ls_ctl, B->werr};
         const int S = B->strips.total;
+        // closest-particle passes on the sweep engine, as the single path:
+        // candidates in SK4 order of the pass's direction from the previous
+        // pass's output, the pass, and the last output back to t0
         for (int k = 0; k < 16; k++) {
+            const int d = k & 3;
             {
                 Timed t(B, K_LS_OTHER);
-                blaunch(B, b_ls_cand_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells, k & 3, k & 1);
+                blaunch(B, b_ls_cand_sk_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells, d, k == 0 ? 1 : 0,
+                        (k + 3) & 3);
             }
-            Timed t(B, K_LS_SWEEP);
-            switch (k & 3) {
-                case 0: blaunch(B, b_wavefront_kernel<ClosestOp<0, 0>, ClosestMaker<0, 0>>, S, 64, B->d_ins,
-                                B->strips, c, ClosestMaker<0, 0>{k & 1}); break;
-                case 1: blaunch(B, b_wavefront_kernel<ClosestOp<1, 0>, ClosestMaker<1, 0>>, S, 64, B->d_ins,
-                                B->strips, c, ClosestMaker<1, 0>{k & 1}); break;
-                case 2: blaunch(B, b_wavefront_kernel<ClosestOp<0, 1>, ClosestMaker<0, 1>>, S, 64, B->d_ins,
-                                B->strips, c, ClosestMaker<0, 1>{k & 1}); break;
-                default: blaunch(B, b_wavefront_kernel<ClosestOp<1, 1>, ClosestMaker<1, 1>>, S, 64, B->d_ins,
-                                 B->strips, c, ClosestMaker<1, 1>{k & 1}); break;
+            {
+                Timed t(B, K_LS_SWEEP);
+                switch (d) {
+                    case 0: blaunch(B, b_sweep_kernel<ClosestSweepOp<0, 0>, false, ClosestSweepMaker<0, 0>>, S,
+                                    kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
+                                    ClosestSweepMaker<0, 0>{}); break;
+                    case 1: blaunch(B, b_sweep_kernel<ClosestSweepOp<1, 0>, false, ClosestSweepMaker<1, 0>>, S,
+                                    kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
+                                    ClosestSweepMaker<1, 0>{}); break;
+                    case 2: blaunch(B, b_sweep_kernel<ClosestSweepOp<0, 1>, false, ClosestSweepMaker<0, 1>>, S,
+                                    kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
+                                    ClosestSweepMaker<0, 1>{}); break;
+                    default: blaunch(B, b_sweep_kernel<ClosestSweepOp<1, 1>, false, ClosestSweepMaker<1, 1>>, S,
+                                     kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
+                                     ClosestSweepMaker<1, 1>{}); break;
+                }
             }
+        }
+        {
+            Timed t(B, K_LS_OTHER);
+            blaunch(B, b_closest_t_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells, 3);
         }
         {
             Timed t(B, K_LS_OTHER);

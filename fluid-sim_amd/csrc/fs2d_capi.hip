@@ -249,6 +249,8 @@ void launch_wavefront(fs2d_sim* s, const Op& op) {
 
 fs2d_status ensure_particle_capacity(fs2d_sim* s, long n) {
     if (n <= s->np_cap) return FS2D_OK;
+    // the closest-particle sweeps carry index + 1 in a 29-bit field
+    if (n >= (1L << kCandPosShift)) return fail(s, FS2D_ERR_CONFIG, "at most 2^29 - 1 particles");
     float* a[4];
     for (int k = 0; k < 4; k++) TRY(dalloc(s, &a[k], (size_t)n));
     int *key, *sorted;
@@ -322,6 +324,11 @@ void diagnostics(fs2d_sim* s) {
 template <class Op, bool B>
 void launch_sweep(fs2d_sim* s, const Op& op);
 int sk_blocks(const fs2d_sim* s);
+template <int FX, int FY>
+void launch_closest(fs2d_sim* s) {
+    launch_sweep<ClosestSweepOp<FX, FY>, false>(
+        s, ClosestSweepOp<FX, FY>{{s->r, s->z, s->q}, nullptr, {s->w}, s->nx, s->ny, s->dx, s->dr});
+}
 
 // FluidSim2D::createWaterLevelSet (src/FluidSim2D.cpp:653-732).
 void stage_level_set(fs2d_sim* s) {
@@ -344,24 +351,33 @@ void stage_level_set(fs2d_sim* s) {
             hipLaunchKernelGGL(ls_seed_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->px, s->py,
                                s->starts, s->counts, s->sorted, s->t0, s->nx, nc, s->dx, s->dr);
         }
-        int* tin = s->t0;
-        int* tout = s->t1;
-        for (int rep = 0; rep < 4; rep++)
-            for (int d = 0; d < 4; d++) {
-                {
-                    Timed t(s, K_LS_OTHER);
-                    hipLaunchKernelGGL(ls_cand_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, tin, s->px,
-                                       s->py, s->lsc, s->nx, s->ny, d & 1, d >> 1, s->dx, s->dr);
-                }
-                Timed t(s, K_LS_SWEEP);
-                switch (d) {
-                    case 0: launch_wavefront(s, ClosestOp<0, 0>{G(s->lsc), G(tout), s->nx, s->ny, s->dx, s->dr}); break;
-                    case 1: launch_wavefront(s, ClosestOp<1, 0>{G(s->lsc), G(tout), s->nx, s->ny, s->dx, s->dr}); break;
-                    case 2: launch_wavefront(s, ClosestOp<0, 1>{G(s->lsc), G(tout), s->nx, s->ny, s->dx, s->dr}); break;
-                    default: launch_wavefront(s, ClosestOp<1, 1>{G(s->lsc), G(tout), s->nx, s->ny, s->dx, s->dr}); break;
-                }
-                std::swap(tin, tout);
+        // 16 closest-particle passes (direction d = k mod 4: fx = d & 1, fy = d
+        // >> 1) on the sweep engine: candidates in SK4 order of the pass's
+        // flipped coordinates (the solver's scratch: r, z, q), read from t0
+        // (row-major) for the first pass and from the previous pass's output
+        // (w, SK4 order of its direction) after; the last output back to t0
+        for (int k = 0; k < 16; k++) {
+            const int d = k & 3, pd = (k + 3) & 3;
+            {
+                Timed t(s, K_LS_OTHER);
+                hipLaunchKernelGGL(ls_cand_sk_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->geom,
+                                   k == 0 ? s->t0 : nullptr, (const float*)s->w, pd & 1, pd >> 1, s->px, s->py, s->r,
+                                   s->z, s->q, d & 1, d >> 1, s->dx, s->dr);
             }
+            Timed t(s, K_LS_SWEEP);
+            switch (d) {
+                case 0: launch_closest<0, 0>(s); break;
+                case 1: launch_closest<1, 0>(s); break;
+                case 2: launch_closest<0, 1>(s); break;
+                default: launch_closest<1, 1>(s); break;
+            }
+        }
+        int* const tin = s->t0;
+        {
+            Timed t(s, K_LS_OTHER);
+            hipLaunchKernelGGL(closest_t_to_rm_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->geom,
+                               s->w, tin, 1, 1);
+        }
         double* a = s->phi;
         double* b = s->phi2;
         {

@@ -1223,44 +1223,175 @@ struct LsCand {
     float x, y;  // the particle's position
 };
 
+// tr(i, j): the adopted particle of cell (i, j) after the previous pass
+template <class TRead>
+__device__ __forceinline__ LsCand ls_cand_cell(TRead tr, const float* __restrict__ px,
+                                               const float* __restrict__ py, int nx, int ny, int i, int j, int fx,
+                                               int fy, double dx, double dr) {
+    LsCand best{HUGE_VAL, -1, 0, 0.0f, 0.0f};
+    const int tc = tr(i, j);
+    if (tc >= 0) {
+        const float x = px[tc], y = py[tc];
+        best = LsCand{corner_dist(i, j, x, y, dx, dr), tc, 0, x, y};
+    }
+    // not yet updated: (i+1, j) sweeping +x (position 2), (i-1, j) sweeping -x (position 1); likewise y
+    const int ii = fx ? i - 1 : i + 1, jj = fy ? j - 1 : j + 1;
+    if (ii >= 0 && ii < nx) {
+        const int tn = tr(ii, j);
+        if (tn >= 0) {
+            const float x = px[tn], y = py[tn];
+            const double d = corner_dist(i, j, x, y, dx, dr);
+            if (d < best.d) best = LsCand{d, tn, fx ? 1 : 2, x, y};
+        }
+    }
+    if (jj >= 0 && jj < ny) {
+        const int tn = tr(i, jj);
+        if (tn >= 0) {
+            const float x = px[tn], y = py[tn];
+            const double d = corner_dist(i, j, x, y, dx, dr);
+            if (d < best.d) best = LsCand{d, tn, fy ? 3 : 4, x, y};
+        }
+    }
+    return best;
+}
 __device__ __forceinline__ void ls_cand_body(const int* __restrict__ t_in, const float* __restrict__ px,
                                              const float* __restrict__ py, LsCand* __restrict__ out, int nx, int ny,
                                              int fx, int fy, double dx, double dr, int vb, int nvb) {
     const long n = (long)nx * ny;
-    for (long c = vb * (long)blockDim.x + threadIdx.x; c < n; c += (long)nvb * blockDim.x) {
-        const int i = (int)(c % nx), j = (int)(c / nx);
-        LsCand best{HUGE_VAL, -1, 0, 0.0f, 0.0f};
-        const int tc = t_in[c];
-        if (tc >= 0) {
-            const float x = px[tc], y = py[tc];
-            best = LsCand{corner_dist(i, j, x, y, dx, dr), tc, 0, x, y};
-        }
-        // not yet updated: (i+1, j) sweeping +x (position 2), (i-1, j) sweeping -x (position 1); likewise y
-        const int ii = fx ? i - 1 : i + 1, jj = fy ? j - 1 : j + 1;
-        if (ii >= 0 && ii < nx) {
-            const int tn = t_in[(long)j * nx + ii];
-            if (tn >= 0) {
-                const float x = px[tn], y = py[tn];
-                const double d = corner_dist(i, j, x, y, dx, dr);
-                if (d < best.d) best = LsCand{d, tn, fx ? 1 : 2, x, y};
-            }
-        }
-        if (jj >= 0 && jj < ny) {
-            const int tn = t_in[(long)jj * nx + i];
-            if (tn >= 0) {
-                const float x = px[tn], y = py[tn];
-                const double d = corner_dist(i, j, x, y, dx, dr);
-                if (d < best.d) best = LsCand{d, tn, fy ? 3 : 4, x, y};
-            }
-        }
-        out[c] = best;
-    }
+    auto tr = [&](int i, int j) { return t_in[(long)j * nx + i]; };
+    for (long c = vb * (long)blockDim.x + threadIdx.x; c < n; c += (long)nvb * blockDim.x)
+        out[c] = ls_cand_cell(tr, px, py, nx, ny, (int)(c % nx), (int)(c / nx), fx, fy, dx, dr);
 }
 __global__ __launch_bounds__(kBlock) void ls_cand_kernel(const int* __restrict__ t_in, const float* __restrict__ px,
                                                          const float* __restrict__ py, LsCand* __restrict__ out,
                                                          int nx, int ny, int fx, int fy, double dx, double dr) {
     ls_cand_body(t_in, px, py, out, nx, ny, fx, fy, dx, dr, blockIdx.x, gridDim.x);
 }
+
+// The closest-particle passes on the solver's sweep engine (fs2d_sweep.hpp).
+// A pass in direction (fx, fy) is a forward sweep in the flipped coordinates
+// (a, b) = (fx ? nx-1-i : i, fy ? ny-1-j : j).  Its candidates (those of
+// ls_cand_cell) are laid out in SK4 order of (a, b) as three words: (t + 1) |
+// pos << 29 (0: none), x, y.  The sweep recomputes a candidate's distance with
+// the same corner_dist call, so every comparison sees the same value.  Words
+// are moved, never computed on, so the bits survive the float arrays.  The
+// SK4 arrays are the solver's scratch (r, z, q in; w out), free between
+// projections.
+constexpr int kCandPosShift = 29;  // particle indices below 2^29 - 1
+__device__ __forceinline__ void ls_cand_sk_body(SkGeom g, const int* __restrict__ t_rm, const float* __restrict__ t_sk,
+                                                int pfx, int pfy, const float* __restrict__ px,
+                                                const float* __restrict__ py, float* __restrict__ w0,
+                                                float* __restrict__ w1, float* __restrict__ w2, int fx, int fy,
+                                                double dx, double dr, int vb, int nvb) {
+    // cells, not SK4 entries: the sweep never reads a padding entry's record
+    // (steps outside the grid reset its state), so padding needs no writes
+    const int nx = g.nx, ny = g.ny;
+    const long n = (long)nx * ny;
+    auto tr_rm = [&](int i, int j) { return t_rm[(long)j * nx + i]; };
+    auto tr_sk = [&](int i, int j) {
+        return (int)__float_as_uint(t_sk[g.idx(pfx ? nx - 1 - i : i, pfy ? ny - 1 - j : j)]);
+    };
+    for (long c = vb * (long)blockDim.x + threadIdx.x; c < n; c += (long)nvb * blockDim.x) {
+        const int i = (int)(c % nx), j = (int)(c / nx);
+        const LsCand best = t_rm ? ls_cand_cell(tr_rm, px, py, nx, ny, i, j, fx, fy, dx, dr)
+                                 : ls_cand_cell(tr_sk, px, py, nx, ny, i, j, fx, fy, dx, dr);
+        uint32_t r0 = 0;
+        float rx = 0.0f, ry = 0.0f;
+        if (best.t >= 0) {
+            r0 = (uint32_t)(best.t + 1) | ((uint32_t)best.pos << kCandPosShift);
+            rx = best.x;
+            ry = best.y;
+        }
+        const long e = g.idx(fx ? nx - 1 - i : i, fy ? ny - 1 - j : j);
+        w0[e] = __uint_as_float(r0);
+        w1[e] = rx;
+        w2[e] = ry;
+    }
+}
+__global__ __launch_bounds__(kBlock) void ls_cand_sk_kernel(SkGeom g, const int* __restrict__ t_rm,
+                                                            const float* __restrict__ t_sk, int pfx, int pfy,
+                                                            const float* __restrict__ px, const float* __restrict__ py,
+                                                            float* __restrict__ w0, float* __restrict__ w1,
+                                                            float* __restrict__ w2, int fx, int fy, double dx,
+                                                            double dr) {
+    ls_cand_sk_body(g, t_rm, t_sk, pfx, pfy, px, py, w0, w1, w2, fx, fy, dx, dr, blockIdx.x, gridDim.x);
+}
+// The adopted indices of a pass (SK4 order of (fx, fy)) back to row-major.
+__device__ __forceinline__ void closest_t_to_rm_body(SkGeom g, const float* __restrict__ tk, int* __restrict__ t,
+                                                     int fx, int fy, int vb, int nvb) {
+    const long n = (long)g.nx * g.ny;
+    for (long c = vb * (long)blockDim.x + threadIdx.x; c < n; c += (long)nvb * blockDim.x) {
+        const int i = (int)(c % g.nx), j = (int)(c / g.nx);
+        t[c] = (int)__float_as_uint(tk[g.idx(fx ? g.nx - 1 - i : i, fy ? g.ny - 1 - j : j)]);
+    }
+}
+__global__ __launch_bounds__(kBlock) void closest_t_to_rm_kernel(SkGeom g, const float* __restrict__ tk,
+                                                                 int* __restrict__ t, int fx, int fy) {
+    closest_t_to_rm_body(g, tk, t, fx, fy, blockIdx.x, gridDim.x);
+}
+
+// The sweep operator: carried state (t + 1, x, y) of the adopted particle (0:
+// none, the engine's zero boundary); step q of lane l is cell (a, b) = (t0 + q
+// - l, 64 s + l), and steps outside the grid leave the state empty, as the
+// wavefront form's boundary state.
+template <int FX, int FY>
+struct ClosestSweepOp {
+    static constexpr int NW = 3, NIN = 3, NF = 3, NB = 0, NOUT = 1;
+    const float* in[NIN];  // (t + 1) | pos << 29, x, y of the cell's candidate
+    const uint8_t* bytes;
+    float* out[NOUT];      // adopted particle index (bits; -1: none)
+    int nx, ny;
+    double dx, dr;
+    __device__ bool skip(bool) const { return false; }
+    __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double&) const {
+        x[0] = in[0];
+        x[1] = in[1];
+        x[2] = in[2];
+    }
+    __device__ void loader_done(int, int, double) const {}
+    template <bool B, bool C>
+    __device__ void group(const float4 (&x)[NF], uint32_t, uint32_t (&st)[NW], const uint4 (&hb)[NW],
+                          float (&o)[NOUT][4], uint32_t (&prod)[NW][4], int t0, int s) const {
+        const int lane = threadIdx.x & 63;
+        const int b = 64 * s + lane;
+        const int j = FY ? ny - 1 - b : b;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uint32_t nb[NW];
+#pragma unroll
+            for (int w = 0; w < NW; w++) nb[w] = nbr<B, C>(u4get(hb[w], q), st[w]);
+            const int a = t0 + q - lane;
+            const bool ing = a >= 0 && a < nx && b < ny;
+            const int i = FX ? nx - 1 - a : a;
+            const uint32_t r0 = __float_as_uint(f4get(x[0], q));
+            int bt = (int)(r0 & ((1u << kCandPosShift) - 1u)) - 1;
+            int bp = (int)(r0 >> kCandPosShift);
+            float bx = f4get(x[1], q), by = f4get(x[2], q);
+            double bd = bt >= 0 ? corner_dist(i, j, bx, by, dx, dr) : HUGE_VAL;
+            // the updated neighbours, as ClosestOp::compute: (a-1, b) then (a, b-1)
+            auto consider = [&](const uint32_t* sw, int pos) {
+                const int tn = (int)sw[0] - 1;
+                const float qx = u2f(sw[1]), qy = u2f(sw[2]);
+                const double d = corner_dist(i, j, qx, qy, dx, dr);
+                const bool take = (tn >= 0) & ((d < bd) | ((d == bd) & (pos < bp)));
+                bd = take ? d : bd;
+                bt = take ? tn : bt;
+                bp = take ? pos : bp;
+                bx = take ? qx : bx;
+                by = take ? qy : by;
+            };
+            consider(st, FX ? 2 : 1);
+            consider(nb, FY ? 4 : 3);
+            st[0] = ing ? (uint32_t)(bt + 1) : 0u;
+            st[1] = ing ? f2u(bx) : 0u;
+            st[2] = ing ? f2u(by) : 0u;
+#pragma unroll
+            for (int w = 0; w < NW; w++) prod[w][q] = st[w];
+            o[0][q] = __uint_as_float((uint32_t)bt);
+        }
+    }
+    __device__ void strip_done(int) const {}
+};
 
 // The sweep itself.  Carried state: (index, x, y) of the adopted particle.
 template <int FX, int FY>
@@ -1500,7 +1631,7 @@ struct EikSweepOp {
     __device__ void loader_done(int, int, double) const {}
     template <bool B, bool C>
     __device__ void group(const float4 (&x)[NF], uint32_t b4, uint32_t (&st)[NW], const uint4 (&hb)[NW],
-                          float (&o)[NOUT][4], uint32_t (&prod)[NW][4]) const {
+                          float (&o)[NOUT][4], uint32_t (&prod)[NW][4], int, int) const {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             uint32_t nb[NW];

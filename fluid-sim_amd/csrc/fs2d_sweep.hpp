@@ -158,11 +158,12 @@ struct SweepLds {
 //   void loader_done(int s, int loader, double acc)
 //   template <bool B, bool C> void group(const float4 (&x)[NF], uint32_t b4,
 //        uint32_t (&st)[NW], const uint4 (&hb)[NW], float (&o)[NOUT][4],
-//        uint32_t (&prod)[NW][4])
+//        uint32_t (&prod)[NW][4], int t, int s)
 //         4 steps; step q uses component q of x[a], byte q of b4 and, on the
 //         edge lane of a consuming strip, component q of hb[w]; st is the
 //         lane's state after the previous step, updated; o[n][q] output n;
-//         prod[w][q] state word w after step q
+//         prod[w][q] state word w after step q; t is the group's first step
+//         and s the strip, so step q of lane l is cell (t + q - l, 64 s + l)
 //   void strip_done(int s)     wave 0, after the last step
 template <class Op, bool B, bool C, bool P, int RING>
 __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<Op, RING>& L, int s, unsigned epoch,
@@ -271,7 +272,7 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
         for (int k = 0; k < kSub; k++) {
             const int j = B ? kSub - 1 - k : k;
             float o[NOUT][4];
-            op.template group<B, C>(cur.x[j], cur.b[j], st, cur.h[j], o, prod[j]);
+            op.template group<B, C>(cur.x[j], cur.b[j], st, cur.h[j], o, prod[j], t0 + 4 * j, s);
 #pragma unroll
             for (int n = 0; n < NOUT; n++) {
                 float* base = op.out[n] + strip_base + (long)t0 * 64;  // uniform
@@ -578,7 +579,7 @@ struct FactorOp {
     __device__ void loader_done(int, int, double) const {}
     template <bool B, bool C>
     __device__ void group(const float4 (&x)[NF], uint32_t b4, uint32_t (&st)[NW], const uint4 (&hb)[NW],
-                          float (&o)[NOUT][4], uint32_t (&prod)[NW][4]) const {
+                          float (&o)[NOUT][4], uint32_t (&prod)[NW][4], int, int) const {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             uint32_t nb[NW];
@@ -643,7 +644,7 @@ struct FwdOp {
     __device__ void loader_done(int, int, double) const {}
     template <bool B, bool C>
     __device__ void group(const float4 (&x)[NF], uint32_t, uint32_t (&st)[NW], const uint4 (&hb)[NW],
-                          float (&o)[NOUT][4], uint32_t (&prod)[NW][4]) const {
+                          float (&o)[NOUT][4], uint32_t (&prod)[NW][4], int, int) const {
         float qp = u2f(st[0]);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -691,7 +692,7 @@ struct BwdOp {
     }
     template <bool B, bool C>
     __device__ void group(const float4 (&x)[NF], uint32_t, uint32_t (&st)[NW], const uint4 (&hb)[NW],
-                          float (&o)[NOUT][4], uint32_t (&prod)[NW][4]) const {
+                          float (&o)[NOUT][4], uint32_t (&prod)[NW][4], int, int) const {
         float zp = u2f(st[0]);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
