@@ -34,7 +34,6 @@ struct InstBufs {
     uint8_t* cell2;
     double *phi, *phi2;
     int *t0, *t1;
-    LsCand* lsc;
     float *p, *r, *z, *q, *w, *s0, *s1, *pk, *Adiag, *P, *fx, *fy, *gx_, *gy_;
     uint8_t* cpl;
     PcgScalars* sc;
@@ -47,7 +46,6 @@ struct InstBufs {
     int *pkey, *counts, *starts, *cursor, *sorted, *tile_sums;
     Diag* dg;
     double* partials;
-    unsigned long long* ls_handoff;  // hand-off rows of the batched level-set passes
     unsigned long long* sk_handoff;  // hand-off rows of the batched solver passes
 };
 
@@ -365,24 +363,6 @@ __global__ void b_gather_status_kernel(const InstBufs* __restrict__ ins, int n, 
     }
 }
 
-// ---------------------------------------------------------------------------
-// Wavefront passes over all simulations.
-// ---------------------------------------------------------------------------
-// Level-set sweeps towards -x (FX) / -y (FY), reading the parity's input buffer.
-template <int FX, int FY>
-struct ClosestMaker {
-    int parity;
-    __device__ ClosestOp<FX, FY> operator()(const InstBufs& I) const {
-        return ClosestOp<FX, FY>{G(I.lsc), G(parity ? I.t0 : I.t1), I.nx, I.ny, I.dx, I.dr};
-    }
-};
-template <int FX, int FY>
-struct EikonalMaker {
-    int parity;
-    __device__ EikonalOp<FX, FY> operator()(const InstBufs& I) const {
-        return EikonalOp<FX, FY>{G(parity ? I.phi2 : I.phi), G(parity ? I.phi : I.phi2), I.nx, I.ny, I.dx};
-    }
-};
 // Cells from particles (createWaterLevelSet without USE_LEVEL_SET).
 __global__ __launch_bounds__(kBlock) void b_pc_clear_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
     int b, vb, nvb;
@@ -407,27 +387,6 @@ __global__ __launch_bounds__(kBlock) void b_pc_fill_kernel(const InstBufs* __res
     if (!bmap(bm, b, vb, nvb)) return;
     const InstBufs& I = ins[b];
     pc_fill_body(I.cell, I.cell2, I.nx, I.ny, vb, nvb);
-}
-
-// Fixed candidates of closest-particle sweep `dir` from the parity's input.
-__global__ __launch_bounds__(kBlock) void b_ls_cand_kernel(const InstBufs* __restrict__ ins, BlockMap bm, int dir,
-                                                           int parity) {
-    int b, vb, nvb;
-    if (!bmap(bm, b, vb, nvb)) return;
-    const InstBufs& I = ins[b];
-    ls_cand_body(parity ? I.t1 : I.t0, I.px, I.py, I.lsc, I.nx, I.ny, dir & 1, dir >> 1, I.dx, I.dr, vb, nvb);
-}
-template <class Op, class Maker>
-__global__ __launch_bounds__(64) void b_wavefront_kernel(const InstBufs* __restrict__ ins, StripMap sm, WaveCtl c,
-                                                         Maker mk) {
-    const WaveTicket tk = wave_ticket(c);
-    if (tk.err == 0 && (int)tk.ticket < sm.total) {
-        const int b = sm.inst[tk.ticket];
-        const InstBufs& I = ins[b];
-        Op op = mk(I);
-        if (!op.skip()) wavefront_strip(op, I.nx, I.ny, (int)tk.ticket - sm.first[b], tk.epoch, I.ls_handoff, c.err);
-    }
-    wave_finish(c, (unsigned)sm.total, tk.epoch);
 }
 
 // Solver passes.  check: the forward solve applies the stopping test first;

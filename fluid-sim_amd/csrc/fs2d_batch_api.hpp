@@ -24,7 +24,6 @@ struct fs2d_batch {
     int g_cells = 0, g_faces = 0, g_sl = 0, g_part = 0, g_diag = 0, g_sk = 0, g_tiles = 0, g_one = 0, g_xrows = 0,
         g_xcols = 0;
     StripMap strips{};
-    unsigned* ls_ctl = nullptr;
     int* werr = nullptr;
     unsigned long long* sk_ticket = nullptr;
     int* d_status = nullptr;
@@ -84,7 +83,7 @@ fs2d_status balloc(fs2d_batch* B, T** p, size_t n) {
 }
 
 // Descriptor of member s (the hand-off rows are the batch's).
-InstBufs inst_bufs(const fs2d_sim* s, unsigned long long* ls_handoff, unsigned long long* sk_handoff) {
+InstBufs inst_bufs(const fs2d_sim* s, unsigned long long* sk_handoff) {
     InstBufs I;
     memset(&I, 0, sizeof I);  // padding too: descriptors are compared bytewise
     I.nx = s->nx; I.ny = s->ny; I.mode = s->mode; I.flags = s->flags; I.np = s->np;
@@ -93,7 +92,7 @@ InstBufs inst_bufs(const fs2d_sim* s, unsigned long long* ls_handoff, unsigned l
     I.pmass = s->k > 0 ? s->rho * s->dx * s->dx / (s->k * s->k) : 0.0;
     I.geom = s->geom;
     I.u = s->u; I.v = s->v; I.u2 = s->u2; I.v2 = s->v2; I.nu = s->nu; I.nv = s->nv; I.dens = s->dens; I.dens2 = s->dens2;
-    I.cell = s->cell; I.cell2 = s->cell2; I.phi = s->phi; I.phi2 = s->phi2; I.t0 = s->t0; I.t1 = s->t1; I.lsc = s->lsc;
+    I.cell = s->cell; I.cell2 = s->cell2; I.phi = s->phi; I.phi2 = s->phi2; I.t0 = s->t0; I.t1 = s->t1;
     I.p = s->p; I.r = s->r; I.z = s->z; I.q = s->q; I.w = s->w; I.s0 = s->s[0]; I.s1 = s->s[1]; I.pk = s->pk;
     I.Adiag = s->Adiag; I.P = s->P; I.fx = s->fx; I.fy = s->fy; I.gx_ = s->gx_; I.gy_ = s->gy_; I.cpl = s->cpl;
     I.sc = s->sc; I.qq = s->qq; I.ws_part = s->ws_part; I.rmax_part = s->rmax_part;
@@ -102,7 +101,7 @@ InstBufs inst_bufs(const fs2d_sim* s, unsigned long long* ls_handoff, unsigned l
     I.px = s->px; I.py = s->py; I.pvx = s->pvx; I.pvy = s->pvy;
     I.pkey = s->pkey; I.counts = s->counts; I.starts = s->starts; I.cursor = s->cursor; I.sorted = s->sorted;
     I.tile_sums = s->tile_sums; I.dg = s->dg; I.partials = s->partials;
-    I.ls_handoff = ls_handoff; I.sk_handoff = sk_handoff;
+    I.sk_handoff = sk_handoff;
     return I;
 }
 
@@ -228,7 +227,7 @@ fs2d_status build_active(fs2d_batch* B, const std::vector<int>& active) {
 fs2d_status refresh_inst(fs2d_batch* B) {
     bool changed = false, counts = false;
     for (int b = 0; b < B->n; b++) {
-        InstBufs I = inst_bufs(B->sims[b], B->h_ins[b].ls_handoff, B->h_ins[b].sk_handoff);
+        InstBufs I = inst_bufs(B->sims[b], B->h_ins[b].sk_handoff);
         if (memcmp(&I, &B->h_ins[b], sizeof I) != 0) {
             counts = counts || I.np != B->h_ins[b].np;
             B->h_ins[b] = I;
@@ -589,14 +588,13 @@ fs2d_status fs2d_batch_create(const fs2d_config* cfgs, int n, fs2d_batch** out) 
     for (int b = 0; b < n; b++) {
         fs2d_sim* s = B->sims[b];
         const int ns = (s->ny + 63) / 64;
-        unsigned long long *lsh = nullptr, *skh = nullptr;
-        if ((st = balloc(B, &lsh, (size_t)ns * s->nx * 4)) != FS2D_OK) return bail(st);
+        unsigned long long* skh = nullptr;
         if ((st = balloc(B, &skh, (size_t)ns * sk_handoff_stride(s->nx, 3))) != FS2D_OK) return bail(st);
-        B->h_ins[b] = inst_bufs(s, lsh, skh);
+        B->h_ins[b] = inst_bufs(s, skh);
     }
     if ((st = balloc(B, &B->d_ins, (size_t)n)) != FS2D_OK) return bail(st);
     if ((st = build_maps(B)) != FS2D_OK) return bail(st);
-    if ((st = balloc(B, &B->ls_ctl, 4)) != FS2D_OK || (st = balloc(B, &B->werr, 1)) != FS2D_OK ||
+    if ((st = balloc(B, &B->werr, 1)) != FS2D_OK ||
         (st = balloc(B, &B->sk_ticket, 1)) != FS2D_OK || (st = balloc(B, &B->d_status, 4 * (size_t)n)) != FS2D_OK)
         return bail(st);
     if (hipHostMalloc((void**)&B->h_status, 4 * n * sizeof(int), hipHostMallocDefault) != hipSuccess ||
@@ -604,9 +602,7 @@ fs2d_status fs2d_batch_create(const fs2d_config* cfgs, int n, fs2d_batch** out) 
         B->err = "hipHostMalloc failed";
         return bail(FS2D_ERR_OOM);
     }
-    const unsigned ctl0[4] = {0u, 0u, 1u, 0u};
-    if (hipMemcpyAsync(B->ls_ctl, ctl0, sizeof ctl0, hipMemcpyHostToDevice, B->stream) != hipSuccess ||
-        hipMemcpyAsync(B->d_ins, B->h_ins.data(), n * sizeof(InstBufs), hipMemcpyHostToDevice, B->stream) !=
+    if (hipMemcpyAsync(B->d_ins, B->h_ins.data(), n * sizeof(InstBufs), hipMemcpyHostToDevice, B->stream) !=
             hipSuccess ||
         hipStreamSynchronize(B->stream) != hipSuccess) {
         B->err = "batch upload failed";

@@ -64,7 +64,6 @@ struct fs2d_sim {
     uint8_t* cell2 = nullptr;  // flags written by the bubble fill (FS2D_FLAG_PARTICLE_CELLS)
     double *phi = nullptr, *phi2 = nullptr;
     int *t0 = nullptr, *t1 = nullptr;
-    LsCand* lsc = nullptr;  // fixed candidates of the closest-particle sweeps
     // projection: row-major pressure + SK4 solver arrays (fs2d_skew.hpp)
     float* p = nullptr;
     SkGeom geom{};
@@ -75,7 +74,7 @@ struct fs2d_sim {
     PcgScalars* h_sc = nullptr;
     double *qq = nullptr, *ws_part = nullptr, *rmax_part = nullptr;
     unsigned long long* sk_ticket = nullptr;
-    unsigned long long* sk_handoff = nullptr;  // separate from the level-set engine's: epochs differ
+    unsigned long long* sk_handoff = nullptr;
     // extrapolation
     uint32_t *mu = nullptr, *mv = nullptr;
     int *fu0 = nullptr, *fv0 = nullptr;    // entries in layer order
@@ -93,8 +92,6 @@ struct fs2d_sim {
     // scalars
     Diag* dg = nullptr;
     double* partials = nullptr;
-    unsigned long long* handoff = nullptr;
-    unsigned* wctl = nullptr;
     int* werr = nullptr;
     Diag* h_dg = nullptr;
     int* h_err = nullptr;
@@ -238,14 +235,6 @@ long n_v(const fs2d_sim* s) { return (long)s->nx * (s->ny + 1); }
 long n_c(const fs2d_sim* s) { return (long)s->nx * s->ny; }
 
 MacView mac_view(const fs2d_sim* s) { return MacView{s->u, s->v, s->nx, s->ny, s->dx}; }
-
-WaveCtl wave_ctl(fs2d_sim* s) { return WaveCtl{s->handoff, s->wctl, s->werr}; }
-
-template <class Op>
-void launch_wavefront(fs2d_sim* s, const Op& op) {
-    const int nstrips = (s->ny + 63) / 64;
-    hipLaunchKernelGGL(wavefront_kernel<Op>, dim3(nstrips), dim3(64), 0, s->stream, op, s->nx, s->ny, wave_ctl(s));
-}
 
 fs2d_status ensure_particle_capacity(fs2d_sim* s, long n) {
     if (n <= s->np_cap) return FS2D_OK;
@@ -392,8 +381,7 @@ void stage_level_set(fs2d_sim* s) {
         // coordinates, in the solver's SK4 scratch (z, q, cpl in; w, s1 out).
         // Each pass's output moves straight to the next pass's order; phi is
         // row-major only before the first pass (in a) and after the last (in
-        // a again: the 16 passes of the wavefront form read a, wrote b and
-        // swapped, an even number of times).
+        // a again, where the stages after redistance read it).
         {
             Timed t(s, K_LS_OTHER);
             hipLaunchKernelGGL(eik_to_sk_kernel, dim3(sk_blocks(s)), dim3(kBlock), 0, s->stream, s->geom, a, s->z,
@@ -775,7 +763,6 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
     if ((st = dalloc(s, &s->p, (n))) != FS2D_OK) return bail(st)
     A(u, nU); A(v, nV); A(u2, nU); A(v2, nV); A(nu, nU); A(nv, nV);
     A(dens, nC); A(dens2, nC); A(cell, nC); A(phi, nC); A(phi2, nC); A(t0, nC); A(t1, nC);
-    if (!(s->flags & (FS2D_FLAG_STATIC_CELLS | FS2D_FLAG_PARTICLE_CELLS))) A(lsc, nC);
     if (s->flags & FS2D_FLAG_PARTICLE_CELLS) A(cell2, nC);
     s->geom = make_geom(s->nx, s->ny);
     const size_t nS = (size_t)s->geom.size();
@@ -788,9 +775,8 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
     A(fpos, nU + nV); A(fsrc, nU + nV); A(fkv, nU + nV);
     A(counts, nC); A(starts, nC); A(cursor, nC); A(tile_sums, (nC + 1023) / 1024 + 1);
     A(dg, 1); A(partials, 3 * kMaxBlocks);
-    A(wctl, 4); A(werr, 1);
+    A(werr, 1);
     const int nstrips = (s->ny + 63) / 64;
-    A(handoff, (size_t)nstrips * s->nx * 4);
     A(sk_handoff, (size_t)nstrips * sk_handoff_stride(s->nx, 3));
 #undef A
     e = hipHostMalloc((void**)&s->h_sc, sizeof(PcgScalars), hipHostMallocDefault);
@@ -803,9 +789,7 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
     memset(s->h_sc, 0, sizeof(PcgScalars));
     memset(s->h_dg, 0, sizeof(Diag));
     *s->h_err = 0;
-    const unsigned ctl0[4] = {0u, 0u, 1u, 0u};
-    if (hipMemcpyAsync(s->wctl, ctl0, sizeof ctl0, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
-        hipMemcpyAsync(s->cell, cfg->initial_values, nC, hipMemcpyHostToDevice, s->stream) != hipSuccess) {
+    if (hipMemcpyAsync(s->cell, cfg->initial_values, nC, hipMemcpyHostToDevice, s->stream) != hipSuccess) {
         s->err = "initial upload failed";
         return bail(FS2D_ERR_DEVICE);
     }

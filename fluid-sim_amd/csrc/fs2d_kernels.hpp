@@ -18,7 +18,7 @@
 #include <stdint.h>
 
 #include "fs2d_device.hpp"
-#include "fs2d_wavefront.hpp"
+#include "fs2d_gmem.hpp"
 #include "fs2d_sweep.hpp"
 
 namespace fs2d {
@@ -1214,7 +1214,7 @@ __global__ __launch_bounds__(kBlock) void ls_seed_kernel(const float* __restrict
 // minimum of (distance, position in that order).  Three of the five do not
 // depend on the sweep front (the cell's own entry and the two neighbours not
 // yet updated in this sweep, all from the previous sweep's array), so
-// ls_cand_body merges them beforehand, in parallel, into one candidate; the
+// ls_cand_sk_body merges them beforehand, in parallel, into one candidate; the
 // sweep compares only the two candidates its front produces.  FX / FY: the
 // sweep runs towards -x / -y.
 struct LsCand {
@@ -1254,20 +1254,6 @@ __device__ __forceinline__ LsCand ls_cand_cell(TRead tr, const float* __restrict
     }
     return best;
 }
-__device__ __forceinline__ void ls_cand_body(const int* __restrict__ t_in, const float* __restrict__ px,
-                                             const float* __restrict__ py, LsCand* __restrict__ out, int nx, int ny,
-                                             int fx, int fy, double dx, double dr, int vb, int nvb) {
-    const long n = (long)nx * ny;
-    auto tr = [&](int i, int j) { return t_in[(long)j * nx + i]; };
-    for (long c = vb * (long)blockDim.x + threadIdx.x; c < n; c += (long)nvb * blockDim.x)
-        out[c] = ls_cand_cell(tr, px, py, nx, ny, (int)(c % nx), (int)(c / nx), fx, fy, dx, dr);
-}
-__global__ __launch_bounds__(kBlock) void ls_cand_kernel(const int* __restrict__ t_in, const float* __restrict__ px,
-                                                         const float* __restrict__ py, LsCand* __restrict__ out,
-                                                         int nx, int ny, int fx, int fy, double dx, double dr) {
-    ls_cand_body(t_in, px, py, out, nx, ny, fx, fy, dx, dr, blockIdx.x, gridDim.x);
-}
-
 // The closest-particle passes on the solver's sweep engine (fs2d_sweep.hpp).
 // A pass in direction (fx, fy) is a forward sweep in the flipped coordinates
 // (a, b) = (fx ? nx-1-i : i, fy ? ny-1-j : j).  Its candidates (those of
@@ -1332,8 +1318,8 @@ __global__ __launch_bounds__(kBlock) void closest_t_to_rm_kernel(SkGeom g, const
 
 // The sweep operator: carried state (t + 1, x, y) of the adopted particle (0:
 // none, the engine's zero boundary); step q of lane l is cell (a, b) = (t0 + q
-// - l, 64 s + l), and steps outside the grid leave the state empty, as the
-// wavefront form's boundary state.
+// - l, 64 s + l), and steps outside the grid leave the state empty: a
+// neighbour outside the grid offers no particle.
 template <int FX, int FY>
 struct ClosestSweepOp {
     static constexpr int NW = 3, NIN = 3, NF = 3, NB = 0, NOUT = 1;
@@ -1368,7 +1354,8 @@ struct ClosestSweepOp {
             int bp = (int)(r0 >> kCandPosShift);
             float bx = f4get(x[1], q), by = f4get(x[2], q);
             double bd = bt >= 0 ? corner_dist(i, j, bx, by, dx, dr) : HUGE_VAL;
-            // the updated neighbours, as ClosestOp::compute: (a-1, b) then (a, b-1)
+            // the updated neighbours, (a-1, b) then (a, b-1): positions 1 / 3 sweeping
+            // +x / +y, else 2 / 4; selects, not branches
             auto consider = [&](const uint32_t* sw, int pos) {
                 const int tn = (int)sw[0] - 1;
                 const float qx = u2f(sw[1]), qy = u2f(sw[2]);
@@ -1391,65 +1378,6 @@ struct ClosestSweepOp {
         }
     }
     __device__ void strip_done(int) const {}
-};
-
-// The sweep itself.  Carried state: (index, x, y) of the adopted particle.
-template <int FX, int FY>
-struct ClosestOp {
-    static constexpr int NW = 3;
-    gptr<const LsCand> cand;
-    gptr<int> t_out;
-    int nx, ny;
-    double dx, dr;
-    using Rec = LsCand;
-    __device__ bool skip() const { return false; }
-    __device__ void boundary(uint32_t* s) const {
-        s[0] = 0xFFFFFFFFu;
-        s[1] = s[2] = 0;
-    }
-    // Branch-free load: an out-of-grid address is clamped into the array and
-    // its value replaced (wait counts stay exact, see wavefront_strip).
-    __device__ Rec load(int a, int b, bool ok) const {
-        const int i = clampi(FX ? nx - 1 - a : a, 0, nx - 1);
-        const int j = clampi(FY ? ny - 1 - b : b, 0, ny - 1);
-        const gptr<const LsCand> p = cand + ((long)j * nx + i);
-        LsCand c;  // field by field: a struct copy cannot read through a global-typed pointer
-        c.d = p->d;
-        c.t = p->t;
-        c.pos = p->pos;
-        c.x = p->x;
-        c.y = p->y;
-        return ok ? c : LsCand{HUGE_VAL, -1, 0, 0.0f, 0.0f};
-    }
-    __device__ void compute(int a, int b, const Rec& r, const uint32_t* left, const uint32_t* below,
-                            uint32_t* out) {
-        const int i = FX ? nx - 1 - a : a;
-        const int j = FY ? ny - 1 - b : b;
-        double bd = r.d;
-        int bt = r.t, bp = r.pos;
-        float bx = r.x, by = r.y;
-        // the updated neighbours: (i-1, j) / (j-1) sweeping +x / +y (positions 1, 3), else (i+1, j) / (j+1);
-        // selects rather than branches (the distance of an absent candidate is computed and discarded)
-        auto consider = [&](const uint32_t* st, int pos) {
-            const int tn = (int)st[0];
-            const float qx = u2f(st[1]), qy = u2f(st[2]);
-            const double d = corner_dist(i, j, qx, qy, dx, dr);
-            const bool take = (tn >= 0) & ((d < bd) | ((d == bd) & (pos < bp)));  // no short-circuit branches
-            bd = take ? d : bd;
-            bt = take ? tn : bt;
-            bp = take ? pos : bp;
-            bx = take ? qx : bx;
-            by = take ? qy : by;
-        };
-        consider(left, FX ? 2 : 1);
-        consider(below, FY ? 4 : 3);
-        t_out[(long)j * nx + i] = bt;
-        out[0] = (uint32_t)bt;
-        out[1] = f2u(bx);
-        out[2] = f2u(by);
-    }
-    __device__ void strip_done(int) {}
-    __device__ void last_done(int) {}
 };
 
 // phi from the adopted particle; then the surface marks (:803-834) and the
@@ -1502,41 +1430,6 @@ __device__ __forceinline__ double eik_update(double cur, double left, double bel
     }
     return cur;
 }
-
-// One directional sweep of the eikonal update (:845-902) on cells with phi < 0,
-// for logical a >= 1 and b >= 1.  Carried state: phi (FP64).
-template <int FX, int FY>
-struct EikonalOp {
-    static constexpr int NW = 2;
-    gptr<const double> in;
-    gptr<double> outp;
-    int nx, ny;
-    double dx;
-    struct Rec {
-        double v;
-    };
-    __device__ bool skip() const { return false; }
-    __device__ void boundary(uint32_t* s) const { s[0] = s[1] = 0; }
-    __device__ Rec load(int a, int b, bool ok) const {
-        const int i = clampi(FX ? nx - 1 - a : a, 0, nx - 1);
-        const int j = clampi(FY ? ny - 1 - b : b, 0, ny - 1);
-        const double v = in[(long)j * nx + i];  // branch-free (see ClosestOp::load)
-        return Rec{ok ? v : 0.0};
-    }
-    __device__ void compute(int a, int b, const Rec& r, const uint32_t* left, const uint32_t* below,
-                            uint32_t* out) {
-#pragma clang fp contract(off)
-        const int i = FX ? nx - 1 - a : a;
-        const int j = FY ? ny - 1 - b : b;
-        const long e = (long)j * nx + i;
-        double cur = r.v;
-        if (a >= 1 && b >= 1) cur = eik_update(cur, u2d(left), u2d(below), dx);
-        outp[e] = cur;
-        d2u(cur, out);
-    }
-    __device__ void strip_done(int) {}
-    __device__ void last_done(int) {}
-};
 
 // ---------------------------------------------------------------------------
 // The eikonal passes on the solver's sweep engine (fs2d_sweep.hpp), whose

@@ -1,8 +1,12 @@
-// fs2d_sweep.hpp -- warp-specialised wavefront passes of the pressure solve.
+// fs2d_sweep.hpp -- warp-specialised wavefront passes: the pressure solve's
+// recurrences and the level set's Gauss-Seidel sweeps.
 //
 // The MIC(0) factor and both triangular solves of applyPreconditioner
 // (src/FluidSim2D.cpp:364-421) are lexicographic recurrences: cell (i, j)
 // needs (i-1, j) and (i, j-1) (forward) or (i+1, j) and (i, j+1) (backward).
+// So are the closest-particle and eikonal sweeps of the level set
+// (src/FluidSim2D.cpp:775-792, 845-902), each a forward sweep in coordinates
+// mirrored by its direction.
 //
 // One workgroup owns a strip of 64 rows j = 64 s + l.  Its compute wave holds
 // row l in lane l and at step t processes column i = t - l: the row neighbour
@@ -24,7 +28,7 @@
 //                   transform (e.g. P*r) and fill an LDS ring of supergroups.
 //   wave L+1        poller: reads the neighbouring strip's boundary row from
 //                   tagged 64-bit words ({payload, epoch}, agent-scope relaxed
-//                   atomics, see fs2d_wavefront.hpp) into an LDS ring indexed
+//                   atomics, see fs2d_gmem.hpp) into an LDS ring indexed
 //                   by step; steps whose edge column is outside the grid get
 //                   the boundary value.
 // The producing lane's 16 boundary values per supergroup go through a 2-entry
@@ -51,7 +55,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "fs2d_wavefront.hpp"
+#include "fs2d_gmem.hpp"
 
 namespace fs2d {
 
