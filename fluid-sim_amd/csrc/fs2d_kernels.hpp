@@ -285,36 +285,50 @@ __device__ __forceinline__ double block_sum_bcast(const double* a, int n) {
 // every neighbour exists.
 __device__ __forceinline__ long sk_step_back(long e) { return (e & 3) ? e - 1 : e - 253; }    // t-1, same lane
 __device__ __forceinline__ long sk_step_fwd(long e) { return (e & 3) != 3 ? e + 1 : e + 253; }  // t+1, same lane
+// (i, j-1) of an entry in lane 0: lane 63 of the strip below at step t+63;
+// (i, j+1) of an entry in lane 63: lane 0 of the strip above at step t-63.
+// Closed forms of SkGeom::idx from the entry's group r = e >> 8 and step
+// residue e4 = e & 3 (no decode of the cell, no division).
+__device__ __forceinline__ long sk_below_edge(const SkGeom& g, long e) {
+    const long e4 = e & 3;
+    return (((e >> 8) - g.T4 + 15 + ((e4 + 3) >> 2)) * 64 + 63) * 4 + ((e4 + 3) & 3);
+}
+__device__ __forceinline__ long sk_above_edge(const SkGeom& g, long e) {
+    const long e4 = e & 3;
+    return ((e >> 8) + g.T4 + ((e4 - 63) >> 2)) * 256 + ((e4 + 1) & 3);
+}
 __device__ __forceinline__ void pcg_spmv_sk_body(
     SkGeom g, const float* __restrict__ Adiag, const uint8_t* __restrict__ flags, const float* __restrict__ z,
     const float* __restrict__ s_old, float* __restrict__ s_new, float* __restrict__ w, float sA,
     PcgScalars* sc, const double* sig_cur, const double* sig_prev, int first, double* partials, int vb, int nvb) {
     if (sc->done) return;
+    // An entry's own values do not depend on beta: the first entry's are
+    // loaded before the reductions that give beta, and each later entry's at
+    // the end of the previous round.
+    const XcdLoop it = xcd_loop(g.size(), vb, nvb);
+    long e = it.e;
+    float so = 0.0f, zc = 0.0f, ad = 0.0f;
+    uint32_t f = 0;
+    if (e < it.end) {
+        so = s_old[e];
+        zc = z[e];
+        ad = Adiag[e];
+        f = flags[e];
+    }
     const double sig = block_sum_bcast(sig_cur, g.S * kLoaders);
     double beta_d = 0;
     if (!first) beta_d = sig / block_sum_bcast(sig_prev, g.S * kLoaders);
     if (vb == 0 && threadIdx.x == 0) sc->sigma = sig;
     const float beta = (float)beta_d;
     double acc = 0;
-    const XcdLoop it = xcd_loop(g.size(), vb, nvb);
-    for (long e = it.e; e < it.end; e += it.stride) {
-        const float sc_ = fmaf(beta, s_old[e], z[e]);
+    while (e < it.end) {
+        const float sc_ = fmaf(beta, so, zc);
         s_new[e] = sc_;
-        const float ad = Adiag[e];
         float wv = 0.0f;
         if (ad != 0.0f) {  // FLUID: interior, all four neighbours exist
-            const uint32_t f = flags[e];
             const int l = (int)((e >> 2) & 63);
-            long nd, nu;
-            if (l > 0 && l < 63) {
-                nd = sk_step_back(e) - 4;
-                nu = sk_step_fwd(e) + 4;
-            } else {
-                int i, j;
-                sk_cell(g, e, i, j);
-                nd = g.idx(i, j - 1);
-                nu = g.idx(i, j + 1);
-            }
+            const long nd = l > 0 ? sk_step_back(e) - 4 : sk_below_edge(g, e);
+            const long nu = l < 63 ? sk_step_fwd(e) + 4 : sk_above_edge(g, e);
             float nsum = 0.0f;
             if (f & CPL_L) { const long n = sk_step_back(e); nsum += fmaf(beta, s_old[n], z[n]); }
             if (f & CPL_R) { const long n = sk_step_fwd(e); nsum += fmaf(beta, s_old[n], z[n]); }
@@ -324,6 +338,13 @@ __device__ __forceinline__ void pcg_spmv_sk_body(
         }
         w[e] = wv;
         acc += (double)wv * (double)sc_;
+        e += it.stride;
+        if (e < it.end) {
+            so = s_old[e];
+            zc = z[e];
+            ad = Adiag[e];
+            f = flags[e];
+        }
     }
     double bs = block_sum(acc);
     if (threadIdx.x == 0) partials[vb] = bs;
@@ -343,31 +364,55 @@ __device__ __forceinline__ void pcg_axpy_sk_body(
     int n_ws, double* partials, int vb, int nvb) {
     if (sc->done) return;
     // n_ws < 0: sigma and w.s were already reduced into sc, in the same order
-    const double sig = n_ws < 0 ? sc->sigma : block_sum_bcast(sig_cur, g.S * kLoaders);
-    const double ws = n_ws < 0 ? sc->ws : block_sum_bcast(ws_partials, n_ws);
-    const float a = (float)(sig / ws);
-    float m = 0.0f;
-    // one float4 of each array per thread and iteration (the SK4 arrays hold
-    // a multiple of 256 entries)
+    // One float4 of each array per thread and round (the SK4 arrays hold a
+    // multiple of 256 entries).  Rounds go in chunks of U; the first chunk's
+    // loads do not depend on alpha and go out before the reductions that give
+    // it.
     float4* p4 = (float4*)p;
     float4* r4 = (float4*)r;
     const float4* s4 = (const float4*)s;
     const float4* w4 = (const float4*)w;
     const XcdLoop it = xcd_loop(g.size() / 4, vb, nvb);
-    for (long e = it.e; e < it.end; e += it.stride) {
-        const float4 sv = s4[e], wv = w4[e];
-        float4 pv = p4[e], rv = r4[e];
-        pv.x = fmaf(a, sv.x, pv.x);
-        pv.y = fmaf(a, sv.y, pv.y);
-        pv.z = fmaf(a, sv.z, pv.z);
-        pv.w = fmaf(a, sv.w, pv.w);
-        rv.x = fmaf(-a, wv.x, rv.x);
-        rv.y = fmaf(-a, wv.y, rv.y);
-        rv.z = fmaf(-a, wv.z, rv.z);
-        rv.w = fmaf(-a, wv.w, rv.w);
-        p4[e] = pv;
-        r4[e] = rv;
-        m = fmaxf(m, fmaxf(fmaxf(fabsf(rv.x), fabsf(rv.y)), fmaxf(fabsf(rv.z), fabsf(rv.w))));
+    constexpr int U = 2;
+    float4 sv[U], wv[U], pv[U], rv[U];
+    auto load = [&](long e0) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const long e = e0 + u * it.stride;
+            if (e < it.end) {
+                sv[u] = s4[e];
+                wv[u] = w4[e];
+                pv[u] = p4[e];
+                rv[u] = r4[e];
+            }
+        }
+    };
+    load(it.e);
+    const double sig = n_ws < 0 ? sc->sigma : block_sum_bcast(sig_cur, g.S * kLoaders);
+    const double ws = n_ws < 0 ? sc->ws : block_sum_bcast(ws_partials, n_ws);
+    const float a = (float)(sig / ws);
+    float m = 0.0f;
+    for (long e0 = it.e; e0 < it.end;) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const long e = e0 + u * it.stride;
+            if (e < it.end) {
+                float4 pn = pv[u], rn = rv[u];
+                pn.x = fmaf(a, sv[u].x, pn.x);
+                pn.y = fmaf(a, sv[u].y, pn.y);
+                pn.z = fmaf(a, sv[u].z, pn.z);
+                pn.w = fmaf(a, sv[u].w, pn.w);
+                rn.x = fmaf(-a, wv[u].x, rn.x);
+                rn.y = fmaf(-a, wv[u].y, rn.y);
+                rn.z = fmaf(-a, wv[u].z, rn.z);
+                rn.w = fmaf(-a, wv[u].w, rn.w);
+                p4[e] = pn;
+                r4[e] = rn;
+                m = fmaxf(m, fmaxf(fmaxf(fabsf(rn.x), fabsf(rn.y)), fmaxf(fabsf(rn.z), fabsf(rn.w))));
+            }
+        }
+        e0 += U * it.stride;
+        if (e0 < it.end) load(e0);
     }
     double bm = block_max((double)m);
     if (threadIdx.x == 0) partials[vb] = bm;

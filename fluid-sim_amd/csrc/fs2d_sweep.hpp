@@ -533,8 +533,16 @@ struct PcgScalars {
 };
 
 __device__ __forceinline__ double wave_max_of(const double* a, int n) {
+    // lane k takes a[k], a[k + 64], ... in that order, with 8 loads in flight
     double m = 0;
-    for (int k = threadIdx.x & 63; k < n; k += 64) m = a[k] > m ? a[k] : m;
+    for (int k0 = threadIdx.x & 63; k0 < n; k0 += 64 * 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = k0 + 64 * u < n ? a[k0 + 64 * u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (k0 + 64 * u < n) m = v[u] > m ? v[u] : m;
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         double x = __shfl_xor(m, o, 64);
@@ -544,7 +552,14 @@ __device__ __forceinline__ double wave_max_of(const double* a, int n) {
 }
 __device__ __forceinline__ double wave_sum_of(const double* a, int n) {
     double t = 0;
-    for (int k = threadIdx.x & 63; k < n; k += 64) t += a[k];
+    for (int k0 = threadIdx.x & 63; k0 < n; k0 += 64 * 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = k0 + 64 * u < n ? a[k0 + 64 * u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (k0 + 64 * u < n) t += v[u];
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
     return t;
