@@ -20,7 +20,7 @@ struct fs2d_batch {
     std::vector<InstBufs> h_ins;
     InstBufs* d_ins = nullptr;
     int* d_maps = nullptr;
-    BlockMap m_cells{}, m_faces{}, m_sl{}, m_part{}, m_diag{}, m_sk{}, m_tiles{}, m_one{}, m_xrows{}, m_xcols{};
+    BlockMap m_cells{}, m_faces{}, m_sl{}, m_part{}, m_diag{}, m_sk{}, m_tiles{}, m_one{}, m_xrows{}, m_xcols{}, m_pcg{};
     int g_cells = 0, g_faces = 0, g_sl = 0, g_part = 0, g_diag = 0, g_sk = 0, g_tiles = 0, g_one = 0, g_xrows = 0,
         g_xcols = 0;
     StripMap strips{};
@@ -135,7 +135,7 @@ int add_map(std::vector<int>& H, const std::vector<int>& cnt, MapOff& off) {
 fs2d_status build_maps(fs2d_batch* B) {
     const int n = B->n;
     std::vector<int> c_cells(n), c_faces(n), c_sl(n), c_part(n), c_diag(n), c_sk(n), c_tiles(n), c_one(n, 1);
-    std::vector<int> c_xrows(n), c_xcols(n);
+    std::vector<int> c_xrows(n), c_xcols(n), c_pcg(n);
     std::vector<int> nstrips(n);
     B->any_particles = false;
     for (int b = 0; b < n; b++) {
@@ -147,6 +147,7 @@ fs2d_status build_maps(fs2d_batch* B) {
         c_part[b] = s->np > 0 ? blocks_for(s->np) : 0;
         c_diag[b] = blocks_for(nc + s->np);
         c_sk[b] = sk_blocks(s);
+        c_pcg[b] = pcg_blocks(s);
         c_tiles[b] = (int)((nc + 1023) / 1024);
         c_xrows[b] = extrap_row_blocks(s->ny);
         c_xcols[b] = extrap_col_blocks(s->nx);
@@ -154,7 +155,7 @@ fs2d_status build_maps(fs2d_batch* B) {
         B->any_particles = B->any_particles || s->np > 0;
     }
     std::vector<int> H;
-    MapOff o_cells, o_faces, o_sl, o_part, o_diag, o_sk, o_tiles, o_one, o_xrows, o_xcols;
+    MapOff o_cells, o_faces, o_sl, o_part, o_diag, o_sk, o_tiles, o_one, o_xrows, o_xcols, o_pcg;
     B->g_cells = add_map(H, c_cells, o_cells);
     B->g_faces = add_map(H, c_faces, o_faces);
     B->g_sl = add_map(H, c_sl, o_sl);
@@ -165,6 +166,7 @@ fs2d_status build_maps(fs2d_batch* B) {
     B->g_one = add_map(H, c_one, o_one);
     B->g_xrows = add_map(H, c_xrows, o_xrows);
     B->g_xcols = add_map(H, c_xcols, o_xcols);
+    (void)add_map(H, c_pcg, o_pcg);
     // Strip list of the wavefront passes: each member's strips contiguous and
     // in order (a strip's upstream neighbour is taken earlier), members with
     // the longest strip chains first, so that their pipelines start in the
@@ -193,7 +195,7 @@ fs2d_status build_maps(fs2d_batch* B) {
     auto mk = [&](const MapOff& o) { return BlockMap{maps + o.inst, maps + o.first, maps + o.count}; };
     B->m_cells = mk(o_cells); B->m_faces = mk(o_faces); B->m_sl = mk(o_sl); B->m_part = mk(o_part);
     B->m_diag = mk(o_diag); B->m_sk = mk(o_sk); B->m_tiles = mk(o_tiles); B->m_one = mk(o_one);
-    B->m_xrows = mk(o_xrows); B->m_xcols = mk(o_xcols);
+    B->m_xrows = mk(o_xrows); B->m_xcols = mk(o_xcols); B->m_pcg = mk(o_pcg);
     B->strips = StripMap{maps + strip_inst, maps + strip_first, total};
     return FS2D_OK;
 }
@@ -203,7 +205,7 @@ fs2d_status build_maps(fs2d_batch* B) {
 fs2d_status build_active(fs2d_batch* B, const std::vector<int>& active) {
     const int n = B->n;
     std::vector<int> cnt(n);
-    for (int b = 0; b < n; b++) cnt[b] = active[b] ? sk_blocks(B->sims[b]) : 0;
+    for (int b = 0; b < n; b++) cnt[b] = active[b] ? pcg_blocks(B->sims[b]) : 0;
     B->h_act.clear();
     MapOff o;
     B->g_act = add_map(B->h_act, cnt, o);
@@ -430,13 +432,13 @@ fs2d_status bstage_projection(fs2d_batch* B) {
             }
             {
                 Timed t(B, K_PCG_AXPY);
-                blaunch(B, b_ws_reduce_kernel, B->n_act, 64, B->d_ins, B->act_members, B->n_act, B->m_sk.count);
+                blaunch(B, b_ws_reduce_kernel, B->n_act, 64, B->d_ins, B->act_members, B->n_act, B->m_pcg.count);
                 blaunch(B, b_axpy_kernel, B->g_act, kBlock, B->d_ins, B->m_act, h);
             }
             {
                 Timed t(B, K_PCG_FWD);
                 blaunch(B, b_sweep_kernel<FwdOp, false, FwdMaker>, S, kSweepThreads, B->d_ins, B->strips,
-                        B->sk_ticket, B->werr, FwdMaker{B->m_sk.count, 1});
+                        B->sk_ticket, B->werr, FwdMaker{B->m_pcg.count, 1});
             }
             {
                 Timed t(B, K_PCG_BWD);

@@ -313,6 +313,7 @@ void diagnostics(fs2d_sim* s) {
 template <class Op, bool B>
 void launch_sweep(fs2d_sim* s, const Op& op);
 int sk_blocks(const fs2d_sim* s);
+int pcg_blocks(const fs2d_sim* s);
 template <int FX, int FY>
 void launch_closest(fs2d_sim* s) {
     launch_sweep<ClosestSweepOp<FX, FY>, false>(
@@ -461,6 +462,15 @@ void launch_sweep(fs2d_sim* s, const Op& op) {
 
 // Workgroups of the SK4 stencil kernels: a multiple of 8 (see xcd_loop).
 int sk_blocks(const fs2d_sim* s) { return std::max(8, blocks_for(s->geom.size()) / 8 * 8); }
+// Workgroups of the PCG SpMV and axpy: about 8 entries per thread, at most
+// kMaxBlocks, a multiple of 8 (xcd_loop).  Each workgroup pays two partial
+// reductions and barriers, so one entry per thread (sk_blocks) made them
+// reduction-bound on small grids and in batches; the w.s and |r|_inf partial
+// counts follow this number, in the single and the batched step alike.
+int pcg_blocks(const fs2d_sim* s) {
+    const long per = (long)kBlock * 8;
+    return std::max(8, (int)std::min<long>(kMaxBlocks, (s->geom.size() + per - 1) / per) / 8 * 8);
+}
 
 // Ghost-fluid matrix and rhs of the current velocity (src/FluidSim2D.cpp:252-362).
 void assemble(fs2d_sim* s) {
@@ -500,6 +510,7 @@ fs2d_status pcg_solve(fs2d_sim* s, float* out) {
         launch_sweep<BwdOp, true>(s, op);
     }
     int h = 0;
+    const int nbp = pcg_blocks(s);
     int batch = std::max(2, std::min(s->last_iters_hint, s->max_iters));
     while (h < s->max_iters) {
         const int nbatch = std::min(batch, s->max_iters - h);
@@ -510,18 +521,18 @@ fs2d_status pcg_solve(fs2d_sim* s, float* out) {
             double* sig_prev = (h & 1) ? qq0 : qq1;  // sigma_{h-1}; receives sigma_{h+1}
             {
                 Timed t(s, K_PCG_SPMV);
-                hipLaunchKernelGGL(pcg_spmv_sk_kernel, dim3(nb), dim3(kBlock), 0, s->stream, g, s->Adiag, s->cpl,
+                hipLaunchKernelGGL(pcg_spmv_sk_kernel, dim3(nbp), dim3(kBlock), 0, s->stream, g, s->Adiag, s->cpl,
                                    s->z, s_old, s_new, s->w, (float)sA, s->sc, sig_cur, sig_prev, h == 0 ? 1 : 0,
                                    s->ws_part);
             }
             {
                 Timed t(s, K_PCG_AXPY);
-                hipLaunchKernelGGL(pcg_axpy_sk_kernel, dim3(nb), dim3(kBlock), 0, s->stream, g, s->pk, s->r,
-                                   s_new, s->w, s->sc, sig_cur, s->ws_part, nb, s->rmax_part);
+                hipLaunchKernelGGL(pcg_axpy_sk_kernel, dim3(nbp), dim3(kBlock), 0, s->stream, g, s->pk, s->r,
+                                   s_new, s->w, s->sc, sig_cur, s->ws_part, nbp, s->rmax_part);
             }
             {
                 Timed t(s, K_PCG_FWD);
-                FwdOp op{{s->r, s->P, s->fx, s->fy}, nullptr, {s->q}, s->sc, s->rmax_part, nb, 1};
+                FwdOp op{{s->r, s->P, s->fx, s->fy}, nullptr, {s->q}, s->sc, s->rmax_part, nbp, 1};
                 launch_sweep<FwdOp, false>(s, op);
             }
             {
