@@ -462,14 +462,17 @@ void launch_sweep(fs2d_sim* s, const Op& op) {
 
 // Workgroups of the SK4 stencil kernels: a multiple of 8 (see xcd_loop).
 int sk_blocks(const fs2d_sim* s) { return std::max(8, blocks_for(s->geom.size()) / 8 * 8); }
-// Workgroups of the PCG SpMV and axpy: about 8 entries per thread, at most
-// kMaxBlocks, a multiple of 8 (xcd_loop).  Each workgroup pays two partial
-// reductions and barriers, so one entry per thread (sk_blocks) made them
-// reduction-bound on small grids and in batches; the w.s and |r|_inf partial
-// counts follow this number, in the single and the batched step alike.
+// Workgroups of the PCG SpMV and axpy, a multiple of 8 (xcd_loop): about 8
+// entries per thread, at most kMaxBlocks, but no fewer than sk_blocks' one
+// entry per thread would give up to 256.  Each workgroup pays partial
+// reductions and barriers, so one entry per thread made the batched step
+// reduction-bound; a lone small grid still wants its 256 workgroups for
+// latency (a 256^2 solve took 17% longer at 40).  The w.s and |r|_inf
+// partial counts follow this number, in the single and the batched step alike.
 int pcg_blocks(const fs2d_sim* s) {
     const long per = (long)kBlock * 8;
-    return std::max(8, (int)std::min<long>(kMaxBlocks, (s->geom.size() + per - 1) / per) / 8 * 8);
+    const int want = (int)std::min<long>(kMaxBlocks, (s->geom.size() + per - 1) / per) / 8 * 8;
+    return std::max(std::max(8, want), std::min(sk_blocks(s), 256));
 }
 
 // Ghost-fluid matrix and rhs of the current velocity (src/FluidSim2D.cpp:252-362).
