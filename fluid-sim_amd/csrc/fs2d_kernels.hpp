@@ -185,8 +185,10 @@ __device__ __forceinline__ XcdLoop xcd_loop(long n, int vb, int nvb) {
 // Decode an SK4 element index into its cell; false for padding entries.
 __device__ __forceinline__ bool sk_cell(const SkGeom& g, long e, int& i, int& j) {
     const int e4 = (int)(e & 3), l = (int)((e >> 2) & 63);
-    const long r = e >> 8;
-    const int grp = (int)(r % g.T4), s = (int)(r / g.T4);
+    // 32-bit quotient: e >> 8 < 2^32 for any array that fits in memory
+    const unsigned r = (unsigned)(e >> 8), t4 = (unsigned)g.T4;
+    const unsigned sq = r / t4;
+    const int grp = (int)(r - sq * t4), s = (int)sq;
     const int t = grp * 4 + e4;
     i = t - l;
     j = s * 64 + l;
@@ -1314,16 +1316,19 @@ __device__ __forceinline__ void ls_cand_sk_body(SkGeom g, const int* __restrict_
                                                 const float* __restrict__ py, float* __restrict__ w0,
                                                 float* __restrict__ w1, float* __restrict__ w2, int fx, int fy,
                                                 double dx, double dr, int vb, int nvb) {
-    // cells, not SK4 entries: the sweep never reads a padding entry's record
-    // (steps outside the grid reset its state), so padding needs no writes
+    // SK4 entries of this pass, so the three stores coalesce; the sweep never
+    // reads a padding entry's record (steps outside the grid reset its
+    // state), so padding entries are skipped
     const int nx = g.nx, ny = g.ny;
-    const long n = (long)nx * ny;
     auto tr_rm = [&](int i, int j) { return t_rm[(long)j * nx + i]; };
     auto tr_sk = [&](int i, int j) {
         return (int)__float_as_uint(t_sk[g.idx(pfx ? nx - 1 - i : i, pfy ? ny - 1 - j : j)]);
     };
-    for (long c = vb * (long)blockDim.x + threadIdx.x; c < n; c += (long)nvb * blockDim.x) {
-        const int i = (int)(c % nx), j = (int)(c / nx);
+    const long n = g.size();
+    for (long e = vb * (long)blockDim.x + threadIdx.x; e < n; e += (long)nvb * blockDim.x) {
+        int a, b;
+        if (!sk_cell(g, e, a, b)) continue;
+        const int i = fx ? nx - 1 - a : a, j = fy ? ny - 1 - b : b;
         const LsCand best = t_rm ? ls_cand_cell(tr_rm, px, py, nx, ny, i, j, fx, fy, dx, dr)
                                  : ls_cand_cell(tr_sk, px, py, nx, ny, i, j, fx, fy, dx, dr);
         uint32_t r0 = 0;
@@ -1333,7 +1338,6 @@ __device__ __forceinline__ void ls_cand_sk_body(SkGeom g, const int* __restrict_
             rx = best.x;
             ry = best.y;
         }
-        const long e = g.idx(fx ? nx - 1 - i : i, fy ? ny - 1 - j : j);
         w0[e] = __uint_as_float(r0);
         w1[e] = rx;
         w2[e] = ry;
