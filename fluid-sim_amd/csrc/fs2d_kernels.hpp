@@ -1371,17 +1371,38 @@ __global__ __launch_bounds__(kBlock) void closest_t_to_rm_kernel(SkGeom g, const
 // neighbour outside the grid offers no particle.
 template <int FX, int FY>
 struct ClosestSweepOp {
-    static constexpr int NW = 3, NIN = 3, NF = 3, NB = 0, NOUT = 1;
+    static constexpr int NW = 3, NIN = 3, NF = 5, NB = 0, NOUT = 1;
     const float* in[NIN];  // (t + 1) | pos << 29, x, y of the cell's candidate
     const uint8_t* bytes;
     float* out[NOUT];      // adopted particle index (bits; -1: none)
     int nx, ny;
     double dx, dr;
     __device__ bool skip(bool) const { return false; }
-    __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double&) const {
+    // The loaders also compute the candidate's distance (the ring's words 3
+    // and 4: its low and high halves), which takes one of the compute wave's
+    // three FP64 corner distances per step off the chain.
+    __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double&, int t0, int s) const {
+        const int lane = threadIdx.x & 63;
+        const int b = 64 * s + lane;
+        const int j = FY ? ny - 1 - b : b;
         x[0] = in[0];
         x[1] = in[1];
         x[2] = in[2];
+        float lo[4], hi[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int a = t0 + q - lane;
+            const int i = FX ? nx - 1 - a : a;
+            const uint32_t r0 = __float_as_uint(f4get(in[0], q));
+            const int bt = (int)(r0 & ((1u << kCandPosShift) - 1u)) - 1;
+            const double bd = bt >= 0 ? corner_dist(i, j, f4get(in[1], q), f4get(in[2], q), dx, dr) : HUGE_VAL;
+            uint32_t w[2];
+            d2u(bd, w);
+            lo[q] = __uint_as_float(w[0]);
+            hi[q] = __uint_as_float(w[1]);
+        }
+        x[3] = make_float4(lo[0], lo[1], lo[2], lo[3]);
+        x[4] = make_float4(hi[0], hi[1], hi[2], hi[3]);
     }
     __device__ void loader_done(int, int, double) const {}
     template <bool B, bool C>
@@ -1402,7 +1423,8 @@ struct ClosestSweepOp {
             int bt = (int)(r0 & ((1u << kCandPosShift) - 1u)) - 1;
             int bp = (int)(r0 >> kCandPosShift);
             float bx = f4get(x[1], q), by = f4get(x[2], q);
-            double bd = bt >= 0 ? corner_dist(i, j, bx, by, dx, dr) : HUGE_VAL;
+            const uint32_t bw[2] = {__float_as_uint(f4get(x[3], q)), __float_as_uint(f4get(x[4], q))};
+            double bd = u2d(bw);  // the candidate's distance, from the loaders
             // the updated neighbours, (a-1, b) then (a, b-1): positions 1 / 3 sweeping
             // +x / +y, else 2 / 4; selects, not branches
             auto consider = [&](const uint32_t* sw, int pos) {
@@ -1566,7 +1588,7 @@ struct EikSweepOp {
     float* out[NOUT];      // low, high words of the result
     double dx;
     __device__ bool skip(bool) const { return false; }
-    __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double&) const {
+    __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double&, int, int) const {
         x[0] = in[0];
         x[1] = in[1];
     }

@@ -157,8 +157,9 @@ struct SweepLds {
 //         arrays in the ring; NB (0/1) byte input array `bytes`; NOUT float
 //         output arrays out[]
 //   bool skip(bool first)      wave 0 only; first: strip order 0
-//   void xform(const float4 (&in)[NIN], float4 (&x)[NF], double& acc)
-//         loader: ring values of one SK4 group from its inputs
+//   void xform(const float4 (&in)[NIN], float4 (&x)[NF], double& acc, int t, int s)
+//         loader: ring values of one SK4 group from its inputs; t and s as
+//         for group()
 //   void loader_done(int s, int loader, double acc)
 //   template <bool B, bool C> void group(const float4 (&x)[NF], uint32_t b4,
 //        uint32_t (&st)[NW], const uint4 (&hb)[NW], float (&o)[NOUT][4],
@@ -434,10 +435,11 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
             for (int k = 0; k < kBatch; k++) {
                 if (sb + k < NSG) {
                     const int slot = (sb + k) & (RING - 1);
+                    const int t0 = BACKWARD ? T - 16 * (sb + k + 1) : 16 * (sb + k);
 #pragma unroll
                     for (int j = 0; j < kSub; j++) {
                         float4 x[NF];
-                        op.xform(v[k][j], x, acc);
+                        op.xform(v[k][j], x, acc, t0 + 4 * j, s);
 #pragma unroll
                         for (int a = 0; a < NF; a++) L.ring_f[slot][j][a][lane] = x[a];
                         if (NB) L.ring_b[NB ? slot : 0][j][lane] = b[k][j];
@@ -510,9 +512,16 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
     }
 }
 
+// Supergroups in a single launch's ring: kRing, or 4 for operators with more
+// than 3 ring arrays (8 slots of 5 arrays would need 160 KB of LDS).
+template <class Op>
+constexpr int sweep_ring() {
+    return Op::NF > 3 ? 4 : kRing;
+}
+
 template <class Op, bool BACKWARD>
 __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(Op op, SkGeom g, SkCtl c) {
-    __shared__ __align__(16) SweepLds<Op> L;
+    __shared__ __align__(16) SweepLds<Op, sweep_ring<Op>()> L;
     const unsigned long long ticket = sweep_ticket(L, c.ticket);
     const int order = (int)(ticket % (unsigned long long)g.S);
     const unsigned epoch = (unsigned)(ticket / (unsigned long long)g.S) + 1u;
@@ -594,7 +603,7 @@ struct FactorOp {
         }
         return false;
     }
-    __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double&) const { x[0] = in[0]; }
+    __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double&, int, int) const { x[0] = in[0]; }
     __device__ void loader_done(int, int, double) const {}
     template <bool B, bool C>
     __device__ void group(const float4 (&x)[NF], uint32_t b4, uint32_t (&st)[NW], const uint4 (&hb)[NW],
@@ -655,7 +664,7 @@ struct FwdOp {
         }
         return conv;
     }
-    __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double&) const {
+    __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double&, int, int) const {
         x[0] = make_float4(in[1].x * in[0].x, in[1].y * in[0].y, in[1].z * in[0].z, in[1].w * in[0].w);
         x[1] = in[2];
         x[2] = in[3];
@@ -697,7 +706,7 @@ struct BwdOp {
         if (first && (threadIdx.x & 63) == 0 && next_iter > 0) sc->iter = next_iter;
         return false;
     }
-    __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double& acc) const {
+    __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double& acc, int, int) const {
         const float4 q = in[0], P = in[1];
         x[0] = make_float4(P.x * q.x, P.y * q.y, P.z * q.z, P.w * q.w);
         x[1] = in[2];
