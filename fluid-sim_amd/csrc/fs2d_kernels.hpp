@@ -365,7 +365,6 @@ __device__ __forceinline__ void pcg_axpy_sk_body(
     const float* __restrict__ w, const PcgScalars* sc, const double* sig_cur, const double* ws_partials,
     int n_ws, double* partials, int vb, int nvb) {
     if (sc->done) return;
-    // n_ws < 0: sigma and w.s were already reduced into sc, in the same order
     // One float4 of each array per thread and round (the SK4 arrays hold a
     // multiple of 256 entries).  Rounds go in chunks of U; the first chunk's
     // loads do not depend on alpha and go out before the reductions that give
@@ -390,6 +389,7 @@ __device__ __forceinline__ void pcg_axpy_sk_body(
         }
     };
     load(it.e);
+    // n_ws < 0: sigma and w.s were already reduced into sc, in the same order
     const double sig = n_ws < 0 ? sc->sigma : block_sum_bcast(sig_cur, g.S * kLoaders);
     const double ws = n_ws < 0 ? sc->ws : block_sum_bcast(ws_partials, n_ws);
     const float a = (float)(sig / ws);
