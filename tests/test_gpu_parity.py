@@ -292,12 +292,15 @@ def test_golden_semilag_rows_on_gpu():
     np.testing.assert_allclose(np.array(got)[:, 1:], np.array(ref[:2])[:, 1:], rtol=1e-6)
 
 
-@pytest.mark.parametrize("nx,ny", [(5, 130), (40, 200), (1000, 65), (77, 64), (130, 5)])
+@pytest.mark.parametrize("nx,ny", [(5, 130), (40, 200), (1000, 65), (77, 64), (130, 5), (1024, 600)])
 def test_projection_odd_shapes(nx, ny):
     """Matched-iteration PCG on shapes whose strips and columns do not fill the
     solver's tiles: a partial last strip (ny mod 64 != 0), a single strip,
     widths far from a multiple of 32 and a 5-wide grid (the hand-off rows then
-    carry mostly padding columns).  The narrow grid's fluid is kept short (16
+    carry mostly padding columns).  The shapes also cover the SpMV / axpy
+    workgroup counts: one entry per thread (77 x 64), the floor of 256
+    (1000 x 65) and about 8 entries per thread (1024 x 600: 336 workgroups,
+    so 336 w.s and |r| partials).  The narrow grid's fluid is kept short (16
     rows of 3 cells): a long thin column is a near-1D Laplacian whose condition
     number grows as its length squared (85 rows of one cell put the FP32 error
     at ~2e-4), and on which MIC(0) is nearly exact, so after a few iterations
