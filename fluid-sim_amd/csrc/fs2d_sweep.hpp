@@ -367,6 +367,31 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     __shared__ int sh_skip;
     const int s = BACKWARD ? g.S - 1 - order : order;
+    const int NSG = g.T4 / kSub, T = g.T4 * 4, nx = g.nx;
+
+    // A loader's batch of supergroups: all loads in flight together; indices
+    // past the strip are clamped (their ring writes are skipped later), so
+    // every register index stays a constant.
+    const long strip_base = (long)s * g.T4 * 256 + lane * 4;
+    float4 v[kBatch][kSub][NIN];
+    uint32_t vb[kBatch][kSub];
+    auto load_batch = [&](int sb) {
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) {
+            const int sg = min(sb + k, NSG - 1);
+            const int t0 = BACKWARD ? T - 16 * (sg + 1) : 16 * sg;
+#pragma unroll
+            for (int j = 0; j < kSub; j++) {
+                const long e = strip_base + (long)(t0 + 4 * j) * 64;
+#pragma unroll
+                for (int a = 0; a < NIN; a++) v[k][j][a] = *(const float4*)(op.in[a] + e);
+                vb[k][j] = NB ? *(const uint32_t*)(op.bytes + e) : 0u;
+            }
+        }
+    };
+    // A loader's first batch goes out before the skip test, whose reads it
+    // overlaps (loads only: a skipped pass discards them).
+    if (wave >= 1 && wave <= kLoaders) load_batch((wave - 1) * kBatch);
     if (wave == 0) {
         int sk = __hip_atomic_load(c.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
         if (!sk) sk = op.skip(order == 0);
@@ -375,7 +400,6 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
     __syncthreads();
     if (sh_skip) return;
 
-    const int NSG = g.T4 / kSub, T = g.T4 * 4, nx = g.nx;
     const bool consume = BACKWARD ? (s + 1 < g.S) : (s > 0);
     const bool produce = BACKWARD ? (s > 0) : (s + 1 < g.S);
     const int edge_lane = BACKWARD ? 63 : 0;
@@ -404,7 +428,6 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
         // are clamped and their ring writes skipped (a uniform branch), so
         // every register index stays a constant.
         const int me = wave - 1;
-        const long strip_base = (long)s * g.T4 * 256 + lane * 4;
         bool bad = false;
         double acc = 0.0;
         for (int sb = me * kBatch; sb < NSG; sb += kLoaders * kBatch) {
@@ -417,20 +440,7 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
-            float4 v[kBatch][kSub][NIN];
-            uint32_t b[kBatch][kSub];
-#pragma unroll
-            for (int k = 0; k < kBatch; k++) {
-                const int sg = min(sb + k, NSG - 1);
-                const int t0 = BACKWARD ? T - 16 * (sg + 1) : 16 * sg;
-#pragma unroll
-                for (int j = 0; j < kSub; j++) {
-                    const long e = strip_base + (long)(t0 + 4 * j) * 64;
-#pragma unroll
-                    for (int a = 0; a < NIN; a++) v[k][j][a] = *(const float4*)(op.in[a] + e);
-                    b[k][j] = NB ? *(const uint32_t*)(op.bytes + e) : 0u;
-                }
-            }
+            if (sb != me * kBatch) load_batch(sb);  // the first batch is already loaded
 #pragma unroll
             for (int k = 0; k < kBatch; k++) {
                 if (sb + k < NSG) {
@@ -442,7 +452,7 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
                         op.xform(v[k][j], x, acc, t0 + 4 * j, s);
 #pragma unroll
                         for (int a = 0; a < NF; a++) L.ring_f[slot][j][a][lane] = x[a];
-                        if (NB) L.ring_b[NB ? slot : 0][j][lane] = b[k][j];
+                        if (NB) L.ring_b[NB ? slot : 0][j][lane] = vb[k][j];
                     }
                 }
             }
