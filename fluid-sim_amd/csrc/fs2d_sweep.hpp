@@ -423,10 +423,9 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
             sweep_compute<Op, BACKWARD, false, false>(op, g, L, s, epoch, dst, c.err);
     } else if (wave <= kLoaders) {
         // ------------------------------------------------------------- loaders
-        // Loader m takes batches of kBatch supergroups, m-th of every kLoaders.
-        // All loads of a batch are in flight together; addresses past the strip
-        // are clamped and their ring writes skipped (a uniform branch), so
-        // every register index stays a constant.
+        // Loader m takes batches of kBatch supergroups, m-th of every kLoaders
+        // (load_batch above); ring writes past the strip are skipped (a
+        // uniform branch).
         const int me = wave - 1;
         bool bad = false;
         double acc = 0.0;
