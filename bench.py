@@ -45,9 +45,15 @@ random points, once in random order and once sorted by cell, and velDiv at
 every cell (16 algorithmic bytes per cell); HIP events on the grid's stream.
 
 `cpu_baseline` times the oracle (FP64 CPU restatement of the reference step,
-OpenMP) on the same starting state with PCG capped at two iteration counts,
-and scales to the GPU's mean iteration count: step = fixed + iters x
-per-iteration.  A full oracle step at 2048^2 takes minutes.
+OpenMP, one thread per physical core of this process's affinity mask, at most
+16, OMP_PROC_BIND=close) on the same starting state with PCG capped at three
+iteration counts, and extrapolates to the GPU's mean iteration count: step =
+fixed + iters x per-iteration, fitted on the outer two caps and checked on the
+middle one.  A full oracle step at 2048^2 takes minutes.  The host's CPU model,
+logical and physical core counts are reported beside it.
+
+`config.pcg_iters_max` and `config.pcg_capped_steps` show whether any timed
+step stopped at the iteration cap instead of converging.
 """
 import argparse
 import json
@@ -135,10 +141,43 @@ def build_gpu_sim(n, tol, tau, max_iters):
     return sim, cells, dt
 
 
-def cpu_baseline(state, cells, n, dt, tol, tau, threads, iters_gpu, caps=(20, 60)):
+def host_cpus():
+    """What the host offers this process: logical CPUs in its affinity mask,
+    the physical cores among them (distinct (package, core id) pairs in
+    /proc/cpuinfo) and the CPU model name."""
+    allowed = sorted(os.sched_getaffinity(0))
+    model, cores = None, set()
+    try:
+        phys, core, proc = None, None, None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "processor":
+                proc = int(v)
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif k == "model name" and model is None:
+                model = v
+            elif not k and proc is not None:
+                if proc in allowed:
+                    cores.add((phys, core))
+                proc = phys = core = None
+        if proc is not None and proc in allowed:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return {"nproc": len(allowed), "os_cpu_count": os.cpu_count(),
+            "physical_cores": len(cores) if cores else None, "model": model}
+
+
+def cpu_baseline(state, cells, n, dt, tol, tau, threads, iters_gpu, host, caps=(20, 40, 60)):
     """The oracle (CPU restatement, OpenMP) on steps of the same workload from
-    the GPU's starting state with PCG capped at caps[0] and caps[1]
-    iterations; per-step time scaled to the GPU's mean iteration count."""
+    the GPU's starting state with PCG capped at caps[0], caps[1] and caps[2]
+    iterations.  A full step at 2048^2 takes minutes, so the per-step time is
+    extrapolated: fixed + iterations x per-iteration, fitted on caps[0] and
+    caps[2] and checked against the step measured at caps[1]."""
     from oracle import oracle as O
     O.set_threads(threads)
     times = []
@@ -154,14 +193,22 @@ def cpu_baseline(state, cells, n, dt, tol, tau, threads, iters_gpu, caps=(20, 60
         o.close()
         if done != cap:
             raise RuntimeError("oracle converged in %d < %d iterations: sample too small" % (done, cap))
-    per_iter = (times[1] - times[0]) / (caps[1] - caps[0])
+    per_iter = (times[2] - times[0]) / (caps[2] - caps[0])
     fixed = times[0] - caps[0] * per_iter
     step = fixed + iters_gpu * per_iter
+    check = fixed + caps[1] * per_iter
     return {"value": n * n / step, "unit": "cell-steps/s", "cores": threads, "kind": "port",
-            "sample": "oracle (FP64, OpenMP) steps of the %dx%d workload from the GPU's starting state with "
-                      "PCG capped at %d and %d iterations (%.1f s, %.1f s): %.3f s fixed + %.4f s/iteration, "
-                      "scaled to %.1f iterations" % (n, n, caps[0], caps[1], times[0], times[1], fixed, per_iter,
-                                                     iters_gpu)}
+            "extrapolated": True,
+            "host": dict(host, omp_proc_bind=os.environ.get("OMP_PROC_BIND"),
+                         omp_places=os.environ.get("OMP_PLACES")),
+            "fit": {"caps": list(caps), "seconds": times, "fixed_s": fixed, "per_iter_s": per_iter,
+                    "check_rel_err": (check - times[1]) / times[1]},
+            "sample": "oracle (FP64, OpenMP, %d threads) steps of the %dx%d workload from the GPU's starting state "
+                      "with PCG capped at %d, %d and %d iterations (%.1f, %.1f, %.1f s); extrapolated: %.3f s fixed "
+                      "+ %.4f s/iteration (fitted on %d and %d, off by %.1f%% at %d), scaled to the GPU's %.1f "
+                      "iterations per step" % (threads, n, n, caps[0], caps[1], caps[2], times[0], times[1], times[2],
+                                               fixed, per_iter, caps[0], caps[2], 100 * (check - times[1]) / times[1],
+                                               caps[1], iters_gpu)}
 
 
 def ensemble_leg(count, steps, warmup, loop_steps):
@@ -337,7 +384,8 @@ def main():
     ap.add_argument("--tau", type=float, default=0.97)
     ap.add_argument("--max-iters", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads (0: the physical cores this process may run on, at most 16)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=5)
@@ -352,6 +400,12 @@ def main():
               "(each GPU would run its own independent grids; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    # read before OpenMP starts: with OMP_PROC_BIND its runtime pins this
+    # thread to one core, and the affinity mask shrinks to that core
+    host = host_cpus()
+    # OpenMP reads these when the oracle's runtime starts (the cpu_baseline leg)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import fs2d
     if args.only == "ensemble":
         print(json.dumps(ensemble_leg(args.ensemble_members, args.ensemble_steps, args.warmup, args.loop_steps)))
@@ -380,11 +434,14 @@ def main():
     sim.reset_kernel_times()
     sim.set_timing(True, only=[dom])
     iters = []
+    capped = 0
     sim.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         sim.step()
-        iters.append(sim.stats()["pcg_iters"])
+        st = sim.stats()
+        iters.append(st["pcg_iters"])
+        capped += int(st["pcg_hit_max"])
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     kt = sim.kernel_times()
@@ -414,7 +471,8 @@ def main():
             "workload": "all-fluid %dx%d semi-Lagrangian step: advect u, v, density; MIC(0)-PCG "
                         "projection to 1e-6; velocity update" % (n, n),
             "grid": [n, n], "pcg_tol": args.tol, "tau": args.tau, "cfl": 1.0, "dt": dt,
-            "pcg_iters_mean": float(np.mean(iters)),
+            "pcg_iters_mean": float(np.mean(iters)), "pcg_iters_max": int(max(iters)),
+            "pcg_max_iters": args.max_iters, "pcg_capped_steps": capped,
         },
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -432,8 +490,11 @@ def main():
     if not args.no_grid3d:
         out["grid3d"] = grid3d_leg()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(state, cells, n, dt, args.tol, args.tau, args.cpu_threads,
-                                           float(np.mean(iters)))
+        threads = args.cpu_threads
+        if threads <= 0:
+            threads = max(1, min(16, host["physical_cores"] or host["nproc"]))
+        out["cpu_baseline"] = cpu_baseline(state, cells, n, dt, args.tol, args.tau, threads,
+                                           float(np.mean(iters)), host)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

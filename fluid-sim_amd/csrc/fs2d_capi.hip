@@ -645,10 +645,9 @@ fs2d_status run_stage_impl(fs2d_sim* s, int stage, bool fuse_gravity, fs2d_statu
 
 bool valid_dtype(int field, int dtype) {
     if (field == FS2D_FIELD_CELL) return dtype == FS2D_U8;
+    if (field == FS2D_FIELD_PARTICLE_CELL || field == FS2D_FIELD_PARTICLE_ORDER) return dtype == FS2D_I32;
     return dtype == FS2D_F32 || dtype == FS2D_F64;
 }
-
-size_t dtype_size(int dtype) { return dtype == FS2D_F64 ? 8 : dtype == FS2D_F32 ? 4 : 1; }
 
 }  // namespace
 
@@ -927,6 +926,7 @@ size_t fs2d_field_count(const fs2d_sim* s, int field) {
         case FS2D_FIELD_V: case FS2D_FIELD_NEWV: return (size_t)n_v(s);
         case FS2D_FIELD_P: case FS2D_FIELD_PHI: case FS2D_FIELD_DENSITY: case FS2D_FIELD_CELL: return (size_t)n_c(s);
         case FS2D_FIELD_PARTICLE_POS: case FS2D_FIELD_PARTICLE_VEL: return (size_t)s->np * 2;
+        case FS2D_FIELD_PARTICLE_CELL: case FS2D_FIELD_PARTICLE_ORDER: return (size_t)s->np;
         default: return 0;
     }
 }
@@ -943,15 +943,24 @@ static float* fp32_field(fs2d_sim* s, int field) {
     }
 }
 
+static bool is_particle_field(int field) {
+    return field == FS2D_FIELD_PARTICLE_POS || field == FS2D_FIELD_PARTICLE_VEL ||
+           field == FS2D_FIELD_PARTICLE_CELL || field == FS2D_FIELD_PARTICLE_ORDER;
+}
+
 fs2d_status fs2d_get_field(fs2d_sim* s, int field, int dtype, void* host, size_t count) {
     if (!s || !host) return FS2D_ERR_ARG;
     const size_t n = fs2d_field_count(s, field);
-    if (n == 0 && field != FS2D_FIELD_PARTICLE_POS && field != FS2D_FIELD_PARTICLE_VEL)
-        return fail(s, FS2D_ERR_ARG, "unknown field %d", field);
+    if (n == 0 && !is_particle_field(field)) return fail(s, FS2D_ERR_ARG, "unknown field %d", field);
     if (count != n) return fail(s, FS2D_ERR_ARG, "field %d has %zu elements, buffer has %zu", field, n, count);
     if (!valid_dtype(field, dtype)) return fail(s, FS2D_ERR_ARG, "dtype %d invalid for field %d", dtype, field);
     HIPCHK(s, hipStreamSynchronize(s->stream));
     if (n == 0) return FS2D_OK;
+    if (field == FS2D_FIELD_PARTICLE_CELL || field == FS2D_FIELD_PARTICLE_ORDER) {
+        sort_particles(s);
+        TRY(d2h(s, host, field == FS2D_FIELD_PARTICLE_CELL ? s->pkey : s->sorted, n * sizeof(int)));
+        return FS2D_OK;
+    }
     if (field == FS2D_FIELD_CELL) {
         TRY(d2h(s, host, s->cell, n));
         return FS2D_OK;
@@ -993,8 +1002,9 @@ static bool particle_in_box(const fs2d_sim* s, double x, double y) {
 fs2d_status fs2d_set_field(fs2d_sim* s, int field, int dtype, const void* host, size_t count) {
     if (!s || !host) return FS2D_ERR_ARG;
     const size_t n = fs2d_field_count(s, field);
-    if (n == 0 && field != FS2D_FIELD_PARTICLE_POS && field != FS2D_FIELD_PARTICLE_VEL)
-        return fail(s, FS2D_ERR_ARG, "unknown field %d", field);
+    if (field == FS2D_FIELD_PARTICLE_CELL || field == FS2D_FIELD_PARTICLE_ORDER)
+        return fail(s, FS2D_ERR_ARG, "field %d is read-only", field);
+    if (n == 0 && !is_particle_field(field)) return fail(s, FS2D_ERR_ARG, "unknown field %d", field);
     if (count != n) return fail(s, FS2D_ERR_ARG, "field %d has %zu elements, buffer has %zu", field, n, count);
     if (!valid_dtype(field, dtype)) return fail(s, FS2D_ERR_ARG, "dtype %d invalid for field %d", dtype, field);
     HIPCHK(s, hipStreamSynchronize(s->stream));

@@ -22,8 +22,8 @@ EMPTY, FLUID, SOLID = 0, 1, 2
 (STAGE_INIT, STAGE_LEVEL_SET, STAGE_P2G, STAGE_SL, STAGE_GRAVITY, STAGE_SOLID,
  STAGE_PROJECT, STAGE_UPDATE_VELOCITY, STAGE_G2P, STAGE_ADVECT) = range(10)
 (F_U, F_V, F_NEWU, F_NEWV, F_P, F_PHI, F_DENSITY, F_CELL,
- F_PARTICLE_POS, F_PARTICLE_VEL) = range(10)
-F32, F64, U8 = 0, 1, 2
+ F_PARTICLE_POS, F_PARTICLE_VEL, F_PARTICLE_CELL, F_PARTICLE_ORDER) = range(12)
+F32, F64, U8, I32 = 0, 1, 2, 3
 FLAG_STATIC_CELLS, FLAG_DENSITY, FLAG_PARTICLE_CELLS, FLAG_NO_GHOST_PRESSURE = 1, 2, 4, 8
 
 
@@ -248,12 +248,16 @@ class Sim:
         n = self.L.fs2d_field_count(self.h, f)
         if f in (F_PARTICLE_POS, F_PARTICLE_VEL):
             return (n // 2, 2)
+        if f in (F_PARTICLE_CELL, F_PARTICLE_ORDER):
+            return (n,)
         return (self.ny, self.nx)
 
     def get(self, f, dtype=np.float32):
         n = self.L.fs2d_field_count(self.h, f)
         if f == F_CELL:
             dtype, code = np.uint8, U8
+        elif f in (F_PARTICLE_CELL, F_PARTICLE_ORDER):
+            dtype, code = np.int32, I32
         else:
             code = F64 if np.dtype(dtype) == np.float64 else F32
         out = np.empty(n, dtype=dtype)

@@ -79,10 +79,18 @@ typedef enum {
     FS2D_FIELD_DENSITY = 6,  /* passive density (no reference counterpart) */
     FS2D_FIELD_CELL = 7,     /* cell     (FluidSim2D.h:74), uint8 only */
     FS2D_FIELD_PARTICLE_POS = 8, /* particles    (FluidSim2D.h:75), 2 per particle */
-    FS2D_FIELD_PARTICLE_VEL = 9  /* particleVels (FluidSim2D.h:76), 2 per particle */
+    FS2D_FIELD_PARTICLE_VEL = 9, /* particleVels (FluidSim2D.h:76), 2 per particle */
+    /* Read-only, FS2D_I32, 1 per particle: each particle's cell key
+     * y_cell * size_x + x_cell, with x_cell = (int)(x / dx) as the reference's
+     * LevelSet::constructFromParticles computes it (FluidSim2D.cpp:766), and
+     * the particle indices sorted by that key (ties by index), the order in
+     * which P2G and the level set gather particles.  Reading either sorts
+     * the current positions if they changed since the last sort. */
+    FS2D_FIELD_PARTICLE_CELL = 10,
+    FS2D_FIELD_PARTICLE_ORDER = 11
 } fs2d_field;
 
-typedef enum { FS2D_F32 = 0, FS2D_F64 = 1, FS2D_U8 = 2 } fs2d_dtype;
+typedef enum { FS2D_F32 = 0, FS2D_F64 = 1, FS2D_U8 = 2, FS2D_I32 = 3 } fs2d_dtype;
 
 /* Switches.  STATIC_CELLS and DENSITY have no reference counterpart; the
  * other two select the reference's compile-time variants

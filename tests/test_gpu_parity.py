@@ -53,6 +53,18 @@ def push(g, o, fields):
             o.set(f, g.get(f).astype(np.float64))
 
 
+def check_particle_keys(g, dx):
+    """Bit-exact particle cell keys and sort order: the GPU's keys equal the
+    reference's rule x_cell = (int)(x / dx) in FP64 (FluidSim2D.cpp:766) on
+    the GPU's own positions, and its gather order is the stable sort of those
+    keys (ties by particle index)."""
+    pos = g.get(fs2d.F_PARTICLE_POS).astype(np.float64)
+    want = (pos[:, 1] / dx).astype(np.int64) * g.nx + (pos[:, 0] / dx).astype(np.int64)
+    keys = g.get(fs2d.F_PARTICLE_CELL)
+    np.testing.assert_array_equal(keys, want)
+    np.testing.assert_array_equal(g.get(fs2d.F_PARTICLE_ORDER), np.argsort(want, kind="stable"))
+
+
 def smooth_field(shape, rng, scale=1.0):
     from scipy.ndimage import gaussian_filter
     return (scale * gaussian_filter(rng.standard_normal(shape), 2.0, mode="nearest") * 4).astype(np.float32)
@@ -180,7 +192,7 @@ def test_projection_converged_and_velocity_update(mode):
     print("converged: gpu iters %d (res %.2e), oracle iters %d, p rel-L2 %.3e"
           % (sg["pcg_iters"], sg["pcg_residual"], o.stats()["last_iters"], ep))
     assert sg["pcg_residual"] <= 1e-6
-    assert ep < 2e-4
+    assert ep < REL
     # velocity update from the same pressure
     push(g, o, [fs2d.F_P])
     g.run_stage(fs2d.STAGE_UPDATE_VELOCITY)
@@ -244,11 +256,17 @@ def test_g2p_and_particle_advection():
     e = rel_l2(g.get(fs2d.F_PARTICLE_POS), o.get(O.F_POS))
     print("advect rel-L2 %.3e" % e)
     assert e < REL
-    # particle cell indices after the move are those of the oracle's positions
+    check_particle_keys(g, 1.0 / n)
+    # against the oracle's FP64 positions the cells may differ only where the
+    # two positions straddle a cell edge within FP32 accuracy
     pg = g.get(fs2d.F_PARTICLE_POS).astype(np.float64)
-    kg = (pg / (1.0 / n)).astype(int)
-    ko = (o.get(O.F_POS) / (1.0 / n)).astype(int)
-    assert np.mean(np.all(kg == ko, axis=1)) > 0.999
+    po = o.get(O.F_POS)
+    kg, ko = np.floor(pg * n).astype(int), np.floor(po * n).astype(int)
+    diff = np.any(kg != ko, axis=1)
+    near = np.any(np.abs(pg * n - np.round(pg * n)) < 1e-6 * n, axis=1)
+    print("cell indices differing from the oracle's: %d of %d (%d particles within 1e-6 of an edge)"
+          % (diff.sum(), len(pg), near.sum()))
+    assert not (diff & ~near).any()
 
 
 @pytest.mark.parametrize("mode", [fs2d.SEMILAGRANGIAN, fs2d.PICFLIP])
@@ -269,6 +287,7 @@ def test_teacher_forced_steps(mode):
         ep = rel_l2(g.get(fs2d.F_P), o.get(O.F_P))
         print("step %d: u %.2e v %.2e pos %.2e p %.2e" % (step, eu, ev, ex, ep))
         assert ex < REL and eu < REL and ev < REL and ep < REL
+        check_particle_keys(g, 1.0 / n)
 
 
 def test_golden_semilag_rows_on_gpu():

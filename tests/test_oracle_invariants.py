@@ -290,6 +290,81 @@ def test_p2g_uniform_velocity_gives_uniform_field():
     np.testing.assert_allclose(o.get(O.F_V), -0.3, rtol=1e-14)
 
 
+def tent_p2g(pts, vals, shape, offset):
+    """P2G by its definition: face (i, j) sits at grid position (i, j) + offset
+    (u faces at (i, j + 1/2), v faces at (i + 1/2, j)); a particle at grid
+    position (X, Y) gives it the weight max(0, 1 - |X - i - ox|) *
+    max(0, 1 - |Y - j - oy|).  Returns the weight sums and the normalised
+    values (0 where no weight arrives)."""
+    ny, nx = shape
+    J, I = np.mgrid[0:ny, 0:nx].astype(np.float64)
+    W = np.zeros(shape)
+    S = np.zeros(shape)
+    for (X, Y), val in zip(pts, vals):
+        w = (np.maximum(0.0, 1.0 - np.abs(X - I - offset[0])) *
+             np.maximum(0.0, 1.0 - np.abs(Y - J - offset[1])))
+        W += w
+        S += w * val
+    out = np.where(W > 0, S / np.where(W > 0, W, 1.0), 0.0)
+    return W, out
+
+
+def test_p2g_tent_weights_closed_form():
+    """transferVelocityToGrid (FluidSim2D.cpp:144-204, Array2D.h:361-376) on a
+    few particles, against the tent-kernel definition and the layered BFS
+    (pinned above) for the faces left unknown.  The particles are placed so
+    that a half-cell shift of either face offset, or swapped tent weights
+    (disp <-> 1 - disp), changes the known faces or their values:
+      a, b : overlapping stencils with different velocities (normalised mix);
+      c    : vx = 0 exactly, so its u faces hold 0.0 and are marked unknown
+             (the reference's == 0.0 rule, :192-201), while its v faces are known;
+      d    : on a u node (X = 6, Y - 1/2 = 7): one u face gets weight 1, its
+             neighbours weight 0 and stay undivided and unknown.
+    Valid particles lie in [dx, (N-1)dx) (seeding and clampPos), where the
+    reference's truncating (int) equals floor and its index clamps never
+    engage, so the tent kernel is the reference's rule there."""
+    n, dx = 10, 0.125  # dx a power of two: x / dx is exact
+    cells = O.dam_break(n, n)
+    o = O.Oracle(cells, k=0, dx=dx, mode=O.MODE_PIC)
+    grid_pos = np.array([[3.3, 4.8], [3.9, 5.15], [7.25, 2.6], [6.0, 7.5]])
+    vel = np.array([[1.5, -0.7], [-0.4, 2.2], [0.0, 0.9], [0.6, -1.1]])
+    o.set(O.F_POS, grid_pos * dx)
+    o.set(O.F_VEL, vel)
+    o.run_stage(O.ST_P2G)
+    for f, comp, shape, offset in ((O.F_U, 0, (n, n + 1), (0.0, 0.5)),
+                                   (O.F_V, 1, (n + 1, n), (0.5, 0.0))):
+        W, val = tent_p2g(grid_pos, vel[:, comp], shape, offset)
+        known = (W > 0) & (val != 0.0)
+        mask = np.where(known, 0, np.uint32(0xFFFFFFFF)).astype(np.uint32)
+        want, _ = layered_extrapolate(val, mask)
+        got = o.get(f)
+        np.testing.assert_allclose(got[known], val[known], rtol=1e-14, atol=0)
+        np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-15)
+    # the cases the comment names are present
+    Wu, vu = tent_p2g(grid_pos, vel[:, 0], (n, n + 1), (0.0, 0.5))
+    assert Wu[7, 6] == 1.0 and Wu[7, 7] == 0.0 and Wu[8, 6] == 0.0
+    assert ((Wu > 0) & (vu == 0.0)).sum() == 4      # particle c's u faces
+    Wv, vv = tent_p2g(grid_pos, vel[:, 1], (n + 1, n), (0.5, 0.0))
+    assert ((Wv > 0) & (vv != 0.0)).sum() > 4 * 3
+
+
+def test_p2g_tent_weights_reject_shifted_offsets():
+    """The closed form above separates the right face offsets from the
+    plausible slips: every half-cell shift and the swapped weights give a
+    different set of known faces or values for the same particles."""
+    grid_pos = np.array([[3.3, 4.8], [3.9, 5.15]])
+    vel = np.array([1.5, -0.4])
+    shape = (10, 11)
+    W0, v0 = tent_p2g(grid_pos, vel, shape, (0.0, 0.5))
+    for off in ((0.5, 0.5), (0.0, 0.0), (0.5, 0.0), (-0.5, 0.5), (0.0, 1.0)):
+        W1, v1 = tent_p2g(grid_pos, vel, shape, off)
+        assert not (np.array_equal(W0 > 0, W1 > 0) and np.allclose(v0, v1))
+    # swapped tent weights: mirror the fractional parts within each cell
+    mirrored = np.floor(grid_pos) + 1.0 - (grid_pos - np.floor(grid_pos))
+    W2, v2 = tent_p2g(mirrored, vel, shape, (0.0, 0.5))
+    assert not np.allclose(v0, v2)
+
+
 def test_g2p_bilinear_field_exact_and_flip_limit():
     n, dx = 16, 1.0 / 16
     cells = O.dam_break(n, n)
