@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1247,6 +1248,157 @@ const char* fs2d_status_string(int st) {
         case FS2D_ERR_HANG: return "FS2D_ERR_HANG";
         default: return "unknown";
     }
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// applyProjection without allocation (fs2d_project_noalloc).
+//
+// Every array the projection needs beyond its inputs and output lives in
+// `scratch`, a file-scope device array: one copy per GPU, held from the
+// library's load until the process ends.  Host code never names `scratch`
+// itself (in host code the name is a host-side stand-in, not a device
+// address): the device address comes from hipGetSymbolAddress, once per
+// device with that device current, and only that address reaches kernels.
+// Calls share their GPU's copy, so calls that overlap on one GPU race; each
+// call clears what must start at zero (the strip ticket, the hand-off words,
+// the error flag) with fill kernels before its first sweep.
+// ===========================================================================
+namespace {
+
+constexpr long kNoallocEntries = 1L << 19;  // SK4 entries (512 x 512 cells need 294,912)
+constexpr int kNoallocMaxStrips = 128;     // S * T4 <= kNoallocEntries / 256 and T4 >= 24 give S <= 85
+// hand-off words of S strips: S * (3 * (nx + 192) + 64) with nx <= 4 T4 - 64
+constexpr long kNoallocHandWords = 12L * (kNoallocEntries / 256) + 448L * kNoallocMaxStrips;
+constexpr int kNoallocFloatArrays = 13;  // r z q w s0 s1 pk Adiag P fx fy gx gy
+
+constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kNaFloats = na_align(sizeof(float) * kNoallocEntries);
+constexpr size_t kNaCpl = na_align(kNoallocEntries);
+constexpr size_t kNaSc = na_align(sizeof(PcgScalars));
+constexpr size_t kNaQq = na_align(sizeof(double) * 2 * kNoallocMaxStrips * kLoaders);
+constexpr size_t kNaPart = na_align(sizeof(double) * kMaxBlocks);
+constexpr size_t kNaWord = 256;
+constexpr size_t kNaHand = na_align(sizeof(unsigned long long) * kNoallocHandWords);
+constexpr size_t kNoallocBytes =
+    kNoallocFloatArrays * kNaFloats + kNaCpl + kNaSc + kNaQq + 2 * kNaPart + 2 * kNaWord + kNaHand;
+
+__device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+unsigned char* g_noalloc_ptr[kNoallocMaxDevices] = {};
+
+// Failures are explained by fs2d_last_error(NULL), like those of fs2d_create.
+fs2d_status noalloc_fail(fs2d_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_create_error = buf;
+    return st;
+}
+
+// The device address of `scratch` on the current device (looked up once).
+fs2d_status noalloc_scratch(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= kNoallocMaxDevices) return noalloc_fail(FS2D_ERR_DEVICE, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return noalloc_fail(FS2D_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_noalloc_ptr[dev] = (unsigned char*)p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return FS2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+fs2d_status fs2d_project_noalloc(const fs2d_projection_args* a, int* iters, double* residual) {
+    if (!a || !a->cell || !a->phi || !a->u || !a->v || !a->p)
+        return noalloc_fail(FS2D_ERR_ARG, "fs2d_project_noalloc: NULL argument");
+    if (a->size_x < 3 || a->size_y < 3 || a->size_x > 16384 || a->size_y > 16384)
+        return noalloc_fail(FS2D_ERR_ARG, "fs2d_project_noalloc: size %dx%d outside [3, 16384]", a->size_x,
+                            a->size_y);
+    const SkGeom geom = make_geom(a->size_x, a->size_y);
+    const long long entries = geom.size();
+    const long hand = (long)geom.S * (long)sk_handoff_stride(a->size_x, 3);
+    if (entries > kNoallocEntries || geom.S > kNoallocMaxStrips || hand > kNoallocHandWords)
+        return noalloc_fail(FS2D_ERR_ARG,
+                            "fs2d_project_noalloc: a %d x %d grid needs %lld solver entries; the static scratch "
+                            "holds %ld (512 x 512 cells fit)",
+                            a->size_x, a->size_y, entries, kNoallocEntries);
+    if (!(a->dx > 0) || !(a->rho > 0) || !(a->dt >= 0) || !(a->tau >= 0) || !(a->sigma >= 0) ||
+        !(a->pcg_tol >= 0) || a->pcg_max_iters < 0 || (a->flags & ~FS2D_FLAG_NO_GHOST_PRESSURE))
+        return noalloc_fail(FS2D_ERR_ARG, "fs2d_project_noalloc: invalid solver parameters or flags");
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return noalloc_fail(FS2D_ERR_DEVICE, "hipGetDevice: %s", hipGetErrorString(e));
+    unsigned char* base = nullptr;
+    fs2d_status st = noalloc_scratch(dev, &base);
+    if (st != FS2D_OK) return st;
+
+    // A handle on the host stack over the caller's arrays and the scratch.
+    fs2d_sim t;
+    t.nx = a->size_x;
+    t.ny = a->size_y;
+    t.dt = a->dt;
+    t.dx = a->dx;
+    t.rho = a->rho;
+    t.tau = a->tau;
+    t.sigma = a->sigma;
+    t.tol = a->pcg_tol;
+    t.max_iters = a->pcg_max_iters;
+    t.flags = a->flags;
+    t.stream = (hipStream_t)a->stream;
+    t.device = dev;
+    t.geom = geom;
+    t.cell = const_cast<uint8_t*>(a->cell);
+    t.phi = const_cast<double*>(a->phi);
+    t.u = const_cast<float*>(a->u);
+    t.v = const_cast<float*>(a->v);
+    unsigned char* q = base;
+    float** fl[kNoallocFloatArrays] = {&t.r, &t.z, &t.q, &t.w, &t.s[0], &t.s[1], &t.pk,
+                                       &t.Adiag, &t.P, &t.fx, &t.fy, &t.gx_, &t.gy_};
+    for (float** f : fl) {
+        *f = (float*)q;
+        q += kNaFloats;
+    }
+    t.cpl = q;                                   q += kNaCpl;
+    t.sc = (PcgScalars*)q;                       q += kNaSc;
+    t.qq = (double*)q;                           q += kNaQq;
+    t.ws_part = (double*)q;                      q += kNaPart;
+    t.rmax_part = (double*)q;                    q += kNaPart;
+    t.sk_ticket = (unsigned long long*)q;        q += kNaWord;
+    t.werr = (int*)q;                            q += kNaWord;
+    t.sk_handoff = (unsigned long long*)q;
+    PcgScalars h_sc{};
+    t.h_sc = &h_sc;  // pageable: read back with a synchronising copy
+
+    // A fresh start, as fs2d_create's zeroed buffers give a handle.
+    fill(&t, t.sk_ticket, 0ull, 1);
+    fill(&t, t.sk_handoff, 0ull, (size_t)hand);
+    fill(&t, t.werr, 0, 1);
+    assemble(&t);
+    st = pcg_solve(&t, a->p);
+    int h_err = 0;
+    e = hipMemcpyAsync(&h_err, t.werr, sizeof(int), hipMemcpyDeviceToHost, t.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t.stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) return noalloc_fail(FS2D_ERR_DEVICE, "fs2d_project_noalloc: %s", hipGetErrorString(e));
+    if (st < 0) return noalloc_fail(st, "fs2d_project_noalloc: %s", t.err.c_str());
+    if (h_err) return noalloc_fail(FS2D_ERR_HANG, "fs2d_project_noalloc: a wavefront hand-off wait expired");
+    if (iters) *iters = h_sc.iter;
+    if (residual) *residual = h_sc.rhs_norm > 0 ? h_sc.r_norm / h_sc.rhs_norm : 0.0;
+    if (st == FS2D_WARN_MAXITER)
+        noalloc_fail(st, "PCG: maximum iteration limit (%d) exceeded", t.max_iters);
+    return st;
 }
 
 }  // extern "C"

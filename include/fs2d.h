@@ -215,6 +215,45 @@ const char* fs2d_last_error(const fs2d_sim* sim);
 const char* fs2d_status_string(int status);
 
 /*
+ * applyProjection without allocation.  No reference counterpart: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It runs the same
+ * kernels as fs2d_run_stage(sim, FS2D_STAGE_APPLY_PROJECTION) (ghost-fluid
+ * assembly, MIC(0) factor, PCG; src/FluidSim2D.cpp:252-467) on the caller's
+ * device arrays and gives the same pressure bit for bit.  Every other array
+ * lives in a static device buffer of about 28 MB per GPU, held from the
+ * library's load until the process ends; grids whose solver layout has at most
+ * 2^19 entries fit: ceil(size_y / 64) * 64 * ceil((size_x + 64) / 32) * 32
+ * <= 524288 (512 x 512 does, 700 x 700 does not).  Larger grids are rejected
+ * with FS2D_ERR_ARG before any device work.  Calls on one GPU share that
+ * buffer: calls that overlap on one GPU (two threads, or two streams) race.
+ * The call returns after the solve has finished on `stream`; the host reads
+ * the convergence flag between chunks of iterations with synchronising
+ * copies into its own stack.  Failures are explained by fs2d_last_error(NULL).
+ */
+typedef struct {
+    int size_x;
+    int size_y;
+    double dt;
+    double dx;
+    double rho;
+    double tau;
+    double sigma;
+    double pcg_tol;
+    int pcg_max_iters;
+    int flags;              /* 0 or FS2D_FLAG_NO_GHOST_PRESSURE */
+    const uint8_t* cell;    /* device, size_x * size_y flags (SOLID border) */
+    const double* phi;      /* device, size_x * size_y level set */
+    const float* u;         /* device, (size_x + 1) * size_y */
+    const float* v;         /* device, size_x * (size_y + 1) */
+    float* p;               /* device, size_x * size_y: the pressure out */
+    void* stream;           /* hipStream_t to run on; NULL: the default stream */
+} fs2d_projection_args;
+/* Iterations and |r|_inf / |rhs|_inf at exit go to *iters and *residual when
+ * they are not NULL; FS2D_WARN_MAXITER when pcg_max_iters was reached. */
+fs2d_status fs2d_project_noalloc(const fs2d_projection_args* args, int* iters, double* residual);
+
+/*
  * Ensembles: many independent simulations of different sizes stepped
  * together.  No reference counterpart (the reference steps one FluidSim2D per
  * runFrame, src/FluidSim2D.cpp:94-138); each member steps exactly as
