@@ -25,13 +25,24 @@ the step, so the timed region times the dominant class only; the dominant
 class and `kernel_ms_per_step` come from 2 steps with every class timed,
 run after the warmup and before the timed region.
 
+`graph`: the same 2048^2 steps taken by a second simulation in lockstep
+with the first (same state, bit-identical results) whose fs2d_step replays
+captured HIP graphs (fs2d_set_graph), timed without events.
+
+Both simulations, and both ensemble batches, run the launch shapes the
+autotuner picks (fs2d_autotune / fs2d_batch_autotune, which change no
+result), read from `--tune-cache` when it holds the problem's key; the choice
+is reported as `tuning`.
+
 `ensemble` (the batched half of the metric): 256 independent PIC/FLIP dam
 breaks, side uniform in {64, 96, ..., 512} (seed 2), fluid block width and
 height uniform in [0.2, 0.7], 4 particles per fluid cell, FLIP blend 0.95,
 dt 0.002, PCG to 1e-6 with converged members masked, stepped as one batch
-(fs2d_batch_step) and, for comparison, by looping fs2d_step over the same
-256 simulations.  Value = total cells x steps / wall time, with no events;
-`kernel_ms_per_step` comes from 2 further steps with every class timed.
+whose step replays captured HIP graphs (fs2d_batch_set_graph; the value),
+by uncaptured batch steps of a second batch in lockstep (`no_graph_*`) and,
+for comparison, by looping fs2d_step over the same 256 simulations.  Value =
+total cells x steps / wall time, with no events; `kernel_ms_per_step` comes
+from 2 further uncaptured steps with every class timed.
 
 `adjoint`: the differentiable rollout, 256 x 256 all-fluid smoke, 64 steps,
 dt 0.01, velocity seeded like the 2048^2 workload (seed 3), loss = L2 of the
@@ -211,19 +222,34 @@ def cpu_baseline(state, cells, n, dt, tol, tau, threads, iters_gpu, host, caps=(
                                                caps[1], iters_gpu)}
 
 
-def ensemble_leg(count, steps, warmup, loop_steps):
-    """The batched ensemble and the loop over the same simulations."""
+def ensemble_leg(count, steps, warmup, loop_steps, tune_cache=None):
+    """The batched ensemble: stepped by replaying captured graphs (the
+    value) and by uncaptured batch steps, two batches in lockstep from the
+    same state (their members stay bit-identical); and the loop over the same
+    simulations."""
     import fs2d
     from fs2d import scenes
     members = scenes.ensemble_members(count)
     cells = int(sum(c.shape[0] * c.shape[1] for c, _ in members))
     batch = fs2d.Batch(members)
+    gbatch = fs2d.Batch(members)
+    gbatch.set_graph(True)
+    batch.step()
+    gbatch.step()
+    tune = batch.autotune(iters=24, cache=tune_cache)
+    gbatch.set_tuning(tune["pcg_grid"], tune["sweep_prefetch"])
     for _ in range(warmup):
         batch.step()
+        gbatch.step()
     t0 = time.perf_counter()
     for _ in range(steps):
-        batch.step()  # returns after the step's work (fs2d_batch_step synchronizes)
+        gbatch.step()  # returns after the step's work (fs2d_batch_step synchronizes)
+    tg = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.step()
     tb = (time.perf_counter() - t0) / steps
+    gbatch.close()
     # per-class breakdown from further steps (events around every launch cost time)
     batch.reset_kernel_times()
     batch.set_timing(True)
@@ -247,10 +273,12 @@ def ensemble_leg(count, steps, warmup, loop_steps):
         for s in singles:
             s.close()
     return {
-        "metric": "cell-steps/s of the batched ensemble", "value": cells / tb, "unit": "cell-steps/s",
-        "ms_per_step": tb * 1e3, "steps": steps, "warmup": warmup,
+        "metric": "cell-steps/s of the batched ensemble", "value": cells / tg, "unit": "cell-steps/s",
+        "ms_per_step": tg * 1e3, "steps": steps, "warmup": warmup, "graph": True,
+        "no_graph_value": cells / tb, "no_graph_ms_per_step": tb * 1e3,
+        "tuning": tune,
         "loop_value": cells / tl if tl else None, "loop_ms_per_step": tl * 1e3 if tl else None,
-        "loop_steps": loop_steps, "speedup_vs_loop": tl / tb if tl else None,
+        "loop_steps": loop_steps, "speedup_vs_loop": tl / tg if tl else None,
         "config": {"workload": "%d PIC/FLIP dam breaks, sides 64..512 (seed 2), block fractions U[0.2,0.7], "
                                "k=2, FLIP 0.95, dt 0.002, PCG 1e-6 masked per member" % count,
                    "members": count, "cells": cells, "pcg_iters_max": int(max(its)),
@@ -392,6 +420,8 @@ def main():
     ap.add_argument("--loop-steps", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-grid3d", action="store_true", help="skip the 3D grid leg")
+    ap.add_argument("--tune-cache", default=os.path.join(ROOT, "fluid-sim_amd", "tune_cache.txt"),
+                    help="autotuner record (read, or appended to after timing the candidates)")
     ap.add_argument("--only", choices=["ensemble", "adjoint", "grid3d"],
                     help="run one leg alone and print its object (profiling runs)")
     args = ap.parse_args()
@@ -408,7 +438,8 @@ def main():
     os.environ.setdefault("OMP_PLACES", "cores")
     import fs2d
     if args.only == "ensemble":
-        print(json.dumps(ensemble_leg(args.ensemble_members, args.ensemble_steps, args.warmup, args.loop_steps)))
+        print(json.dumps(ensemble_leg(args.ensemble_members, args.ensemble_steps, args.warmup, args.loop_steps,
+                                      args.tune_cache)))
         return
     if args.only == "adjoint":
         print(json.dumps(adjoint_leg()))
@@ -419,14 +450,23 @@ def main():
     n = args.n
     sim, cells, dt = build_gpu_sim(n, args.tol, args.tau, args.max_iters)
     state = [sim.get(f) for f in (fs2d.F_U, fs2d.F_V, fs2d.F_DENSITY)]
+    # the same workload stepped by replaying captured graphs, in lockstep
+    gsim, _, _ = build_gpu_sim(n, args.tol, args.tau, args.max_iters)
+    gsim.set_graph(True)
+    sim.step()
+    gsim.step()
+    tune = sim.autotune(iters=24, cache=args.tune_cache)
+    gsim.set_tuning(pcg_grid=tune["pcg_grid"])
     for _ in range(args.warmup):
         sim.step()
+        gsim.step()
     # per-class breakdown and the dominant class, with every class timed
     sim.synchronize()
     sim.reset_kernel_times()
     sim.set_timing(True)
     for _ in range(PROFILE_STEPS):
         sim.step()
+        gsim.step()
     kt_all = sim.kernel_times()
     sim.set_timing(False)
     dom = max((k for k in kt_all if k in BYTES_PER_CELL), key=lambda k: kt_all[k][0])
@@ -446,6 +486,15 @@ def main():
     elapsed = time.perf_counter() - t0
     kt = sim.kernel_times()
     sim.set_timing(False)
+    gsim.synchronize()
+    t0 = time.perf_counter()
+    giters = []
+    for _ in range(args.steps):
+        gsim.step()
+        giters.append(gsim.stats()["pcg_iters"])
+    gsim.synchronize()
+    g_elapsed = time.perf_counter() - t0
+    gsim.close()
 
     value = n * n * args.steps / elapsed
     tot_ms, launches = kt[dom]
@@ -473,7 +522,11 @@ def main():
             "grid": [n, n], "pcg_tol": args.tol, "tau": args.tau, "cfl": 1.0, "dt": dt,
             "pcg_iters_mean": float(np.mean(iters)), "pcg_iters_max": int(max(iters)),
             "pcg_max_iters": args.max_iters, "pcg_capped_steps": capped,
+            "tuning": tune,
         },
+        "graph": {"value": n * n * args.steps / g_elapsed, "ms_per_step": g_elapsed / args.steps * 1e3,
+                  "pcg_iters_mean": float(np.mean(giters)),
+                  "note": "the same steps replayed from captured HIP graphs (fs2d_set_graph), no events"},
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, n),
@@ -484,7 +537,8 @@ def main():
     }
     sim.close()
     if not args.no_ensemble:
-        out["ensemble"] = ensemble_leg(args.ensemble_members, args.ensemble_steps, 2, args.loop_steps)
+        out["ensemble"] = ensemble_leg(args.ensemble_members, args.ensemble_steps, 2, args.loop_steps,
+                                       args.tune_cache)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()
     if not args.no_grid3d:

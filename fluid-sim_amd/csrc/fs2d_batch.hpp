@@ -17,6 +17,8 @@
 // simulations: their workgroups return at once.
 #pragma once
 
+#include <climits>
+
 #include "fs2d_kernels.hpp"
 
 namespace fs2d {
@@ -123,9 +125,11 @@ __global__ __launch_bounds__(kBlock) void b_gravity_kernel(const InstBufs* __res
                  (float)(I.dt * I.gy), vb, nvb);
 }
 
-// PcgScalars of every simulation: zero, tolerance set (one workgroup each).
-__global__ void b_pcg_init_kernel(const InstBufs* __restrict__ ins, int n, double tol) {
+// PcgScalars of every simulation: zero, tolerance set (one workgroup each);
+// no member is known to be done.
+__global__ void b_pcg_init_kernel(const InstBufs* __restrict__ ins, int n, double tol, int* all_done_since) {
     const int b = blockIdx.x;
+    if (b == 0 && threadIdx.x == 0) *all_done_since = INT_MAX;
     if (b >= n || threadIdx.x != 0) return;
     PcgScalars z{};
     z.tol = tol;
@@ -141,39 +145,148 @@ __global__ __launch_bounds__(kBlock) void b_assemble_kernel(const InstBufs* __re
                      I.rmax_part, (I.flags & 8) ? 0 : 1 /* FS2D_FLAG_NO_GHOST_PRESSURE */, vb, nvb);
 }
 
-// Iteration h of the PCG loop: s double-buffered by parity, sigma_h and
-// sigma_{h-1} in the two halves of qq.
-__global__ __launch_bounds__(kBlock) void b_spmv_kernel(const InstBufs* __restrict__ ins, BlockMap bm, int h) {
-    int b, vb, nvb;
-    if (!bmap(bm, b, vb, nvb)) return;
-    const InstBufs& I = ins[b];
-    const double* qq0 = I.qq;
-    const double* qq1 = I.qq + (size_t)I.geom.S * kLoaders;
-    const float sA = (float)(I.dt / (I.rho * I.dx * I.dx));
-    pcg_spmv_sk_body(I.geom, I.Adiag, I.cpl, I.z, (h & 1) ? I.s1 : I.s0, (h & 1) ? I.s0 : I.s1, I.w, sA, I.sc,
-                     (h & 1) ? qq1 : qq0, (h & 1) ? qq0 : qq1, h == 0 ? 1 : 0, I.ws_part, vb, nvb);
+// The members still iterating and their virtual workgroups, packed: member b
+// owns virtual workgroups first[b] .. first[b] + count[b] - 1 of the
+// iteration's SpMV and axpy loops, each range starting at a multiple of 8 (so
+// vb % 8 stays the XCD), and map[k] names the member of virtual workgroup k
+// (-1 for padding).  Built once per iteration, from the members' done flags,
+// which only the forward solve's stopping test and the factor write, so the
+// SpMV and the axpy of an iteration share it.  Members are packed in `order`,
+// heaviest virtual workgroups first: workgroup g of a loop runs k = g, g + G,
+// g + 2 G, ..., and each round of G consecutive entries then spans members of
+// similar weight.
+struct PcgPlanBufs {
+    int* map;    // [sum of aligned counts]
+    int* first;  // [n]
+    int* total;  // [1]
+};
+__device__ __forceinline__ int block_excl_scan(int v, int& total) {
+    __shared__ int wsum[kBlock / 64];
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if ((threadIdx.x & 63) >= o) x += y;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = x;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+#pragma unroll
+    for (int k = 0; k < kBlock / 64; k++) {
+        if (k < (int)(threadIdx.x >> 6)) before += wsum[k];
+        total += wsum[k];
+    }
+    return before + x - v;
 }
-// w.s of every member still iterating, summed once per member in the order
-// block_sum_bcast sums it (one wave per member), so that the axpy's workgroups
-// need not each re-read all of the member's partials.  `nparts`: the
-// member's SpMV workgroup count.
-__global__ __launch_bounds__(64) void b_ws_reduce_kernel(const InstBufs* __restrict__ ins, const int* members,
-                                                         int n, const int* nparts) {
-    if ((int)blockIdx.x >= n) return;
-    const int b = members[blockIdx.x];
+// n workgroups: every one forms the prefix sums of the members' aligned counts
+// (dynamic LDS: n ints), workgroup i writes the map range of member order[i],
+// and workgroup 0 the total.  When no member iterates it records h in
+// all_done_since: the solves of iteration h and later then skip their strips
+// at once.
+__global__ __launch_bounds__(kBlock) void b_pcg_plan_kernel(const InstBufs* __restrict__ ins, int n,
+                                                            const int* __restrict__ count,
+                                                            const int* __restrict__ order, PcgPlanBufs P, int h,
+                                                            int* all_done_since) {
+    extern __shared__ int vals[];  // aligned count of order[i], 0 if done
+    __shared__ int my_first;
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+        const int b = order[i];
+        vals[i] = ins[b].sc->done ? 0 : (count[b] + 7) & ~7;
+    }
+    __syncthreads();
+    const int c = (n + kBlock - 1) / kBlock;
+    const int i0 = min((int)threadIdx.x * c, n), i1 = min(i0 + c, n);
+    int sum = 0;
+    for (int i = i0; i < i1; i++) sum += vals[i];
+    int total;
+    int ps = block_excl_scan(sum, total);
+    for (int i = i0; i < i1; i++) {
+        if (i == (int)blockIdx.x) my_first = ps;
+        ps += vals[i];
+    }
+    __syncthreads();
+    const int i = blockIdx.x;
+    const int b = order[i];
+    const int v = vals[i];
+    if (v) {
+        const int f = my_first, cnt = count[b];
+        for (int k = threadIdx.x; k < v; k += kBlock) P.map[f + k] = k < cnt ? b : -1;
+        if (threadIdx.x == 0) P.first[b] = f;
+    }
+    if (i == 0 && threadIdx.x == 0) {
+        *P.total = total;
+        if (total == 0 && *all_done_since > h) *all_done_since = h;
+    }
+}
+
+// Iteration h of the PCG loop over the members still iterating (one
+// persistent launch of any size: workgroup g runs virtual workgroups g, g +
+// gridDim.x, ... of the iteration's plan; s double-buffered by parity,
+// sigma_h and sigma_{h-1} in the two halves of qq).  Each member's virtual
+// workgroups compute what its single launch computes, so members step
+// bit-identically to single simulations.
+__global__ __launch_bounds__(kBlock) void b_spmv_kernel(const InstBufs* __restrict__ ins,
+                                                        const int* __restrict__ count, PcgPlanBufs P, int h) {
+    const int total = *P.total;
+    int cached = -1;
+    SpmvScalars S{};
+    int b_next = (int)blockIdx.x < total ? P.map[blockIdx.x] : -1;
+    for (int k = blockIdx.x; k < total; k += gridDim.x) {
+        const int b = b_next;
+        if (k + (int)gridDim.x < total) b_next = P.map[k + gridDim.x];
+        if (b < 0) continue;  // padding to a multiple of 8
+        const int vb = k - P.first[b], nvb = count[b];
+        const InstBufs& I = ins[b];
+        const float* s_old = (h & 1) ? I.s1 : I.s0;
+        float* s_new = (h & 1) ? I.s0 : I.s1;
+        const float sA = (float)(I.dt / (I.rho * I.dx * I.dx));
+        const SpmvPre pre = spmv_pre(I.geom, I.Adiag, I.cpl, I.z, s_old, vb, nvb);
+        if (b != cached) {
+            const double* qq0 = I.qq;
+            const double* qq1 = I.qq + (size_t)I.geom.S * kLoaders;
+            S = spmv_scalars(I.geom, (h & 1) ? qq1 : qq0, (h & 1) ? qq0 : qq1, h == 0 ? 1 : 0);
+            cached = b;
+        }
+        if (vb == 0 && threadIdx.x == 0) I.sc->sigma = S.sig;
+        spmv_core(pre, I.geom, I.Adiag, I.cpl, I.z, s_old, s_new, I.w, sA, S.beta, I.ws_part, vb);
+    }
+}
+// w.s of every member still iterating, summed once per member (one wave per
+// member) in the order the single launch's axpy sums it, so that the axpy's
+// workgroups need not each re-read all of the member's partials.  `nparts`:
+// the member's SpMV virtual workgroup count.
+__global__ __launch_bounds__(64) void b_ws_reduce_kernel(const InstBufs* __restrict__ ins, int n,
+                                                         const int* nparts) {
+    const int b = blockIdx.x;
+    if (b >= n) return;
     const InstBufs& I = ins[b];
     if (I.sc->done) return;
     const double t = wave_sum_of(I.ws_part, nparts[b]);
     if (threadIdx.x == 0) I.sc->ws = t;
 }
-__global__ __launch_bounds__(kBlock) void b_axpy_kernel(const InstBufs* __restrict__ ins, BlockMap bm, int h) {
-    int b, vb, nvb;
-    if (!bmap(bm, b, vb, nvb)) return;
-    const InstBufs& I = ins[b];
-    (void)h;
-    // sigma (written by the SpMV) and w.s come reduced in the member's scalars
-    pcg_axpy_sk_body(I.geom, I.pk, I.r, (h & 1) ? I.s0 : I.s1, I.w, I.sc, nullptr, nullptr, -1, I.rmax_part, vb,
-                     nvb);
+__global__ __launch_bounds__(kBlock) void b_axpy_kernel(const InstBufs* __restrict__ ins,
+                                                        const int* __restrict__ count, PcgPlanBufs P, int h) {
+    const int total = *P.total;
+    int b_next = (int)blockIdx.x < total ? P.map[blockIdx.x] : -1;
+    for (int k = blockIdx.x; k < total; k += gridDim.x) {
+        const int b = b_next;
+        if (k + (int)gridDim.x < total) b_next = P.map[k + gridDim.x];
+        if (b < 0) continue;
+        const int vb = k - P.first[b], nvb = count[b];
+        const InstBufs& I = ins[b];
+        float4* p4 = (float4*)I.pk;
+        float4* r4 = (float4*)I.r;
+        const float4* s4 = (const float4*)((h & 1) ? I.s0 : I.s1);
+        const float4* w4 = (const float4*)I.w;
+        const XcdLoop it = xcd_loop(I.geom.size() / 4, vb, nvb);
+        AxpyBuf Bf;
+        Bf.load(it, it.e, p4, r4, s4, w4);
+        // sigma (written by the SpMV) and w.s come reduced in the member's scalars
+        const float a = (float)(I.sc->sigma / I.sc->ws);
+        axpy_core(Bf, it, p4, r4, s4, w4, a, I.rmax_part, vb);
+    }
 }
 __global__ __launch_bounds__(kBlock) void b_sk_to_rowmajor_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
     int b, vb, nvb;
@@ -470,10 +583,20 @@ struct BwdMaker {
 // so two workgroups fit a CU and a launch over a thousand strips needs half
 // the rounds of residency.  The ring depth changes no arithmetic.
 constexpr int kBatchRing = 4;
-template <class Op, bool BACKWARD, class Maker>
+// PREFETCH: loaders issue their first loads before the skip test (which a
+// skipped strip then wastes).  all_done_since (PCG solves): when every member
+// had converged before iteration h, the first workgroup advances the ticket
+// by the launch's strip count and every workgroup returns, so the epochs of
+// later launches stay as if every strip had taken its ticket.
+template <class Op, bool BACKWARD, class Maker, bool PREFETCH = true>
 __global__ __launch_bounds__(kSweepThreads) void b_sweep_kernel(const InstBufs* __restrict__ ins, StripMap sm,
-                                                                unsigned long long* ticket, int* err, Maker mk) {
+                                                                unsigned long long* ticket, int* err, Maker mk,
+                                                                const int* all_done_since = nullptr, int h = 0) {
     __shared__ __align__(16) SweepLds<Op, kBatchRing> L;
+    if (all_done_since && *all_done_since <= h) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(ticket, (unsigned long long)sm.total);
+        return;
+    }
     const unsigned long long t = sweep_ticket(L, ticket);
     const int gs = (int)(t % (unsigned long long)sm.total);
     const unsigned epoch = (unsigned)(t / (unsigned long long)sm.total) + 1u;
@@ -481,7 +604,7 @@ __global__ __launch_bounds__(kSweepThreads) void b_sweep_kernel(const InstBufs* 
     const InstBufs& I = ins[b];
     Op op = mk(I, b);
     const SkCtl c{ticket, I.sk_handoff, err};
-    sweep_run<Op, BACKWARD>(op, I.geom, c, gs - sm.first[b], epoch, L);
+    sweep_run<Op, BACKWARD, PREFETCH>(op, I.geom, c, gs - sm.first[b], epoch, L);
 }
 
 }  // namespace fs2d

@@ -28,18 +28,34 @@ struct fs2d_batch {
     unsigned long long* sk_ticket = nullptr;
     int* d_status = nullptr;
     int* h_status = nullptr;  // pinned: per member done, iterations, NaN flag, CFL warning
-    // SpMV / axpy launches over the members still iterating (rebuilt between
-    // chunks of PCG iterations); the sweeps keep the full strip lists, since
-    // their epochs count launches of a fixed strip total.
-    std::vector<int> h_act;
-    int* d_act = nullptr;
-    size_t act_cap = 0;
-    BlockMap m_act{};
-    const int* act_members = nullptr;
-    int g_act = 0, n_act = 0;
+    // The PCG loop's SpMV and axpy are persistent launches of pcg_grid
+    // workgroups over the members still iterating (a plan formed once per
+    // iteration from the members' done flags, b_pcg_plan_kernel, with
+    // plan_lds bytes of LDS); the sweeps keep the full strip lists, since
+    // their epochs count launches of a fixed strip total.  all_done_since:
+    // the first iteration at which no member iterated (INT_MAX while some do).
+    int pcg_grid = 2048;
+    bool tuned = false;  // pcg_grid set by fs2d_batch_set_tuning / autotune
+    size_t plan_lds = 0;
+    PcgPlanBufs plan{};
+    bool sweep_prefetch = true;
+    int* all_done_since = nullptr;
+    const int* pcg_order = nullptr;  // members by SpMV/axpy virtual-workgroup weight, heaviest first (in d_maps)
     int* h_err = nullptr;
     bool sorted_valid = false;
     int last_iters_hint = 8;
+    // Captured steps (fs2d_batch_set_graph), as for a single handle.  Kernel
+    // arguments are the descriptors' and maps' device addresses, so a capture
+    // stays valid until the maps are rebuilt (a member's particle count
+    // changed) or a launch parameter changes.
+    bool use_graph = false;
+    int graph_budget = 0;
+    struct Captured {
+        const int* maps;
+        int budget, pcg_grid, prefetch;
+        hipGraphExec_t pre = nullptr, post = nullptr;
+    };
+    std::vector<Captured> graphs;
     std::vector<void*> allocs;
     std::string err;
 
@@ -187,6 +203,15 @@ fs2d_status build_maps(fs2d_batch* B) {
     }
     const size_t strip_first = H.size();
     H.insert(H.end(), sfirst.begin(), sfirst.end());
+    // members for the PCG plan: entries per SpMV/axpy virtual workgroup,
+    // descending (ties by index)
+    std::vector<int> porder(n);
+    for (int b = 0; b < n; b++) porder[b] = b;
+    std::stable_sort(porder.begin(), porder.end(), [&](int a, int b) {
+        return B->sims[a]->geom.size() * c_pcg[b] > B->sims[b]->geom.size() * c_pcg[a];
+    });
+    const size_t order_at = H.size();
+    H.insert(H.end(), porder.begin(), porder.end());
     int* maps = nullptr;
     TRY(balloc(B, &maps, H.size()));
     BHIPCHK(B, hipMemcpyAsync(maps, H.data(), H.size() * sizeof(int), hipMemcpyHostToDevice, B->stream));
@@ -197,30 +222,15 @@ fs2d_status build_maps(fs2d_batch* B) {
     B->m_diag = mk(o_diag); B->m_sk = mk(o_sk); B->m_tiles = mk(o_tiles); B->m_one = mk(o_one);
     B->m_xrows = mk(o_xrows); B->m_xcols = mk(o_xcols); B->m_pcg = mk(o_pcg);
     B->strips = StripMap{maps + strip_inst, maps + strip_first, total};
-    return FS2D_OK;
-}
-
-// Launch map of the SpMV / axpy over the members with active[b] != 0, and
-// the list of those members (for the w.s reduction).
-fs2d_status build_active(fs2d_batch* B, const std::vector<int>& active) {
-    const int n = B->n;
-    std::vector<int> cnt(n);
-    for (int b = 0; b < n; b++) cnt[b] = active[b] ? pcg_blocks(B->sims[b]) : 0;
-    B->h_act.clear();
-    MapOff o;
-    B->g_act = add_map(B->h_act, cnt, o);
-    const size_t mem = B->h_act.size();
-    for (int b = 0; b < n; b++)
-        if (active[b]) B->h_act.push_back(b);
-    B->n_act = (int)(B->h_act.size() - mem);
-    if (B->h_act.size() > B->act_cap) {
-        B->act_cap = B->h_act.size() + B->h_act.size() / 2;
-        TRY(balloc(B, &B->d_act, B->act_cap));
-    }
-    BHIPCHK(B, hipMemcpyAsync(B->d_act, B->h_act.data(), B->h_act.size() * sizeof(int), hipMemcpyHostToDevice,
-                              B->stream));
-    B->m_act = BlockMap{B->d_act + o.inst, B->d_act + o.first, B->d_act + o.count};
-    B->act_members = B->d_act + mem;
+    B->pcg_order = maps + order_at;
+    int* plan = nullptr;
+    int plan_len = 0;
+    for (int b = 0; b < n; b++) plan_len += (c_pcg[b] + 7) & ~7;
+    TRY(balloc(B, &plan, (size_t)plan_len + n + 1));
+    B->plan = PcgPlanBufs{plan, plan + plan_len, plan + plan_len + n};
+    // untuned: one workgroup per virtual workgroup of a full iteration (the
+    // hardware then balances members of different weights), at most 32768
+    if (!B->tuned) B->pcg_grid = std::max(8, std::min(32768, plan_len));
     return FS2D_OK;
 }
 
@@ -302,16 +312,16 @@ void bstage_level_set(fs2d_batch* B) {
                 switch (d) {
                     case 0: blaunch(B, b_sweep_kernel<ClosestSweepOp<0, 0>, false, ClosestSweepMaker<0, 0>>, S,
                                     kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
-                                    ClosestSweepMaker<0, 0>{}); break;
+                                    ClosestSweepMaker<0, 0>{}, (const int*)nullptr, 0); break;
                     case 1: blaunch(B, b_sweep_kernel<ClosestSweepOp<1, 0>, false, ClosestSweepMaker<1, 0>>, S,
                                     kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
-                                    ClosestSweepMaker<1, 0>{}); break;
+                                    ClosestSweepMaker<1, 0>{}, (const int*)nullptr, 0); break;
                     case 2: blaunch(B, b_sweep_kernel<ClosestSweepOp<0, 1>, false, ClosestSweepMaker<0, 1>>, S,
                                     kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
-                                    ClosestSweepMaker<0, 1>{}); break;
+                                    ClosestSweepMaker<0, 1>{}, (const int*)nullptr, 0); break;
                     default: blaunch(B, b_sweep_kernel<ClosestSweepOp<1, 1>, false, ClosestSweepMaker<1, 1>>, S,
                                      kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
-                                     ClosestSweepMaker<1, 1>{}); break;
+                                     ClosestSweepMaker<1, 1>{}, (const int*)nullptr, 0); break;
                 }
             }
         }
@@ -337,7 +347,7 @@ void bstage_level_set(fs2d_batch* B) {
             {
                 Timed t(B, K_LS_SWEEP);
                 blaunch(B, b_sweep_kernel<EikSweepOp, false, EikSweepMaker>, S, kSweepThreads, B->d_ins, B->strips,
-                        B->sk_ticket, B->werr, EikSweepMaker{});
+                        B->sk_ticket, B->werr, EikSweepMaker{}, (const int*)nullptr, 0);
             }
             Timed t(B, K_LS_OTHER);
             if (k < 15)
@@ -395,8 +405,49 @@ fs2d_status gather_status(fs2d_batch* B) {
     return FS2D_OK;
 }
 
-fs2d_status bstage_projection(fs2d_batch* B) {
-    blaunch(B, b_pcg_init_kernel, B->n, 64, B->d_ins, B->n, B->tol);
+// Iterations h0 .. h1-1 of the batched PCG loop (FluidSim2D.cpp:431-463):
+// every launch has a fixed shape, and members that have converged (or, after
+// all have, whole launches) return at once.
+template <class Fwd, class Bwd>
+void bpcg_iterations(fs2d_batch* B, int h0, int h1, Fwd fwd, Bwd bwd) {
+    const int S = B->strips.total;
+    for (int h = h0; h < h1; h++) {
+        {
+            Timed t(B, K_PCG_SPMV);
+            hipLaunchKernelGGL(b_pcg_plan_kernel, dim3(B->n), dim3(kBlock), B->plan_lds, B->stream, B->d_ins, B->n,
+                               B->m_pcg.count, B->pcg_order, B->plan, h, B->all_done_since);
+            blaunch(B, b_spmv_kernel, B->pcg_grid, kBlock, B->d_ins, B->m_pcg.count, B->plan, h);
+        }
+        {
+            Timed t(B, K_PCG_AXPY);
+            blaunch(B, b_ws_reduce_kernel, B->n, 64, B->d_ins, B->n, B->m_pcg.count);
+            blaunch(B, b_axpy_kernel, B->pcg_grid, kBlock, B->d_ins, B->m_pcg.count, B->plan, h);
+        }
+        {
+            Timed t(B, K_PCG_FWD);
+            blaunch(B, fwd, S, kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
+                    FwdMaker{B->m_pcg.count, 1}, (const int*)B->all_done_since, h);
+        }
+        {
+            Timed t(B, K_PCG_BWD);
+            blaunch(B, bwd, S, kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr, BwdMaker{h},
+                    (const int*)B->all_done_since, h);
+        }
+    }
+}
+void bpcg_iterations(fs2d_batch* B, int h0, int h1) {
+    if (B->sweep_prefetch)
+        bpcg_iterations(B, h0, h1, b_sweep_kernel<FwdOp, false, FwdMaker, true>,
+                        b_sweep_kernel<BwdOp, true, BwdMaker, true>);
+    else
+        bpcg_iterations(B, h0, h1, b_sweep_kernel<FwdOp, false, FwdMaker, false>,
+                        b_sweep_kernel<BwdOp, true, BwdMaker, false>);
+}
+
+// Assembly, MIC(0) factor and the first preconditioner application of every
+// member (FluidSim2D.cpp:252-430).
+void bpcg_begin(fs2d_batch* B) {
+    blaunch(B, b_pcg_init_kernel, B->n, 64, B->d_ins, B->n, B->tol, B->all_done_since);
     {
         Timed t(B, K_ASSEMBLE);
         blaunch(B, b_assemble_kernel, B->g_sk, kBlock, B->d_ins, B->m_sk);
@@ -405,64 +456,49 @@ fs2d_status bstage_projection(fs2d_batch* B) {
     {
         Timed t(B, K_FACTOR);
         blaunch(B, b_sweep_kernel<FactorOp, false, FactorMaker>, S, kSweepThreads, B->d_ins, B->strips,
-                B->sk_ticket, B->werr, FactorMaker{B->m_sk.count});
+                B->sk_ticket, B->werr, FactorMaker{B->m_sk.count}, (const int*)nullptr, 0);
     }
     {
         Timed t(B, K_PCG_FWD);
         blaunch(B, b_sweep_kernel<FwdOp, false, FwdMaker>, S, kSweepThreads, B->d_ins, B->strips, B->sk_ticket,
-                B->werr, FwdMaker{B->m_sk.count, 0});
+                B->werr, FwdMaker{B->m_sk.count, 0}, (const int*)nullptr, 0);
     }
     {
         Timed t(B, K_PCG_BWD);
         blaunch(B, b_sweep_kernel<BwdOp, true, BwdMaker>, S, kSweepThreads, B->d_ins, B->strips, B->sk_ticket,
-                B->werr, BwdMaker{-1});
+                B->werr, BwdMaker{-1}, (const int*)nullptr, 0);
     }
-    int h = 0;
-    int chunk = std::max(2, std::min(B->last_iters_hint, B->max_iters));
-    int iters = 0;
-    bool hit = false;
-    std::vector<int> active(B->n, 1);
-    TRY(build_active(B, active));
+}
+
+bool all_members_done(const fs2d_batch* B) {
+    for (int b = 0; b < B->n; b++)
+        if (B->h_status[4 * b] == 0) return false;
+    return true;
+}
+
+// Queues chunks of iterations from h until every member has converged or
+// max_iters is reached (one host round trip per chunk).
+fs2d_status bpcg_continue(fs2d_batch* B, int h, int chunk) {
     while (h < B->max_iters) {
         const int nchunk = std::min(chunk, B->max_iters - h);
-        for (int k = 0; k < nchunk; k++, h++) {
-            {
-                Timed t(B, K_PCG_SPMV);
-                blaunch(B, b_spmv_kernel, B->g_act, kBlock, B->d_ins, B->m_act, h);
-            }
-            {
-                Timed t(B, K_PCG_AXPY);
-                blaunch(B, b_ws_reduce_kernel, B->n_act, 64, B->d_ins, B->act_members, B->n_act, B->m_pcg.count);
-                blaunch(B, b_axpy_kernel, B->g_act, kBlock, B->d_ins, B->m_act, h);
-            }
-            {
-                Timed t(B, K_PCG_FWD);
-                blaunch(B, b_sweep_kernel<FwdOp, false, FwdMaker>, S, kSweepThreads, B->d_ins, B->strips,
-                        B->sk_ticket, B->werr, FwdMaker{B->m_pcg.count, 1});
-            }
-            {
-                Timed t(B, K_PCG_BWD);
-                blaunch(B, b_sweep_kernel<BwdOp, true, BwdMaker>, S, kSweepThreads, B->d_ins, B->strips,
-                        B->sk_ticket, B->werr, BwdMaker{h});
-            }
-        }
+        bpcg_iterations(B, h, h + nchunk);
+        h += nchunk;
         TRY(gather_status(B));
-        bool all = true, changed = false;
-        for (int b = 0; b < B->n; b++) {
-            const int a = B->h_status[4 * b] == 0;
-            all = all && !a;
-            changed = changed || a != active[b];
-            active[b] = a;
-        }
-        if (all) break;
-        if (changed) TRY(build_active(B, active));
+        if (all_members_done(B)) break;
         chunk = 8;
     }
-    {
-        Timed t(B, K_MISC);
-        blaunch(B, b_sk_to_rowmajor_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
-    }
-    TRY(gather_status(B));
+    return FS2D_OK;
+}
+
+void bpcg_end(fs2d_batch* B) {
+    Timed t(B, K_MISC);
+    blaunch(B, b_sk_to_rowmajor_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
+}
+
+// Iteration counts from the members' scalars (after gather_status).
+fs2d_status bpcg_status(fs2d_batch* B) {
+    int iters = 0;
+    bool hit = false;
     for (int b = 0; b < B->n; b++) {
         iters = std::max(iters, B->h_status[4 * b + 1]);
         hit = hit || B->h_status[4 * b + 1] >= B->max_iters;
@@ -470,6 +506,14 @@ fs2d_status bstage_projection(fs2d_batch* B) {
     }
     B->last_iters_hint = std::max(2, iters + 1);
     return hit ? FS2D_WARN_MAXITER : FS2D_OK;
+}
+
+fs2d_status bstage_projection(fs2d_batch* B) {
+    bpcg_begin(B);
+    TRY(bpcg_continue(B, 0, std::max(2, std::min(B->last_iters_hint, B->max_iters))));
+    bpcg_end(B);
+    TRY(gather_status(B));
+    return bpcg_status(B);
 }
 
 void bstage_update_velocity(fs2d_batch* B) {
@@ -544,9 +588,123 @@ fs2d_status bfinish(fs2d_batch* B, fs2d_status st) {
     return st;
 }
 
+void bgraph_free(fs2d_batch::Captured& G) {
+    if (G.pre) (void)hipGraphExecDestroy(G.pre);
+    if (G.post) (void)hipGraphExecDestroy(G.post);
+    G.pre = G.post = nullptr;
+}
+
+fs2d_status bend_capture(fs2d_batch* B, hipGraphExec_t* exec) {
+    hipGraph_t g = nullptr;
+    BHIPCHK(B, hipStreamEndCapture(B->stream, &g));
+    hipError_t e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return bfail(B, FS2D_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    return FS2D_OK;
+}
+
+void bstep_stages(const fs2d_batch* B, std::vector<int>& before, std::vector<int>& after) {
+    if (B->mode == FS2D_SEMILAGRANGIAN) {
+        before = {FS2D_STAGE_CREATE_WATER_LEVEL_SET, FS2D_STAGE_SEMI_LAGRANGIAN_ADVECTION,
+                  FS2D_STAGE_CREATE_SOLID_LEVEL_SET};
+        after = {FS2D_STAGE_UPDATE_VELOCITY, FS2D_STAGE_APPLY_ADVECTION};
+    } else {
+        before = {FS2D_STAGE_CREATE_WATER_LEVEL_SET, FS2D_STAGE_TRANSFER_VELOCITY_TO_GRID, FS2D_STAGE_APPLY_GRAVITY,
+                  FS2D_STAGE_CREATE_SOLID_LEVEL_SET};
+        after = {FS2D_STAGE_UPDATE_VELOCITY, FS2D_STAGE_UPDATE_PARTICLE_VELOCITIES, FS2D_STAGE_APPLY_ADVECTION};
+    }
+}
+
+// `pre`: the stages before the projection, the assembly, factor and
+// graph_budget PCG iterations of every member; `post`: the pressures' copy
+// and the stages after the projection.  Captured with the batch's host flags
+// as fs2d_batch_step sets them (sort invalid at the start of a step).
+fs2d_status bcapture_step(fs2d_batch* B, fs2d_batch::Captured& G) {
+    G.maps = B->d_maps;
+    G.budget = B->graph_budget;
+    G.pcg_grid = B->pcg_grid;
+    G.prefetch = B->sweep_prefetch ? 1 : 0;
+    std::vector<int> st_before, st_after;
+    bstep_stages(B, st_before, st_after);
+    const bool fuse = B->mode == FS2D_SEMILAGRANGIAN;
+    fs2d_status worst = FS2D_OK, st = FS2D_OK;
+    B->sorted_valid = false;
+    BHIPCHK(B, hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+    for (int stg : st_before)
+        if (st == FS2D_OK) st = bstage(B, stg, fuse, &worst);
+    if (st == FS2D_OK) {
+        bpcg_begin(B);
+        bpcg_iterations(B, 0, G.budget);
+    }
+    fs2d_status ce = bend_capture(B, &G.pre);
+    if (st == FS2D_OK && ce == FS2D_OK) {
+        BHIPCHK(B, hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+        bpcg_end(B);
+        for (int stg : st_after)
+            if (st == FS2D_OK) st = bstage(B, stg, fuse, &worst);
+        ce = bend_capture(B, &G.post);
+    }
+    B->sorted_valid = false;
+    if (st != FS2D_OK || ce != FS2D_OK) {
+        bgraph_free(G);
+        return st != FS2D_OK ? st : ce;
+    }
+    return FS2D_OK;
+}
+
+void bupdate_graph_budget(fs2d_batch* B, int needed) {
+    const int want = std::min(B->max_iters, std::max(16, (needed + needed / 16 + 4 + 31) / 32 * 32));
+    if (B->graph_budget <= 0 || needed > B->graph_budget || 2 * needed < B->graph_budget) B->graph_budget = want;
+    B->graph_budget = std::min(B->graph_budget, B->max_iters);
+}
+
+// fs2d_batch_step by replaying a captured step: one host check of every
+// member's convergence after the budget, chunks if some member needs more.
+fs2d_status bstep_graph(fs2d_batch* B) {
+    if (B->graph_budget <= 0) bupdate_graph_budget(B, B->last_iters_hint);
+    fs2d_batch::Captured* G = nullptr;
+    for (auto& g : B->graphs)
+        if (g.maps == B->d_maps && g.budget == B->graph_budget && g.pcg_grid == B->pcg_grid &&
+            g.prefetch == (B->sweep_prefetch ? 1 : 0))
+            G = &g;
+    if (!G) {
+        if (B->graphs.size() >= 2) {
+            bgraph_free(B->graphs.front());
+            B->graphs.erase(B->graphs.begin());
+        }
+        fs2d_batch::Captured ng;
+        TRY(bcapture_step(B, ng));
+        B->graphs.push_back(ng);
+        G = &B->graphs.back();
+    }
+    BHIPCHK(B, hipGraphLaunch(G->pre, B->stream));
+    TRY(gather_status(B));
+    if (!all_members_done(B) && G->budget < B->max_iters) TRY(bpcg_continue(B, G->budget, 8));
+    BHIPCHK(B, hipGraphLaunch(G->post, B->stream));
+    TRY(gather_status(B));
+    const fs2d_status pst = bpcg_status(B);
+    bupdate_graph_budget(B, B->last_iters_hint - 1);
+    for (auto* s : B->sims) {
+        s->current_time += s->dt;
+        s->steps++;
+    }
+    return bfinish(B, pst);
+}
+
 }  // namespace
 
 extern "C" {
+
+fs2d_status fs2d_batch_set_graph(fs2d_batch* B, int enable) {
+    if (!B) return FS2D_ERR_ARG;
+    BHIPCHK(B, hipStreamSynchronize(B->stream));
+    B->use_graph = enable != 0;
+    if (!B->use_graph) {
+        for (auto& g : B->graphs) bgraph_free(g);
+        B->graphs.clear();
+    }
+    return FS2D_OK;
+}
 
 fs2d_status fs2d_batch_create(const fs2d_config* cfgs, int n, fs2d_batch** out) {
     if (!cfgs || !out || n < 1) return FS2D_ERR_ARG;
@@ -596,9 +754,10 @@ fs2d_status fs2d_batch_create(const fs2d_config* cfgs, int n, fs2d_batch** out) 
     }
     if ((st = balloc(B, &B->d_ins, (size_t)n)) != FS2D_OK) return bail(st);
     if ((st = build_maps(B)) != FS2D_OK) return bail(st);
-    if ((st = balloc(B, &B->werr, 1)) != FS2D_OK ||
+    if ((st = balloc(B, &B->werr, 1)) != FS2D_OK || (st = balloc(B, &B->all_done_since, 1)) != FS2D_OK ||
         (st = balloc(B, &B->sk_ticket, 1)) != FS2D_OK || (st = balloc(B, &B->d_status, 4 * (size_t)n)) != FS2D_OK)
         return bail(st);
+    B->plan_lds = sizeof(int) * (size_t)n;
     if (hipHostMalloc((void**)&B->h_status, 4 * n * sizeof(int), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&B->h_err, sizeof(int), hipHostMallocDefault) != hipSuccess) {
         B->err = "hipHostMalloc failed";
@@ -617,6 +776,7 @@ fs2d_status fs2d_batch_create(const fs2d_config* cfgs, int n, fs2d_batch** out) 
 void fs2d_batch_destroy(fs2d_batch* B) {
     if (!B) return;
     if (B->stream) (void)hipStreamSynchronize(B->stream);
+    for (auto& g : B->graphs) bgraph_free(g);
     for (auto* s : B->sims) fs2d_destroy(s);
     for (void* p : B->allocs) (void)hipFree(p);
     for (auto& t : B->pending) {
@@ -637,6 +797,7 @@ fs2d_sim* fs2d_batch_sim(fs2d_batch* B, int i) { return (B && i >= 0 && i < B->n
 fs2d_status fs2d_batch_step(fs2d_batch* B) {
     if (!B) return FS2D_ERR_ARG;
     TRY(refresh_inst(B));
+    if (B->use_graph && !B->timing) return bstep_graph(B);
     B->sorted_valid = false;
     fs2d_status worst = FS2D_OK, st;
     if (B->mode == FS2D_SEMILAGRANGIAN) {

@@ -10,6 +10,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include <algorithm>
 #include <cmath>
@@ -46,6 +47,29 @@ int blocks_for(long n) {
 struct TimedLaunch {
     int cls;
     hipEvent_t a, b;
+};
+
+// What a captured step depends on besides device memory: the kernels'
+// arguments (sizes, parameters, buffer pointers, which the semi-Lagrangian
+// stage swaps) and the host flags that choose launches.  A step is replayed
+// only from the state it was captured in.
+struct GraphKey {
+    double dt, gx, gy, alpha, tau, sigma, tol;
+    int max_iters, budget, pcg_grid, sorted_valid;
+    long np;
+    const void* ptr[12];  // u, v, u2, v2, dens, dens2, px, py, pvx, pvy, pkey, sorted
+    bool operator==(const GraphKey& o) const { return memcmp(this, &o, sizeof o) == 0; }
+};
+// The handle's host state after the captured step.
+struct GraphAfter {
+    float *u, *v, *u2, *v2, *dens, *dens2;
+    bool sorted_valid;
+};
+struct StepGraph {
+    GraphKey key;
+    GraphAfter after;
+    hipGraphExec_t pre = nullptr, post = nullptr;
+    int budget = 0;
 };
 
 }  // namespace
@@ -101,6 +125,13 @@ struct fs2d_sim {
     double current_time = 0, orig_volume = 0;
     long long steps = 0;
     int last_iters_hint = 8;
+    int pcg_grid = 0;  // physical workgroups of the PCG SpMV / axpy (0: one per virtual workgroup)
+    // Captured steps (fs2d_set_graph): each replays the stages up to a fixed
+    // number of PCG iterations, then (after one host check of convergence)
+    // the rest of the step.
+    bool use_graph = false;
+    int graph_budget = 0;
+    std::vector<StepGraph> graphs;
     std::string err;
 
     unsigned long long timing = 0;  // bit k: launches of kernel class k are timed
@@ -485,24 +516,27 @@ void assemble(fs2d_sim* s) {
                        ghost_pressure(s));
 }
 
-// MIC(0) factor and PCG (src/FluidSim2D.cpp:364-466) on the assembled matrix
-// and the rhs in r (with p = s = 0 and the |rhs|_inf partials in rmax_part);
-// the solution goes to `out`, row-major.
-fs2d_status pcg_solve(fs2d_sim* s, float* out) {
-    const SkGeom& g = s->geom;
+// Physical workgroups of the PCG SpMV and axpy launches: their loops run the
+// pcg_blocks virtual workgroups (pcg_grid 0: one each), so this choice
+// changes no result; a multiple of 8 (xcd_loop).
+int pcg_launch(const fs2d_sim* s) {
+    const int nvb = pcg_blocks(s);
+    return s->pcg_grid > 0 ? std::min(nvb, s->pcg_grid) : nvb;
+}
+
+// MIC(0) factor and the first preconditioner application of the PCG
+// (src/FluidSim2D.cpp:364-430) on the assembled matrix and the rhs in r (with
+// p = s = 0 and the |rhs|_inf partials in rmax_part).
+void pcg_begin(fs2d_sim* s) {
     const int nb = sk_blocks(s);
     const double sA = s->dt / (s->rho * s->dx * s->dx);
-    PcgScalars init{};
-    init.tol = s->tol;
-    HIPCHK(s, hipMemcpyAsync(s->sc, &init, sizeof init, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(pcg_init_kernel, dim3(1), dim3(64), 0, s->stream, s->sc, s->tol);
     {
         Timed t(s, K_FACTOR);
         FactorOp op{{s->Adiag}, s->cpl, {s->P, s->fx, s->fy, s->gx_, s->gy_}, s->tau, s->sigma, sA, s->rmax_part, nb,
                     s->max_iters, s->sc};
         launch_sweep<FactorOp, false>(s, op);
     }
-    double* qq0 = s->qq;
-    double* qq1 = s->qq + (size_t)g.S * kLoaders;
     {
         Timed t(s, K_PCG_FWD);
         FwdOp op{{s->r, s->P, s->fx, s->fy}, nullptr, {s->q}, s->sc, s->rmax_part, nb, 0};
@@ -510,51 +544,73 @@ fs2d_status pcg_solve(fs2d_sim* s, float* out) {
     }
     {
         Timed t(s, K_PCG_BWD);
-        BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, qq0, 0};
+        BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, s->qq, 0};
         launch_sweep<BwdOp, true>(s, op);
     }
+}
+
+// Iterations h0 .. h1-1 of the PCG loop (:431-463): SpMV with the s update,
+// the p/r update, forward and backward solves.  Every kernel returns at once
+// once the stopping test has passed, so a fixed number of iterations can be
+// queued (or captured) without host round trips.
+void pcg_iterations(fs2d_sim* s, int h0, int h1) {
+    const SkGeom& g = s->geom;
+    const double sA = s->dt / (s->rho * s->dx * s->dx);
+    double* qq0 = s->qq;
+    double* qq1 = s->qq + (size_t)g.S * kLoaders;
+    const int nbp = pcg_blocks(s), grid = pcg_launch(s);
+    for (int h = h0; h < h1; h++) {
+        float* s_old = s->s[h & 1];
+        float* s_new = s->s[(h + 1) & 1];
+        double* sig_cur = (h & 1) ? qq1 : qq0;   // sigma_h
+        double* sig_prev = (h & 1) ? qq0 : qq1;  // sigma_{h-1}; receives sigma_{h+1}
+        {
+            Timed t(s, K_PCG_SPMV);
+            hipLaunchKernelGGL(pcg_spmv_sk_kernel, dim3(grid), dim3(kBlock), 0, s->stream, g, s->Adiag, s->cpl, s->z,
+                               s_old, s_new, s->w, (float)sA, s->sc, sig_cur, sig_prev, h == 0 ? 1 : 0, s->ws_part,
+                               nbp);
+        }
+        {
+            Timed t(s, K_PCG_AXPY);
+            hipLaunchKernelGGL(pcg_axpy_sk_kernel, dim3(grid), dim3(kBlock), 0, s->stream, g, s->pk, s->r, s_new,
+                               s->w, s->sc, sig_cur, s->ws_part, nbp, s->rmax_part, nbp);
+        }
+        {
+            Timed t(s, K_PCG_FWD);
+            FwdOp op{{s->r, s->P, s->fx, s->fy}, nullptr, {s->q}, s->sc, s->rmax_part, nbp, 1};
+            launch_sweep<FwdOp, false>(s, op);
+        }
+        {
+            Timed t(s, K_PCG_BWD);
+            BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, sig_prev, h + 1};
+            launch_sweep<BwdOp, true>(s, op);
+        }
+    }
+}
+
+// The solution, row-major.
+void pcg_end(fs2d_sim* s, float* out) {
+    Timed t(s, K_MISC);
+    hipLaunchKernelGGL(sk_to_rowmajor_kernel, dim3(blocks_for(n_c(s))), dim3(kBlock), 0, s->stream, s->geom, s->pk,
+                       out);
+}
+
+// The whole PCG: the host reads the stopping flag once per chunk of queued
+// iterations (the first chunk sized by the previous solve's count).
+fs2d_status pcg_solve(fs2d_sim* s, float* out) {
+    pcg_begin(s);
     int h = 0;
-    const int nbp = pcg_blocks(s);
     int batch = std::max(2, std::min(s->last_iters_hint, s->max_iters));
     while (h < s->max_iters) {
         const int nbatch = std::min(batch, s->max_iters - h);
-        for (int k = 0; k < nbatch; k++, h++) {
-            float* s_old = s->s[h & 1];
-            float* s_new = s->s[(h + 1) & 1];
-            double* sig_cur = (h & 1) ? qq1 : qq0;   // sigma_h
-            double* sig_prev = (h & 1) ? qq0 : qq1;  // sigma_{h-1}; receives sigma_{h+1}
-            {
-                Timed t(s, K_PCG_SPMV);
-                hipLaunchKernelGGL(pcg_spmv_sk_kernel, dim3(nbp), dim3(kBlock), 0, s->stream, g, s->Adiag, s->cpl,
-                                   s->z, s_old, s_new, s->w, (float)sA, s->sc, sig_cur, sig_prev, h == 0 ? 1 : 0,
-                                   s->ws_part);
-            }
-            {
-                Timed t(s, K_PCG_AXPY);
-                hipLaunchKernelGGL(pcg_axpy_sk_kernel, dim3(nbp), dim3(kBlock), 0, s->stream, g, s->pk, s->r,
-                                   s_new, s->w, s->sc, sig_cur, s->ws_part, nbp, s->rmax_part);
-            }
-            {
-                Timed t(s, K_PCG_FWD);
-                FwdOp op{{s->r, s->P, s->fx, s->fy}, nullptr, {s->q}, s->sc, s->rmax_part, nbp, 1};
-                launch_sweep<FwdOp, false>(s, op);
-            }
-            {
-                Timed t(s, K_PCG_BWD);
-                BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, sig_prev, h + 1};
-                launch_sweep<BwdOp, true>(s, op);
-            }
-        }
+        pcg_iterations(s, h, h + nbatch);
+        h += nbatch;
         HIPCHK(s, hipMemcpyAsync(s->h_sc, s->sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s->stream));
         HIPCHK(s, hipStreamSynchronize(s->stream));
         if (s->h_sc->done) break;
         batch = 4;
     }
-    {
-        Timed t(s, K_MISC);
-        hipLaunchKernelGGL(sk_to_rowmajor_kernel, dim3(blocks_for(n_c(s))), dim3(kBlock), 0, s->stream, g, s->pk,
-                           out);
-    }
+    pcg_end(s, out);
     TRY(d2h(s, s->h_sc, s->sc, sizeof(PcgScalars)));
     s->last_iters_hint = std::max(2, s->h_sc->iter + 1);
     return s->h_sc->iter >= s->max_iters ? FS2D_WARN_MAXITER : FS2D_OK;
@@ -642,6 +698,142 @@ fs2d_status run_stage_impl(fs2d_sim* s, int stage, bool fuse_gravity, fs2d_statu
     if (st < 0) return st;
     if (st > *worst) *worst = st;
     return FS2D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Captured steps.
+// ---------------------------------------------------------------------------
+GraphKey graph_key(const fs2d_sim* s) {
+    GraphKey k;
+    memset(&k, 0, sizeof k);  // padding too: keys are compared bytewise
+    k.dt = s->dt; k.gx = s->gx; k.gy = s->gy; k.alpha = s->alpha;
+    k.tau = s->tau; k.sigma = s->sigma; k.tol = s->tol;
+    k.max_iters = s->max_iters; k.budget = s->graph_budget; k.pcg_grid = s->pcg_grid;
+    k.sorted_valid = s->sorted_valid ? 1 : 0;
+    k.np = s->np;
+    const void* p[12] = {s->u, s->v, s->u2, s->v2, s->dens, s->dens2, s->px, s->py, s->pvx, s->pvy, s->pkey, s->sorted};
+    memcpy(k.ptr, p, sizeof p);
+    return k;
+}
+GraphAfter graph_after(const fs2d_sim* s) {
+    return GraphAfter{s->u, s->v, s->u2, s->v2, s->dens, s->dens2, s->sorted_valid};
+}
+void graph_restore(fs2d_sim* s, const GraphAfter& a) {
+    s->u = a.u; s->v = a.v; s->u2 = a.u2; s->v2 = a.v2; s->dens = a.dens; s->dens2 = a.dens2;
+    s->sorted_valid = a.sorted_valid;
+}
+void graph_free(StepGraph& G) {
+    if (G.pre) (void)hipGraphExecDestroy(G.pre);
+    if (G.post) (void)hipGraphExecDestroy(G.post);
+    G.pre = G.post = nullptr;
+}
+
+// Stages of fs2d_step before and after applyProjection (runFrame,
+// src/FluidSim2D.cpp:100-133; gravity fused into the SL advection).
+void step_stages(const fs2d_sim* s, std::vector<int>& before, std::vector<int>& after) {
+    if (s->mode == FS2D_SEMILAGRANGIAN) {
+        before = {FS2D_STAGE_CREATE_WATER_LEVEL_SET, FS2D_STAGE_SEMI_LAGRANGIAN_ADVECTION,
+                  FS2D_STAGE_CREATE_SOLID_LEVEL_SET};
+        after = {FS2D_STAGE_UPDATE_VELOCITY, FS2D_STAGE_APPLY_ADVECTION};
+    } else {
+        before = {FS2D_STAGE_CREATE_WATER_LEVEL_SET, FS2D_STAGE_TRANSFER_VELOCITY_TO_GRID, FS2D_STAGE_APPLY_GRAVITY,
+                  FS2D_STAGE_CREATE_SOLID_LEVEL_SET};
+        after = {FS2D_STAGE_UPDATE_VELOCITY, FS2D_STAGE_UPDATE_PARTICLE_VELOCITIES, FS2D_STAGE_APPLY_ADVECTION};
+    }
+}
+
+fs2d_status end_capture(fs2d_sim* s, hipGraphExec_t* exec) {
+    hipGraph_t g = nullptr;
+    HIPCHK(s, hipStreamEndCapture(s->stream, &g));
+    hipError_t e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(s, FS2D_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    return FS2D_OK;
+}
+
+// Captures the step from the handle's current host state: `pre` runs the
+// stages before the projection, the assembly, the MIC(0) factor and
+// graph_budget PCG iterations; `post` the solution's copy and the stages
+// after the projection.  Nothing runs; the host state is left as it was.
+fs2d_status capture_step(fs2d_sim* s, StepGraph& G) {
+    G.key = graph_key(s);
+    G.budget = s->graph_budget;
+    const GraphAfter before = graph_after(s);
+    std::vector<int> st_before, st_after;
+    step_stages(s, st_before, st_after);
+    fs2d_status worst = FS2D_OK, st = FS2D_OK;
+    HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    const bool fuse = s->mode == FS2D_SEMILAGRANGIAN;  // gravity inside the SL advection, as fs2d_step
+    for (int stg : st_before)
+        if (st == FS2D_OK) st = run_stage_impl(s, stg, fuse, &worst);
+    if (st == FS2D_OK) {
+        assemble(s);
+        pcg_begin(s);
+        pcg_iterations(s, 0, G.budget);
+    }
+    fs2d_status ce = end_capture(s, &G.pre);
+    if (st == FS2D_OK && ce == FS2D_OK) {
+        HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        pcg_end(s, s->p);
+        for (int stg : st_after)
+            if (st == FS2D_OK) st = run_stage_impl(s, stg, fuse, &worst);
+        ce = end_capture(s, &G.post);
+    }
+    G.after = graph_after(s);
+    graph_restore(s, before);
+    if (st != FS2D_OK || ce != FS2D_OK) {
+        graph_free(G);
+        return st != FS2D_OK ? st : ce;
+    }
+    return FS2D_OK;
+}
+
+// PCG iterations a captured step runs before its host check: the last
+// solve's count plus a sixteenth and 4, in steps of 32, kept while it
+// suffices and is at most twice what is needed.
+void update_graph_budget(fs2d_sim* s, int needed) {
+    const int want = std::min(s->max_iters, std::max(16, (needed + needed / 16 + 4 + 31) / 32 * 32));
+    if (s->graph_budget <= 0 || needed > s->graph_budget || 2 * needed < s->graph_budget) s->graph_budget = want;
+}
+
+// fs2d_step by replaying a captured step (capturing it first when the host
+// state is new).  Convergence within the budget leaves one host round trip
+// in the step besides finish(); a solve that needs more iterations continues
+// with queued chunks, as the uncaptured step does.
+fs2d_status step_graph(fs2d_sim* s) {
+    if (s->graph_budget <= 0) update_graph_budget(s, s->last_iters_hint);
+    s->graph_budget = std::min(s->graph_budget, s->max_iters);
+    const GraphKey key = graph_key(s);
+    StepGraph* G = nullptr;
+    for (auto& g : s->graphs)
+        if (g.key == key) G = &g;
+    if (!G) {
+        if (s->graphs.size() >= 4) {  // two buffer parities x two budgets
+            graph_free(s->graphs.front());
+            s->graphs.erase(s->graphs.begin());
+        }
+        StepGraph ng;
+        TRY(capture_step(s, ng));
+        s->graphs.push_back(ng);
+        G = &s->graphs.back();
+    }
+    HIPCHK(s, hipGraphLaunch(G->pre, s->stream));
+    TRY(d2h(s, s->h_sc, s->sc, sizeof(PcgScalars)));
+    int h = G->budget;
+    while (!s->h_sc->done && h < s->max_iters) {
+        const int n = std::min(8, s->max_iters - h);
+        pcg_iterations(s, h, h + n);
+        h += n;
+        TRY(d2h(s, s->h_sc, s->sc, sizeof(PcgScalars)));
+    }
+    HIPCHK(s, hipGraphLaunch(G->post, s->stream));
+    graph_restore(s, G->after);
+    const int iters = s->h_sc->iter;
+    s->last_iters_hint = std::max(2, iters + 1);
+    update_graph_budget(s, iters);
+    s->current_time += s->dt;
+    s->steps++;
+    return finish(s, iters >= s->max_iters ? FS2D_WARN_MAXITER : FS2D_OK);
 }
 
 bool valid_dtype(int field, int dtype) {
@@ -868,6 +1060,7 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
 void fs2d_destroy(fs2d_sim* s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto& g : s->graphs) graph_free(g);
     for (void* p : s->allocs) (void)hipFree(p);
     for (auto& t : s->pending) {
         (void)hipEventDestroy(t.a);
@@ -883,6 +1076,7 @@ void fs2d_destroy(fs2d_sim* s) {
 
 fs2d_status fs2d_step(fs2d_sim* s) {
     if (!s) return FS2D_ERR_ARG;
+    if (s->use_graph && !s->timing) return step_graph(s);
     fs2d_status worst = FS2D_OK, st;
     if (s->mode == FS2D_SEMILAGRANGIAN) {
         // createWaterLevelSet, applySemiLagrangianAdvection + applyGravity (fused),
@@ -1202,6 +1396,17 @@ fs2d_status fs2d_max_velocity(fs2d_sim* s, double* out) {
     return FS2D_OK;
 }
 
+fs2d_status fs2d_set_graph(fs2d_sim* s, int enable) {
+    if (!s) return FS2D_ERR_ARG;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    s->use_graph = enable != 0;
+    if (!s->use_graph) {
+        for (auto& g : s->graphs) graph_free(g);
+        s->graphs.clear();
+    }
+    return FS2D_OK;
+}
+
 fs2d_status fs2d_set_timing(fs2d_sim* s, int enable) {
     return fs2d_set_timing_classes(s, enable ? ~0ull : 0ull);
 }
@@ -1405,4 +1610,5 @@ fs2d_status fs2d_project_noalloc(const fs2d_projection_args* a, int* iters, doub
 
 #include "fs2d_adjoint_api.hpp"
 #include "fs2d_batch_api.hpp"
+#include "fs2d_tune_api.hpp"
 #include "fs3d_api.hpp"

@@ -299,29 +299,56 @@ __device__ __forceinline__ long sk_above_edge(const SkGeom& g, long e) {
     const long e4 = e & 3;
     return ((e >> 8) + g.T4 + ((e4 - 63) >> 2)) * 256 + ((e4 + 1) & 3);
 }
-__device__ __forceinline__ void pcg_spmv_sk_body(
-    SkGeom g, const float* __restrict__ Adiag, const uint8_t* __restrict__ flags, const float* __restrict__ z,
-    const float* __restrict__ s_old, float* __restrict__ s_new, float* __restrict__ w, float sA,
-    PcgScalars* sc, const double* sig_cur, const double* sig_prev, int first, double* partials, int vb, int nvb) {
-    if (sc->done) return;
-    // An entry's own values do not depend on beta: the first entry's are
-    // loaded before the reductions that give beta, and each later entry's at
-    // the end of the previous round.
-    const XcdLoop it = xcd_loop(g.size(), vb, nvb);
-    long e = it.e;
-    float so = 0.0f, zc = 0.0f, ad = 0.0f;
-    uint32_t f = 0;
-    if (e < it.end) {
-        so = s_old[e];
-        zc = z[e];
-        ad = Adiag[e];
-        f = flags[e];
+// One virtual workgroup (vb of nvb) of the SpMV: the partial w.s of the
+// entries xcd_loop gives it goes to partials[vb].  The entries' own values do
+// not depend on beta, so the first entry's are loaded (spmv_pre) before the
+// reductions that give beta, and each later entry's at the end of the
+// previous round.
+struct SpmvPre {
+    XcdLoop it;
+    float so, zc, ad;
+    uint32_t f;
+};
+__device__ __forceinline__ SpmvPre spmv_pre(const SkGeom& g, const float* __restrict__ Adiag,
+                                            const uint8_t* __restrict__ flags, const float* __restrict__ z,
+                                            const float* __restrict__ s_old, int vb, int nvb) {
+    SpmvPre r;
+    r.it = xcd_loop(g.size(), vb, nvb);
+    r.so = r.zc = r.ad = 0.0f;
+    r.f = 0;
+    const long e = r.it.e;
+    if (e < r.it.end) {
+        r.so = s_old[e];
+        r.zc = z[e];
+        r.ad = Adiag[e];
+        r.f = flags[e];
     }
-    const double sig = block_sum_bcast(sig_cur, g.S * kLoaders);
+    return r;
+}
+// sigma_k = z.r from the backward solves' per-strip q.q partials and beta =
+// sigma_k / sigma_{k-1} (:457-458), reduced by the whole block in a fixed
+// order (the same in every workgroup).
+struct SpmvScalars {
+    double sig;
+    float beta;
+};
+__device__ __forceinline__ SpmvScalars spmv_scalars(const SkGeom& g, const double* sig_cur, const double* sig_prev,
+                                                    int first) {
+    SpmvScalars r;
+    r.sig = block_sum_bcast(sig_cur, g.S * kLoaders);
     double beta_d = 0;
-    if (!first) beta_d = sig / block_sum_bcast(sig_prev, g.S * kLoaders);
-    if (vb == 0 && threadIdx.x == 0) sc->sigma = sig;
-    const float beta = (float)beta_d;
+    if (!first) beta_d = r.sig / block_sum_bcast(sig_prev, g.S * kLoaders);
+    r.beta = (float)beta_d;
+    return r;
+}
+__device__ __forceinline__ void spmv_core(const SpmvPre& pre, const SkGeom& g, const float* __restrict__ Adiag,
+                                          const uint8_t* __restrict__ flags, const float* __restrict__ z,
+                                          const float* __restrict__ s_old, float* __restrict__ s_new,
+                                          float* __restrict__ w, float sA, float beta, double* partials, int vb) {
+    const XcdLoop it = pre.it;
+    long e = it.e;
+    float so = pre.so, zc = pre.zc, ad = pre.ad;
+    uint32_t f = pre.f;
     double acc = 0;
     while (e < it.end) {
         const float sc_ = fmaf(beta, so, zc);
@@ -351,34 +378,43 @@ __device__ __forceinline__ void pcg_spmv_sk_body(
     double bs = block_sum(acc);
     if (threadIdx.x == 0) partials[vb] = bs;
 }
+// s' = z + beta s (:459, FMA as Array2D.h:185), w = A s' (:433-444), partial
+// w.s' (:450), over nvb virtual workgroups: workgroup k of the launch runs
+// vb = k, k + gridDim.x, ...  Each vb computes what a launch of nvb
+// workgroups computes in its workgroup vb, so the partials, and every result,
+// do not depend on the launch's size.  A launch size that is a multiple of 8
+// keeps vb % 8 the XCD (xcd_loop).  sigma goes to the scalars (vb 0).
 __global__ __launch_bounds__(kBlock) void pcg_spmv_sk_kernel(
     SkGeom g, const float* __restrict__ Adiag, const uint8_t* __restrict__ flags, const float* __restrict__ z,
     const float* __restrict__ s_old, float* __restrict__ s_new, float* __restrict__ w, float sA,
-    PcgScalars* sc, const double* sig_cur, const double* sig_prev, int first, double* partials) {
-    pcg_spmv_sk_body(g, Adiag, flags, z, s_old, s_new, w, sA, sc, sig_cur, sig_prev, first, partials, blockIdx.x, gridDim.x);
+    PcgScalars* sc, const double* sig_cur, const double* sig_prev, int first, double* partials, int nvb) {
+    if (sc->done) return;
+    int vb = blockIdx.x;
+    if (vb >= nvb) return;
+    SpmvPre pre = spmv_pre(g, Adiag, flags, z, s_old, vb, nvb);
+    const SpmvScalars S = spmv_scalars(g, sig_cur, sig_prev, first);
+    if (vb == 0 && threadIdx.x == 0) sc->sigma = S.sig;
+    for (;;) {
+        spmv_core(pre, g, Adiag, flags, z, s_old, s_new, w, sA, S.beta, partials, vb);
+        vb += gridDim.x;
+        if (vb >= nvb) break;
+        pre = spmv_pre(g, Adiag, flags, z, s_old, vb, nvb);
+    }
 }
 
 // alpha = sigma / (w.s') (:450); p += alpha s', r -= alpha w (:451-452);
 // partial |r|_inf for the stopping test (:453), finished by the forward solve.
-__device__ __forceinline__ void pcg_axpy_sk_body(
-    SkGeom g, float* __restrict__ p, float* __restrict__ r, const float* __restrict__ s,
-    const float* __restrict__ w, const PcgScalars* sc, const double* sig_cur, const double* ws_partials,
-    int n_ws, double* partials, int vb, int nvb) {
-    if (sc->done) return;
-    // One float4 of each array per thread and round (the SK4 arrays hold a
-    // multiple of 256 entries).  Rounds go in chunks of U; the first chunk's
-    // loads do not depend on alpha and go out before the reductions that give
-    // it.
-    float4* p4 = (float4*)p;
-    float4* r4 = (float4*)r;
-    const float4* s4 = (const float4*)s;
-    const float4* w4 = (const float4*)w;
-    const XcdLoop it = xcd_loop(g.size() / 4, vb, nvb);
-    constexpr int U = 2;
-    float4 sv[U], wv[U], pv[U], rv[U];
-    auto load = [&](long e0) {
+// One float4 of each array per thread and round (the SK4 arrays hold a
+// multiple of 256 entries).  Rounds go in chunks of kAxpyU; a virtual
+// workgroup's first chunk is loaded (AxpyBuf::load) before alpha is known
+// when it is the launch workgroup's first.
+constexpr int kAxpyU = 2;
+struct AxpyBuf {
+    float4 sv[kAxpyU], wv[kAxpyU], pv[kAxpyU], rv[kAxpyU];
+    __device__ __forceinline__ void load(const XcdLoop& it, long e0, const float4* p4, const float4* r4,
+                                         const float4* s4, const float4* w4) {
 #pragma unroll
-        for (int u = 0; u < U; u++) {
+        for (int u = 0; u < kAxpyU; u++) {
             const long e = e0 + u * it.stride;
             if (e < it.end) {
                 sv[u] = s4[e];
@@ -387,43 +423,72 @@ __device__ __forceinline__ void pcg_axpy_sk_body(
                 rv[u] = r4[e];
             }
         }
-    };
-    load(it.e);
-    // n_ws < 0: sigma and w.s were already reduced into sc, in the same order
-    const double sig = n_ws < 0 ? sc->sigma : block_sum_bcast(sig_cur, g.S * kLoaders);
-    const double ws = n_ws < 0 ? sc->ws : block_sum_bcast(ws_partials, n_ws);
-    const float a = (float)(sig / ws);
+    }
+};
+__device__ __forceinline__ void axpy_core(AxpyBuf& B, const XcdLoop& it, float4* p4, float4* r4, const float4* s4,
+                                          const float4* w4, float a, double* partials, int vb) {
     float m = 0.0f;
     for (long e0 = it.e; e0 < it.end;) {
 #pragma unroll
-        for (int u = 0; u < U; u++) {
+        for (int u = 0; u < kAxpyU; u++) {
             const long e = e0 + u * it.stride;
             if (e < it.end) {
-                float4 pn = pv[u], rn = rv[u];
-                pn.x = fmaf(a, sv[u].x, pn.x);
-                pn.y = fmaf(a, sv[u].y, pn.y);
-                pn.z = fmaf(a, sv[u].z, pn.z);
-                pn.w = fmaf(a, sv[u].w, pn.w);
-                rn.x = fmaf(-a, wv[u].x, rn.x);
-                rn.y = fmaf(-a, wv[u].y, rn.y);
-                rn.z = fmaf(-a, wv[u].z, rn.z);
-                rn.w = fmaf(-a, wv[u].w, rn.w);
+                float4 pn = B.pv[u], rn = B.rv[u];
+                pn.x = fmaf(a, B.sv[u].x, pn.x);
+                pn.y = fmaf(a, B.sv[u].y, pn.y);
+                pn.z = fmaf(a, B.sv[u].z, pn.z);
+                pn.w = fmaf(a, B.sv[u].w, pn.w);
+                rn.x = fmaf(-a, B.wv[u].x, rn.x);
+                rn.y = fmaf(-a, B.wv[u].y, rn.y);
+                rn.z = fmaf(-a, B.wv[u].z, rn.z);
+                rn.w = fmaf(-a, B.wv[u].w, rn.w);
                 p4[e] = pn;
                 r4[e] = rn;
                 m = fmaxf(m, fmaxf(fmaxf(fabsf(rn.x), fabsf(rn.y)), fmaxf(fabsf(rn.z), fabsf(rn.w))));
             }
         }
-        e0 += U * it.stride;
-        if (e0 < it.end) load(e0);
+        e0 += kAxpyU * it.stride;
+        if (e0 < it.end) B.load(it, e0, p4, r4, s4, w4);
     }
     double bm = block_max((double)m);
     if (threadIdx.x == 0) partials[vb] = bm;
 }
+// The axpy over nvb virtual workgroups (as the SpMV).  n_ws >= 0: sigma and
+// w.s are reduced from the partials here; n_ws < 0: they were reduced into sc
+// already, in the same order.
 __global__ __launch_bounds__(kBlock) void pcg_axpy_sk_kernel(
     SkGeom g, float* __restrict__ p, float* __restrict__ r, const float* __restrict__ s,
     const float* __restrict__ w, const PcgScalars* sc, const double* sig_cur, const double* ws_partials,
-    int n_ws, double* partials) {
-    pcg_axpy_sk_body(g, p, r, s, w, sc, sig_cur, ws_partials, n_ws, partials, blockIdx.x, gridDim.x);
+    int n_ws, double* partials, int nvb) {
+    if (sc->done) return;
+    int vb = blockIdx.x;
+    if (vb >= nvb) return;
+    float4* p4 = (float4*)p;
+    float4* r4 = (float4*)r;
+    const float4* s4 = (const float4*)s;
+    const float4* w4 = (const float4*)w;
+    AxpyBuf B;
+    XcdLoop it = xcd_loop(g.size() / 4, vb, nvb);
+    B.load(it, it.e, p4, r4, s4, w4);
+    const double sig = n_ws < 0 ? sc->sigma : block_sum_bcast(sig_cur, g.S * kLoaders);
+    const double ws = n_ws < 0 ? sc->ws : block_sum_bcast(ws_partials, n_ws);
+    const float a = (float)(sig / ws);
+    for (;;) {
+        axpy_core(B, it, p4, r4, s4, w4, a, partials, vb);
+        vb += gridDim.x;
+        if (vb >= nvb) break;
+        it = xcd_loop(g.size() / 4, vb, nvb);
+        B.load(it, it.e, p4, r4, s4, w4);
+    }
+}
+
+// PcgScalars of one projection: zero, tolerance set (:424-430).
+__global__ void pcg_init_kernel(PcgScalars* sc, double tol) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        PcgScalars z{};
+        z.tol = tol;
+        *sc = z;
+    }
 }
 
 // SK4 -> row-major copy of the pressure.

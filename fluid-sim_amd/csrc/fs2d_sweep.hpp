@@ -360,7 +360,7 @@ __device__ __forceinline__ unsigned long long sweep_ticket(SweepLds<Op, RING>& L
 // One strip of a pass: `order` is the strip's position in the pass (strip
 // order for forward passes, reversed for backward ones), `epoch` tags its
 // hand-off words.
-template <class Op, bool BACKWARD, int RING>
+template <class Op, bool BACKWARD, bool PREFETCH, int RING>
 __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& c, int order, unsigned epoch,
                                           SweepLds<Op, RING>& L) {
     constexpr int NW = Op::NW, NIN = Op::NIN, NF = Op::NF, NB = Op::NB;
@@ -389,9 +389,9 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
             }
         }
     };
-    // A loader's first batch goes out before the skip test, whose reads it
-    // overlaps (loads only: a skipped pass discards them).
-    if (wave >= 1 && wave <= kLoaders) load_batch((wave - 1) * kBatch);
+    // With PREFETCH a loader's first batch goes out before the skip test,
+    // whose reads it overlaps (loads only: a skipped pass discards them).
+    if (PREFETCH && wave >= 1 && wave <= kLoaders) load_batch((wave - 1) * kBatch);
     if (wave == 0) {
         int sk = __hip_atomic_load(c.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
         if (!sk) sk = op.skip(order == 0);
@@ -439,7 +439,7 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
-            if (sb != me * kBatch) load_batch(sb);  // the first batch is already loaded
+            if (!PREFETCH || sb != me * kBatch) load_batch(sb);  // with PREFETCH the first batch is loaded
 #pragma unroll
             for (int k = 0; k < kBatch; k++) {
                 if (sb + k < NSG) {
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(Op op, SkGeom g, S
     const unsigned long long ticket = sweep_ticket(L, c.ticket);
     const int order = (int)(ticket % (unsigned long long)g.S);
     const unsigned epoch = (unsigned)(ticket / (unsigned long long)g.S) + 1u;
-    sweep_run<Op, BACKWARD>(op, g, c, order, epoch, L);
+    sweep_run<Op, BACKWARD, true>(op, g, c, order, epoch, L);
 }
 
 // ---------------------------------------------------------------------------

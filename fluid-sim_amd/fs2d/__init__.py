@@ -50,6 +50,10 @@ class Stats(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("pcg_grid", ctypes.c_int), ("sweep_prefetch", ctypes.c_int)]
+
+
 class FS2DError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__("%s: %s" % (status_string(status), msg))
@@ -92,6 +96,14 @@ def load():
         "fs2d_get_stats": (i, [vp, ctypes.POINTER(Stats)]),
         "fs2d_avg_pressure": (i, [vp, i, ctypes.POINTER(d)]),
         "fs2d_max_velocity": (i, [vp, ctypes.POINTER(d)]),
+        "fs2d_set_graph": (i, [vp, i]),
+        "fs2d_get_tuning": (i, [vp, ctypes.POINTER(Tuning)]),
+        "fs2d_set_tuning": (i, [vp, ctypes.POINTER(Tuning)]),
+        "fs2d_autotune": (i, [vp, i, ctypes.c_char_p, ctypes.POINTER(Tuning), ctypes.POINTER(i)]),
+        "fs2d_batch_get_tuning": (i, [vp, ctypes.POINTER(Tuning)]),
+        "fs2d_batch_set_tuning": (i, [vp, ctypes.POINTER(Tuning)]),
+        "fs2d_batch_autotune": (i, [vp, i, ctypes.c_char_p, ctypes.POINTER(Tuning), ctypes.POINTER(i)]),
+        "fs2d_project_noalloc": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(d)]),
         "fs2d_set_timing": (i, [vp, i]),
         "fs2d_set_timing_classes": (i, [vp, ctypes.c_ulonglong]),
         "fs2d_kernel_count": (i, []),
@@ -105,6 +117,7 @@ def load():
         "fs2d_batch_sim": (vp, [vp, i]),
         "fs2d_batch_step": (i, [vp]),
         "fs2d_batch_run_stage": (i, [vp, i]),
+        "fs2d_batch_set_graph": (i, [vp, i]),
         "fs2d_batch_set_timing": (i, [vp, i]),
         "fs2d_batch_get_kernel_time": (i, [vp, i, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_longlong)]),
         "fs2d_batch_reset_kernel_times": (i, [vp]),
@@ -306,6 +319,27 @@ class Sim:
         self._check(self.L.fs2d_max_velocity(self.h, ctypes.byref(d)))
         return d.value
 
+    def set_graph(self, on):
+        """Step by replaying captured HIP graphs (fs2d_set_graph)."""
+        self._check(self.L.fs2d_set_graph(self.h, int(on)))
+
+    def tuning(self):
+        t = Tuning()
+        self._check(self.L.fs2d_get_tuning(self.h, ctypes.byref(t)))
+        return {"pcg_grid": t.pcg_grid, "sweep_prefetch": t.sweep_prefetch}
+
+    def set_tuning(self, pcg_grid=0, sweep_prefetch=0):
+        t = Tuning(pcg_grid, sweep_prefetch)
+        self._check(self.L.fs2d_set_tuning(self.h, ctypes.byref(t)))
+
+    def autotune(self, iters=24, cache=None):
+        """Pick the fastest launch shapes (fs2d_autotune); returns the choice
+        and whether it came from the cache file."""
+        t, fc = Tuning(), ctypes.c_int()
+        self._check(self.L.fs2d_autotune(self.h, iters, cache.encode() if cache else None, ctypes.byref(t),
+                                         ctypes.byref(fc)))
+        return {"pcg_grid": t.pcg_grid, "sweep_prefetch": t.sweep_prefetch, "from_cache": bool(fc.value)}
+
     def set_timing(self, on, only=None):
         """Per-kernel HIP-event timing; `only` names the kernel classes to time
         (each timed launch adds two events to the stream)."""
@@ -376,6 +410,25 @@ class Batch:
 
     def run_stage(self, stage, allow=(OK, WARN_MAXITER, WARN_CFL)):
         return self._check(self.L.fs2d_batch_run_stage(self.h, stage), allow)
+
+    def set_graph(self, on):
+        """Step by replaying captured HIP graphs (fs2d_batch_set_graph)."""
+        self._check(self.L.fs2d_batch_set_graph(self.h, int(on)))
+
+    def tuning(self):
+        t = Tuning()
+        self._check(self.L.fs2d_batch_get_tuning(self.h, ctypes.byref(t)))
+        return {"pcg_grid": t.pcg_grid, "sweep_prefetch": t.sweep_prefetch}
+
+    def set_tuning(self, pcg_grid, sweep_prefetch):
+        t = Tuning(pcg_grid, sweep_prefetch)
+        self._check(self.L.fs2d_batch_set_tuning(self.h, ctypes.byref(t)))
+
+    def autotune(self, iters=24, cache=None):
+        t, fc = Tuning(), ctypes.c_int()
+        self._check(self.L.fs2d_batch_autotune(self.h, iters, cache.encode() if cache else None, ctypes.byref(t),
+                                               ctypes.byref(fc)))
+        return {"pcg_grid": t.pcg_grid, "sweep_prefetch": t.sweep_prefetch, "from_cache": bool(fc.value)}
 
     def set_timing(self, on):
         self._check(self.L.fs2d_batch_set_timing(self.h, int(on)))

@@ -197,6 +197,18 @@ fs2d_status fs2d_get_stats(fs2d_sim* sim, fs2d_stats* out);
 fs2d_status fs2d_avg_pressure(fs2d_sim* sim, int fluid_only, double* out);
 fs2d_status fs2d_max_velocity(fs2d_sim* sim, double* out);
 
+/* Step by replaying captured HIP graphs (no reference counterpart; runFrame,
+ * src/FluidSim2D.cpp:94-138, is what is captured).  fs2d_step then replays
+ * one graph for the stages up to a fixed number of PCG iterations (kernels
+ * of iterations after convergence return at once), checks convergence once on
+ * the host, continues the solve with queued chunks if the budget fell short,
+ * and replays a second graph for the rest of the step.  The budget follows
+ * the previous solve's iteration count; a step is captured again when a
+ * parameter, the particle count, the budget or a buffer it uses changed (at
+ * most four captures are kept).  Results are bit-identical to the uncaptured
+ * step.  Timing (fs2d_set_timing) runs uncaptured steps. */
+fs2d_status fs2d_set_graph(fs2d_sim* sim, int enable);
+
 /* Per-kernel timing with HIP events on the handle's stream (no reference
  * counterpart; PerformanceCounter, src/PerformanceCounter.cpp:9-74, timed
  * stages with a host clock).  Kernel classes are listed by fs2d_kernel_name. */
@@ -213,6 +225,32 @@ fs2d_status fs2d_reset_kernel_times(fs2d_sim* sim);
 
 const char* fs2d_last_error(const fs2d_sim* sim);
 const char* fs2d_status_string(int status);
+
+/*
+ * Launch shapes that change no result, and an autotuner for them.  No
+ * reference counterpart (its PCG loop, src/FluidSim2D.cpp:431-463, is what
+ * they shape).
+ *   pcg_grid: physical workgroups of the PCG SpMV and axpy launches, a
+ *     multiple of 8; 0 (single handles only) launches one per partial sum.
+ *     The loops run a fixed list of virtual workgroups whatever the launch
+ *     size, so every value gives the same bits.
+ *   sweep_prefetch: (ensembles) the batched triangular solves' loaders issue
+ *     their first loads before testing whether their member has converged.
+ * fs2d_autotune / fs2d_batch_autotune time the candidates on the handle's
+ * current state (call them after a first step): `iters` PCG iterations of the
+ * projection each (tolerance 0), best of three after a warm-up; the fastest is
+ * set and appended to the text file `cache_path` (NULL: none) under a key of
+ * the device and the problem, from which later calls with the same key take it
+ * without timing (*from_cache = 1).  Tuning overwrites the pressure and the
+ * solver's scratch, not the state a step starts from.
+ */
+typedef struct {
+    int pcg_grid;
+    int sweep_prefetch;
+} fs2d_tuning;
+fs2d_status fs2d_get_tuning(fs2d_sim* sim, fs2d_tuning* out);
+fs2d_status fs2d_set_tuning(fs2d_sim* sim, const fs2d_tuning* t);
+fs2d_status fs2d_autotune(fs2d_sim* sim, int iters, const char* cache_path, fs2d_tuning* out, int* from_cache);
 
 /*
  * applyProjection without allocation.  No reference counterpart: it is for
@@ -278,6 +316,17 @@ fs2d_sim* fs2d_batch_sim(fs2d_batch* batch, int i);
 fs2d_status fs2d_batch_step(fs2d_batch* batch);
 /* One stage method (FluidSim2D.h:124-140) on every member. */
 fs2d_status fs2d_batch_run_stage(fs2d_batch* batch, int stage);
+/* Step by replaying captured HIP graphs, as fs2d_set_graph does for one
+ * simulation: every member's stages and PCG iterations up to a fixed budget in
+ * one graph (converged members' work returns at once on the device), one host
+ * check of convergence, and the rest of the step in a second graph.  Results
+ * are bit-identical to the uncaptured batch step. */
+fs2d_status fs2d_batch_set_graph(fs2d_batch* batch, int enable);
+/* Tuning of the batched launches (see fs2d_tuning). */
+fs2d_status fs2d_batch_get_tuning(fs2d_batch* batch, fs2d_tuning* out);
+fs2d_status fs2d_batch_set_tuning(fs2d_batch* batch, const fs2d_tuning* t);
+fs2d_status fs2d_batch_autotune(fs2d_batch* batch, int iters, const char* cache_path, fs2d_tuning* out,
+                                int* from_cache);
 /* Per-kernel timing of the batched launches (kernel classes of fs2d_kernel_name). */
 fs2d_status fs2d_batch_set_timing(fs2d_batch* batch, int enable);
 fs2d_status fs2d_batch_get_kernel_time(fs2d_batch* batch, int kernel, double* total_ms, long long* launches);
