@@ -217,12 +217,14 @@ fs2d_status adjoint_step_impl(fs2d_adjoint* a) {
         Timed t(s, K_ADJ_OTHER);
         hipLaunchKernelGGL(adj_extrap_kernel, dim3(2), dim3(1024), 0, s->stream, a->ex[0], a->ex[1]);
     }
-    // pressure: p-bar, then lam = A^-1 p-bar with the forward solver (A = A^T)
+    // pressure: p-bar, then lam = A^-1 p-bar with the forward solver (A = A^T);
+    // the solver holds A / scaleA, so its rhs is p-bar / scaleA:
+    // (dt / (rho dx)) / (dt / (rho dx^2)) = dx
     assemble(s);
     {
         Timed t(s, K_ADJ_OTHER);
         hipLaunchKernelGGL(adj_pressure_rhs_kernel, dim3(sk_blocks(s)), dim3(kBlock), 0, s->stream, s->geom, s->cell,
-                           a->gu, a->gv, s->r, s->pk, s->s[0], s->dt / (s->rho * s->dx), s->rmax_part);
+                           a->gu, a->gv, s->r, s->pk, s->s[0], s->dx, s->rmax_part);
     }
     fs2d_status st = pcg_solve(s, a->lam);
     if (st < 0) return st;

@@ -29,6 +29,9 @@ struct InstBufs {
     long np;
     double dt, dx, dr, rho, gx, gy, alpha, tau, sigma;
     double pmass;  // rho dx^2 / k^2 (0 without particles)
+    // the solver's rhs factor 1 / (dx scaleA) and |rhs|_inf zero threshold in
+    // solver units, computed on the host as the single handle computes them
+    double rhs_scale, rhs_zero;
     int max_iters;
     SkGeom geom;
     float *u, *v, *u2, *v2, *nu, *nv, *dens, *dens2;
@@ -140,8 +143,8 @@ __global__ __launch_bounds__(kBlock) void b_assemble_kernel(const InstBufs* __re
     int b, vb, nvb;
     if (!bmap(bm, b, vb, nvb)) return;
     const InstBufs& I = ins[b];
-    const double sA = I.dt / (I.rho * I.dx * I.dx);
-    assemble_sk_body(I.geom, I.cell, I.phi, I.u, I.v, I.Adiag, I.cpl, I.r, I.pk, I.s0, sA, 1.0 / I.dx,
+    // solver units, A / scaleA and rhs / scaleA (as the single path's assemble)
+    assemble_sk_body(I.geom, I.cell, I.phi, I.u, I.v, I.Adiag, I.cpl, I.r, I.pk, I.s0, I.rhs_scale,
                      I.rmax_part, (I.flags & 8) ? 0 : 1 /* FS2D_FLAG_NO_GHOST_PRESSURE */, vb, nvb);
 }
 
@@ -241,7 +244,6 @@ __global__ __launch_bounds__(kBlock) void b_spmv_kernel(const InstBufs* __restri
         const InstBufs& I = ins[b];
         const float* s_old = (h & 1) ? I.s1 : I.s0;
         float* s_new = (h & 1) ? I.s0 : I.s1;
-        const float sA = (float)(I.dt / (I.rho * I.dx * I.dx));
         const SpmvPre pre = spmv_pre(I.geom, I.Adiag, I.cpl, I.z, s_old, vb, nvb);
         if (b != cached) {
             const double* qq0 = I.qq;
@@ -250,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void b_spmv_kernel(const InstBufs* __restri
             cached = b;
         }
         if (vb == 0 && threadIdx.x == 0) I.sc->sigma = S.sig;
-        spmv_core(pre, I.geom, I.Adiag, I.cpl, I.z, s_old, s_new, I.w, sA, S.beta, I.ws_part, vb);
+        spmv_core(pre, I.geom, I.Adiag, I.cpl, I.z, s_old, s_new, I.w, S.beta, I.ws_part, vb);
     }
 }
 // w.s of every member still iterating, summed once per member (one wave per
@@ -403,7 +405,8 @@ __global__ __launch_bounds__(kBlock) void b_particle_advect_kernel(const InstBuf
     if (!bmap(bm, b, vb, nvb)) return;
     const InstBufs& I = ins[b];
     particle_advect_body(I.px, I.py, I.pvx, I.pvy, I.np, MacView{I.u, I.v, I.nx, I.ny, I.dx}, I.dt,
-                         clamp_box(I.nx, I.ny, I.dx), I.dg, I.partials, vb, nvb);
+                         clamp_box(I.nx, I.ny, I.dx), I.dg, I.partials, (I.flags & 32) ? 1 : 0 /* FS2D_FLAG_RK2 */,
+                         vb, nvb);
 }
 // One workgroup per simulation; `bm` is the map of the launch that wrote the partials.
 __global__ __launch_bounds__(kBlock) void b_cfl_finalize_kernel(const InstBufs* __restrict__ ins, int n, BlockMap bm) {
@@ -507,9 +510,8 @@ __global__ __launch_bounds__(kBlock) void b_pc_fill_kernel(const InstBufs* __res
 struct FactorMaker {
     const int* nparts;  // workgroups of the SK launches per simulation (the rhs partials)
     __device__ FactorOp operator()(const InstBufs& I, int b) const {
-        const double sA = I.dt / (I.rho * I.dx * I.dx);
-        return FactorOp{{I.Adiag}, I.cpl, {I.P, I.fx, I.fy, I.gx_, I.gy_}, I.tau, I.sigma, sA, I.rmax_part,
-                        nparts[b], I.max_iters, I.sc};
+        return FactorOp{{I.Adiag}, I.cpl, {I.P, I.fx, I.fy, I.gx_, I.gy_}, I.tau, I.sigma, I.rmax_part,
+                        nparts[b], I.max_iters, I.sc, I.rhs_zero};
     }
 };
 // The eikonal passes on the sweep engine (EikSweepOp): conversions between a

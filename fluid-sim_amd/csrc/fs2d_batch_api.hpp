@@ -106,6 +106,8 @@ InstBufs inst_bufs(const fs2d_sim* s, unsigned long long* sk_handoff) {
     I.dt = s->dt; I.dx = s->dx; I.dr = s->dr; I.rho = s->rho; I.gx = s->gx; I.gy = s->gy; I.alpha = s->alpha;
     I.tau = s->tau; I.sigma = s->sigma; I.max_iters = s->max_iters;
     I.pmass = s->k > 0 ? s->rho * s->dx * s->dx / (s->k * s->k) : 0.0;
+    I.rhs_scale = 1.0 / (s->dx * solver_scale(s));
+    I.rhs_zero = 1e-12 / solver_scale(s);
     I.geom = s->geom;
     I.u = s->u; I.v = s->v; I.u2 = s->u2; I.v2 = s->v2; I.nu = s->nu; I.nv = s->nv; I.dens = s->dens; I.dens2 = s->dens2;
     I.cell = s->cell; I.cell2 = s->cell2; I.phi = s->phi; I.phi2 = s->phi2; I.t0 = s->t0; I.t1 = s->t1;

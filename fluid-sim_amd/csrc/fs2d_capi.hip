@@ -507,13 +507,25 @@ int pcg_blocks(const fs2d_sim* s) {
     return std::max(std::max(8, want), std::min(sk_blocks(s), 256));
 }
 
-// Ghost-fluid matrix and rhs of the current velocity (src/FluidSim2D.cpp:252-362).
+// The solver works on A / scaleA and rhs / scaleA, scaleA = dt / (rho dx^2):
+// the same solution, the same iterates in exact arithmetic (every CG scalar
+// and the relative stopping test are scale-free), but in FP32 the
+// off-diagonals are exactly -1 and an interior cell's diagonal exactly its
+// count of FLUID neighbours.  With scaleA folded into FP32 coefficients, the
+// cells next to walls (3 * scaleA) round differently from interior ones
+// (4 * scaleA, exact): a perturbation of one sign along every wall, which at
+// 512^2 moved the pressure's smoothest mode by 2e-5 of the solution.
+// (dt = 0 makes the reference's matrix zero; its velocity update then adds
+// nothing whatever p is, and a unit scale keeps p finite.)
+double solver_scale(const fs2d_sim* s) { return s->dt > 0 ? s->dt / (s->rho * s->dx * s->dx) : 1.0; }
+
+// Ghost-fluid matrix and rhs of the current velocity (src/FluidSim2D.cpp:252-362),
+// in solver units.
 void assemble(fs2d_sim* s) {
     Timed t(s, K_ASSEMBLE);
-    const double sA = s->dt / (s->rho * s->dx * s->dx);
     hipLaunchKernelGGL(assemble_sk_kernel, dim3(sk_blocks(s)), dim3(kBlock), 0, s->stream, s->geom, s->cell, s->phi,
-                       s->u, s->v, s->Adiag, s->cpl, s->r, s->pk, s->s[0], sA, 1.0 / s->dx, s->rmax_part,
-                       ghost_pressure(s));
+                       s->u, s->v, s->Adiag, s->cpl, s->r, s->pk, s->s[0], 1.0 / (s->dx * solver_scale(s)),
+                       s->rmax_part, ghost_pressure(s));
 }
 
 // Physical workgroups of the PCG SpMV and axpy launches: their loops run the
@@ -529,12 +541,12 @@ int pcg_launch(const fs2d_sim* s) {
 // p = s = 0 and the |rhs|_inf partials in rmax_part).
 void pcg_begin(fs2d_sim* s) {
     const int nb = sk_blocks(s);
-    const double sA = s->dt / (s->rho * s->dx * s->dx);
     hipLaunchKernelGGL(pcg_init_kernel, dim3(1), dim3(64), 0, s->stream, s->sc, s->tol);
     {
         Timed t(s, K_FACTOR);
-        FactorOp op{{s->Adiag}, s->cpl, {s->P, s->fx, s->fy, s->gx_, s->gy_}, s->tau, s->sigma, sA, s->rmax_part, nb,
-                    s->max_iters, s->sc};
+        // the reference skips the solve when |rhs|_inf <= 1e-12 (:447-448); r holds rhs / scaleA
+        FactorOp op{{s->Adiag}, s->cpl, {s->P, s->fx, s->fy, s->gx_, s->gy_}, s->tau, s->sigma, s->rmax_part, nb,
+                    s->max_iters, s->sc, 1e-12 / solver_scale(s)};
         launch_sweep<FactorOp, false>(s, op);
     }
     {
@@ -555,7 +567,6 @@ void pcg_begin(fs2d_sim* s) {
 // queued (or captured) without host round trips.
 void pcg_iterations(fs2d_sim* s, int h0, int h1) {
     const SkGeom& g = s->geom;
-    const double sA = s->dt / (s->rho * s->dx * s->dx);
     double* qq0 = s->qq;
     double* qq1 = s->qq + (size_t)g.S * kLoaders;
     const int nbp = pcg_blocks(s), grid = pcg_launch(s);
@@ -567,7 +578,7 @@ void pcg_iterations(fs2d_sim* s, int h0, int h1) {
         {
             Timed t(s, K_PCG_SPMV);
             hipLaunchKernelGGL(pcg_spmv_sk_kernel, dim3(grid), dim3(kBlock), 0, s->stream, g, s->Adiag, s->cpl, s->z,
-                               s_old, s_new, s->w, (float)sA, s->sc, sig_cur, sig_prev, h == 0 ? 1 : 0, s->ws_part,
+                               s_old, s_new, s->w, s->sc, sig_cur, sig_prev, h == 0 ? 1 : 0, s->ws_part,
                                nbp);
         }
         {
@@ -655,7 +666,8 @@ void stage_particle_advect(fs2d_sim* s) {
     Timed t(s, K_PARTICLE_ADVECT);
     const int nb = blocks_for(s->np);
     hipLaunchKernelGGL(particle_advect_kernel, dim3(nb), dim3(kBlock), 0, s->stream, s->px, s->py, s->pvx, s->pvy,
-                       s->np, mac_view(s), s->dt, clamp_box(s->nx, s->ny, s->dx), s->dg, s->partials);
+                       s->np, mac_view(s), s->dt, clamp_box(s->nx, s->ny, s->dx), s->dg, s->partials,
+                       (s->flags & FS2D_FLAG_RK2) ? 1 : 0);
     hipLaunchKernelGGL(cfl_finalize_kernel, dim3(1), dim3(kBlock), 0, s->stream, s->partials, nb, s->dg);
     s->sorted_valid = false;
 }
@@ -891,7 +903,7 @@ static fs2d_status validate(const fs2d_config* c, char* why, size_t n) {
         return FS2D_ERR_CONFIG;
     }
     if (c->flags & ~(FS2D_FLAG_STATIC_CELLS | FS2D_FLAG_DENSITY | FS2D_FLAG_PARTICLE_CELLS |
-                     FS2D_FLAG_NO_GHOST_PRESSURE)) {
+                     FS2D_FLAG_NO_GHOST_PRESSURE | FS2D_FLAG_UNIFORM_SEEDING | FS2D_FLAG_RK2)) {
         snprintf(why, n, "unknown flags 0x%x", c->flags);
         return FS2D_ERR_CONFIG;
     }
@@ -1004,7 +1016,9 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
     // jittered samples per FLUID cell in row-major cell order, x jitter drawn
     // before y jitter, with glibc's rand() stream (randf, :19-21).  random_r
     // with a private 128-byte state is that stream without touching the
-    // process-wide generator.
+    // process-wide generator.  FS2D_FLAG_UNIFORM_SEEDING draws each sample
+    // uniformly over the whole cell instead: ((i + U) dx, (j + U) dx),
+    // U = rand()/RAND_MAX, x before y.
     std::vector<float> hx, hy;
     size_t fluid = 0;
     {
@@ -1025,6 +1039,13 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
                 if (cfg->initial_values[(size_t)j * s->nx + i] == FS2D_FLUID) {
                     fluid++;
                     for (int e2 = 0; e2 < ppc; e2++) {
+                        if (cfg->flags & FS2D_FLAG_UNIFORM_SEEDING) {
+                            const double ux = randf(1.0f);
+                            const double uy = randf(1.0f);
+                            hx.push_back((float)(((double)i + ux) * s->dx));
+                            hy.push_back((float)(((double)j + uy) * s->dx));
+                            continue;
+                        }
                         const int x = e2 % k, y = e2 / k;
                         const double jx = randf(dist);
                         const double jy = randf(dist);

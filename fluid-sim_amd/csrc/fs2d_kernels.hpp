@@ -195,15 +195,17 @@ __device__ __forceinline__ bool sk_cell(const SkGeom& g, long e, int& i, int& j)
     return i >= 0 && i < g.nx && j < g.ny;
 }
 
-// Ghost-fluid matrix (:262-303) and rhs (:338-362) written in SK4 order: the
-// diagonal, and the coupling bits of each FLUID cell (its off-diagonals are
-// -scaleA towards FLUID neighbours).  The PCG vectors p and s start at zero
-// (:424).  Partial |rhs|_inf per block.
+// Ghost-fluid matrix (:262-303) and rhs (:338-362) written in SK4 order, in
+// solver units (the reference's matrix and rhs divided by scaleA = dt / (rho
+// dx^2)): the diagonal, and the coupling bits of each FLUID cell (its
+// off-diagonals are -1 towards FLUID neighbours).  `scale` is the rhs factor
+// 1 / (dx scaleA).  The PCG vectors p and s start at zero (:424).  Partial
+// |rhs|_inf per block.
 __device__ __forceinline__ void assemble_sk_body(
     SkGeom g, const uint8_t* __restrict__ cell, const double* __restrict__ phi, const float* __restrict__ u,
     const float* __restrict__ v, float* __restrict__ Adiag, uint8_t* __restrict__ flags, float* __restrict__ r,
-    float* __restrict__ p, float* __restrict__ s0, double scaleA, double scale, double* partials, int ghost, int vb,
-    int nvb) {
+    float* __restrict__ p, float* __restrict__ s0, double scale, double* partials, int ghost, int vb, int nvb) {
+    constexpr double scaleA = 1.0;
     const int nx = g.nx;
     double m = 0;
     for (long e = vb * (long)blockDim.x + threadIdx.x; e < g.size(); e += (long)nvb * blockDim.x) {
@@ -257,9 +259,8 @@ __device__ __forceinline__ void assemble_sk_body(
 __global__ __launch_bounds__(kBlock) void assemble_sk_kernel(
     SkGeom g, const uint8_t* __restrict__ cell, const double* __restrict__ phi, const float* __restrict__ u,
     const float* __restrict__ v, float* __restrict__ Adiag, uint8_t* __restrict__ flags, float* __restrict__ r,
-    float* __restrict__ p, float* __restrict__ s0, double scaleA, double scale, double* partials, int ghost) {
-    assemble_sk_body(g, cell, phi, u, v, Adiag, flags, r, p, s0, scaleA, scale, partials, ghost, blockIdx.x,
-                     gridDim.x);
+    float* __restrict__ p, float* __restrict__ s0, double scale, double* partials, int ghost) {
+    assemble_sk_body(g, cell, phi, u, v, Adiag, flags, r, p, s0, scale, partials, ghost, blockIdx.x, gridDim.x);
 }
 
 // Sum of a small partial array by one wave, broadcast to every thread of the
@@ -341,10 +342,13 @@ __device__ __forceinline__ SpmvScalars spmv_scalars(const SkGeom& g, const doubl
     r.beta = (float)beta_d;
     return r;
 }
+// The matrix is in solver units (off-diagonals -1): w = ad s' - sum of the
+// coupled neighbours' s', with one rounding (an explicit fma, so that no
+// launch path can contract it differently).
 __device__ __forceinline__ void spmv_core(const SpmvPre& pre, const SkGeom& g, const float* __restrict__ Adiag,
                                           const uint8_t* __restrict__ flags, const float* __restrict__ z,
                                           const float* __restrict__ s_old, float* __restrict__ s_new,
-                                          float* __restrict__ w, float sA, float beta, double* partials, int vb) {
+                                          float* __restrict__ w, float beta, double* partials, int vb) {
     const XcdLoop it = pre.it;
     long e = it.e;
     float so = pre.so, zc = pre.zc, ad = pre.ad;
@@ -363,7 +367,7 @@ __device__ __forceinline__ void spmv_core(const SpmvPre& pre, const SkGeom& g, c
             if (f & CPL_R) { const long n = sk_step_fwd(e); nsum += fmaf(beta, s_old[n], z[n]); }
             if (f & CPL_D) nsum += fmaf(beta, s_old[nd], z[nd]);
             if (f & CPL_U) nsum += fmaf(beta, s_old[nu], z[nu]);
-            wv = fmaf(-sA, nsum, ad * sc_);
+            wv = fmaf(ad, sc_, -nsum);
         }
         w[e] = wv;
         acc += (double)wv * (double)sc_;
@@ -386,7 +390,7 @@ __device__ __forceinline__ void spmv_core(const SpmvPre& pre, const SkGeom& g, c
 // keeps vb % 8 the XCD (xcd_loop).  sigma goes to the scalars (vb 0).
 __global__ __launch_bounds__(kBlock) void pcg_spmv_sk_kernel(
     SkGeom g, const float* __restrict__ Adiag, const uint8_t* __restrict__ flags, const float* __restrict__ z,
-    const float* __restrict__ s_old, float* __restrict__ s_new, float* __restrict__ w, float sA,
+    const float* __restrict__ s_old, float* __restrict__ s_new, float* __restrict__ w,
     PcgScalars* sc, const double* sig_cur, const double* sig_prev, int first, double* partials, int nvb) {
     if (sc->done) return;
     int vb = blockIdx.x;
@@ -395,7 +399,7 @@ __global__ __launch_bounds__(kBlock) void pcg_spmv_sk_kernel(
     const SpmvScalars S = spmv_scalars(g, sig_cur, sig_prev, first);
     if (vb == 0 && threadIdx.x == 0) sc->sigma = S.sig;
     for (;;) {
-        spmv_core(pre, g, Adiag, flags, z, s_old, s_new, w, sA, S.beta, partials, vb);
+        spmv_core(pre, g, Adiag, flags, z, s_old, s_new, w, S.beta, partials, vb);
         vb += gridDim.x;
         if (vb >= nvb) break;
         pre = spmv_pre(g, Adiag, flags, z, s_old, vb, nvb);
@@ -1228,10 +1232,13 @@ __global__ __launch_bounds__(kBlock) void g2p_kernel(const float* __restrict__ p
 // FluidSim2D::applyAdvection (src/FluidSim2D.cpp:570-604): CFL scan, forward
 // Ralston RK3 through the Catmull-Rom field, clampPos, NaN flag (on the
 // position before the move, as the reference checks it).
+// rk2: the midpoint rule x + dt v(x + dt/2 v(x)) instead of the reference's
+// RK3 (the 512^2 PIC/FLIP dam-break workload's option).
 __device__ __forceinline__ void particle_advect_body(float* __restrict__ px, float* __restrict__ py,
                                                                  const float* __restrict__ pvx,
                                                                  const float* __restrict__ pvy, long np, MacView m,
-                                                                 double dt, ClampBox cb, Diag* dg, double* partials, int vb, int nvb) {
+                                                                 double dt, ClampBox cb, Diag* dg, double* partials,
+                                                                 int rk2, int vb, int nvb) {
     double cmax = 0;
     int nan = 0;
     for (long e = vb * (long)blockDim.x + threadIdx.x; e < np; e += (long)nvb * blockDim.x) {
@@ -1241,13 +1248,20 @@ __device__ __forceinline__ void particle_advect_body(float* __restrict__ px, flo
         const double k1x = vel_interp_u(m, x, y), k1y = vel_interp_v(m, x, y);
         const double x2 = x + 0.5 * dt * k1x, y2 = y + 0.5 * dt * k1y;
         const double k2x = vel_interp_u(m, x2, y2), k2y = vel_interp_v(m, x2, y2);
-        const double x3 = x + 0.75 * dt * k2x, y3 = y + 0.75 * dt * k2y;
-        const double k3x = vel_interp_u(m, x3, y3), k3y = vel_interp_v(m, x3, y3);
-        double nx_ = x + (2. / 9.) * dt * k1x, ny_ = y + (2. / 9.) * dt * k1y;
-        nx_ = nx_ + (3. / 9.) * dt * k2x;
-        ny_ = ny_ + (3. / 9.) * dt * k2y;
-        nx_ = nx_ + (4. / 9.) * dt * k3x;
-        ny_ = ny_ + (4. / 9.) * dt * k3y;
+        double nx_, ny_;
+        if (rk2) {
+            nx_ = x + dt * k2x;
+            ny_ = y + dt * k2y;
+        } else {
+            const double x3 = x + 0.75 * dt * k2x, y3 = y + 0.75 * dt * k2y;
+            const double k3x = vel_interp_u(m, x3, y3), k3y = vel_interp_v(m, x3, y3);
+            nx_ = x + (2. / 9.) * dt * k1x;
+            ny_ = y + (2. / 9.) * dt * k1y;
+            nx_ = nx_ + (3. / 9.) * dt * k2x;
+            ny_ = ny_ + (3. / 9.) * dt * k2y;
+            nx_ = nx_ + (4. / 9.) * dt * k3x;
+            ny_ = ny_ + (4. / 9.) * dt * k3y;
+        }
         if (isnan(x) || isnan(y)) nan = 1;
         px[e] = (float)clampd(nx_, cb.lox, cb.hix);
         py[e] = (float)clampd(ny_, cb.loy, cb.hiy);
@@ -1266,8 +1280,9 @@ __device__ __forceinline__ void particle_advect_body(float* __restrict__ px, flo
 __global__ __launch_bounds__(kBlock) void particle_advect_kernel(float* __restrict__ px, float* __restrict__ py,
                                                                  const float* __restrict__ pvx,
                                                                  const float* __restrict__ pvy, long np, MacView m,
-                                                                 double dt, ClampBox cb, Diag* dg, double* partials) {
-    particle_advect_body(px, py, pvx, pvy, np, m, dt, cb, dg, partials, blockIdx.x, gridDim.x);
+                                                                 double dt, ClampBox cb, Diag* dg, double* partials,
+                                                                 int rk2) {
+    particle_advect_body(px, py, pvx, pvy, np, m, dt, cb, dg, partials, rk2, blockIdx.x, gridDim.x);
 }
 
 // Max of the per-block CFL partials (one block).

@@ -583,22 +583,25 @@ __device__ __forceinline__ double wave_sum_of(const double* a, int n) {
     return t;
 }
 
-// MIC(0) factor (:364-388) and the solve coefficients.  State: precon (FP64)
-// and the cell's coupling bits; the neighbours' Ax, Ay are -scaleA times their
-// bits.  Outputs (Lt = diag(1/P) + lower(A) diag(P), M = Lt Lt^T):
-//   P,  fx = scaleA c_L P P_L,  fy = scaleA c_D P P_D   (q = P r + fx q_L + fy q_D)
-//       gx = scaleA c_R P^2,    gy = scaleA c_U P^2      (z = P q + gx z_R + gy z_U)
-// The first workgroup also finishes |rhs|_inf and initialises the solver
-// (:424-430, 447-448).
+// MIC(0) factor (:364-388) and the solve coefficients, in solver units (the
+// matrix divided by scaleA: the neighbours' Ax, Ay are -1 times their coupling
+// bits).  State: precon (FP64) and the cell's coupling bits.  Outputs (Lt =
+// diag(1/P) + lower(A) diag(P), M = Lt Lt^T):
+//   P,  fx = c_L P P_L,  fy = c_D P P_D   (q = P r + fx q_L + fy q_D)
+//       gx = c_R P^2,    gy = c_U P^2      (z = P q + gx z_R + gy z_U)
+// Products are rounded one by one, as the reference's expression reads
+// (no contraction, the same in every launch path).  The first workgroup also
+// finishes |rhs|_inf and initialises the solver (:424-430, 447-448).
 struct FactorOp {
     static constexpr int NW = 3, NIN = 1, NF = 1, NB = 1, NOUT = 5;
     const float* in[NIN];  // Adiag
     const uint8_t* bytes;  // coupling bits
     float* out[NOUT];      // P, fx, fy, gx, gy
-    double tau, sigma, sA;
+    double tau, sigma;
     const double* rhs_partials;
     int n_partials, max_iters;
     PcgScalars* sc;
+    double rhs_zero;  // |rhs|_inf at or below which no iteration runs (1e-12, :448, in the rhs' units)
     __device__ bool skip(bool first) const {
         if (first) {
             const double rn = wave_max_of(rhs_partials, n_partials);
@@ -607,7 +610,7 @@ struct FactorOp {
                 sc->r_norm = rn;
                 sc->iter = 0;
                 sc->sigma = 0;
-                sc->done = (rn <= 1e-12) || (max_iters <= 0);
+                sc->done = (rn <= rhs_zero) || (max_iters <= 0);
             }
         }
         return false;
@@ -617,6 +620,7 @@ struct FactorOp {
     template <bool B, bool C>
     __device__ void group(const float4 (&x)[NF], uint32_t b4, uint32_t (&st)[NW], const uint4 (&hb)[NW],
                           float (&o)[NOUT][4], uint32_t (&prod)[NW][4], int, int) const {
+#pragma clang fp contract(off)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             uint32_t nb[NW];
@@ -627,8 +631,8 @@ struct FactorOp {
             const double PL = u2d(st), PB = u2d(nb);
             const uint32_t fL = st[2], fB = nb[2];
             // Ax(i-1,j), Ay(i-1,j), Ay(i,j-1), Ax(i,j-1)
-            const double axl = (f & CPL_L) ? -sA : 0.0, ayl = (fL & CPL_U) ? -sA : 0.0;
-            const double ayb = (f & CPL_D) ? -sA : 0.0, axb = (fB & CPL_R) ? -sA : 0.0;
+            const double axl = (f & CPL_L) ? -1.0 : 0.0, ayl = (fL & CPL_U) ? -1.0 : 0.0;
+            const double ayb = (f & CPL_D) ? -1.0 : 0.0, axb = (fB & CPL_R) ? -1.0 : 0.0;
             double Pc = 0;
             if (A0 != 0.0) {  // FLUID cells have a non-zero diagonal, all others zero
                 const double t1 = axl * PL, t2 = ayb * PB;
@@ -639,8 +643,8 @@ struct FactorOp {
             o[0][q] = (float)Pc;
             o[1][q] = (float)(-axl * Pc * PL);
             o[2][q] = (float)(-ayb * Pc * PB);
-            o[3][q] = (f & CPL_R) ? (float)(sA * Pc * Pc) : 0.0f;
-            o[4][q] = (f & CPL_U) ? (float)(sA * Pc * Pc) : 0.0f;
+            o[3][q] = (f & CPL_R) ? (float)(Pc * Pc) : 0.0f;
+            o[4][q] = (f & CPL_U) ? (float)(Pc * Pc) : 0.0f;
             d2u(Pc, st);
             st[2] = f;
 #pragma unroll

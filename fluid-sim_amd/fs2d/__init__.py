@@ -25,6 +25,7 @@ EMPTY, FLUID, SOLID = 0, 1, 2
  F_PARTICLE_POS, F_PARTICLE_VEL, F_PARTICLE_CELL, F_PARTICLE_ORDER) = range(12)
 F32, F64, U8, I32 = 0, 1, 2, 3
 FLAG_STATIC_CELLS, FLAG_DENSITY, FLAG_PARTICLE_CELLS, FLAG_NO_GHOST_PRESSURE = 1, 2, 4, 8
+FLAG_UNIFORM_SEEDING, FLAG_RK2 = 16, 32
 
 
 class Config(ctypes.Structure):
@@ -183,7 +184,8 @@ _SHAPES = {
 
 def make_config(cells, k=2, dt=0.005, dx=0.01, rho=997.0, gravity=(0.0, -9.81), mode=PICFLIP,
                 alpha=1.0, tau=0.999, sigma=0.25, tol=1e-12, max_iters=200, seed=1,
-                static_cells=False, density=False, particle_cells=False, ghost_pressure=True):
+                static_cells=False, density=False, particle_cells=False, ghost_pressure=True,
+                uniform_seeding=False, rk2=False):
     """A Config for `cells` (uint8, row-major [j, i]); the caller keeps `cells`
     alive until the handle is created."""
     c = default_config()
@@ -196,7 +198,8 @@ def make_config(cells, k=2, dt=0.005, dx=0.01, rho=997.0, gravity=(0.0, -9.81), 
     c.tau, c.sigma, c.pcg_tol, c.pcg_max_iters = tau, sigma, tol, max_iters
     c.seed = seed
     c.flags = ((FLAG_STATIC_CELLS if static_cells else 0) | (FLAG_DENSITY if density else 0) |
-               (FLAG_PARTICLE_CELLS if particle_cells else 0) | (0 if ghost_pressure else FLAG_NO_GHOST_PRESSURE))
+               (FLAG_PARTICLE_CELLS if particle_cells else 0) | (0 if ghost_pressure else FLAG_NO_GHOST_PRESSURE) |
+               (FLAG_UNIFORM_SEEDING if uniform_seeding else 0) | (FLAG_RK2 if rk2 else 0))
     return c
 
 

@@ -100,7 +100,14 @@ enum {
     FS2D_FLAG_DENSITY = 2,          /* advect a passive cell-centred density in the SL stage */
     FS2D_FLAG_PARTICLE_CELLS = 4,   /* USE_LEVEL_SET undefined: cells from particle positions and a
                                        bubble fill (FluidSim2D.cpp:665-706); needs NO_GHOST_PRESSURE */
-    FS2D_FLAG_NO_GHOST_PRESSURE = 8 /* USE_GHOST_PRESSURE undefined (FluidSim2D.cpp:304-331, 486-530) */
+    FS2D_FLAG_NO_GHOST_PRESSURE = 8, /* USE_GHOST_PRESSURE undefined (FluidSim2D.cpp:304-331, 486-530) */
+    /* Options of the 512^2 PIC/FLIP dam-break workload, no reference counterpart:
+     * seed each of the k^2 particles of a FLUID cell uniformly over the whole cell
+     * (the reference jitters one per sub-cell, FluidSim2D.cpp:52-64), and advect
+     * particles by the midpoint rule (RK2) instead of applyAdvection's RK3
+     * (FluidSim2D.cpp:588-603). */
+    FS2D_FLAG_UNIFORM_SEEDING = 16,
+    FS2D_FLAG_RK2 = 32
 };
 
 /*

@@ -231,6 +231,11 @@ struct Sim {
     // set (USE_LEVEL_SET undefined); ghost = 0 drops the ghost-fluid pressure
     // terms (USE_GHOST_PRESSURE undefined).
     int particleCells = 0, ghost = 1;
+    // Options of the 512^2 PIC/FLIP dam-break workload that the reference
+    // does not have: particles advected by the midpoint rule (RK2) instead of
+    // applyAdvection's RK3 (:588-603), and k^2 particles per FLUID cell drawn
+    // uniformly over the whole cell instead of one per sub-cell (:52-64).
+    int rk2 = 0;
     ArrD dens;
     // diagnostics of the last projection
     int lastIters = 0, hitMax = 0;
@@ -284,8 +289,10 @@ struct Sim {
     }
 
     // ---- FluidSim2D::create (src/FluidSim2D.cpp:23-76) ----
+    // uniform: each particle at ((i + U) dx, (j + U) dx), U = rand()/RAND_MAX,
+    // x drawn before y, instead of the jittered sub-cell grid.
     void create(int nx, int ny, int kk, double dt_, double dx_, double rho_, double gx, double gy,
-                int mode_, double alpha_, const uint8_t* init, unsigned seed) {
+                int mode_, double alpha_, const uint8_t* init, unsigned seed, int uniform = 0) {
         sizeX = nx; sizeY = ny; k = kk; ppc = kk * kk;
         dt = dt_; dx = dx_; dr = 0.9 * dx_; rho = rho_;
         gravity = V2{gx, gy};
@@ -307,6 +314,12 @@ struct Sim {
                 if (cell(i, j) == FLUID) {
                     fluidCount++;
                     for (int e = 0; e < ppc; e++) {
+                        if (uniform) {
+                            const double ux = randf(1.0f);
+                            const double uy = randf(1.0f);
+                            pos.push_back(V2{((double)i + ux) * dx, ((double)j + uy) * dx});
+                            continue;
+                        }
                         int x = e % kk, y = e / kk;
                         // braced initialiser: x jitter is drawn before y jitter
                         double jx = randf(dist);
@@ -748,10 +761,16 @@ struct Sim {
             V2 ps = pos[e];
             V2 k1 = velInterp(ps);
             V2 k2 = velInterp(V2{ps.x + 0.5 * dt * k1.x, ps.y + 0.5 * dt * k1.y});
-            V2 k3 = velInterp(V2{ps.x + 0.75 * dt * k2.x, ps.y + 0.75 * dt * k2.y});
-            V2 nx{ps.x + (2. / 9.) * dt * k1.x, ps.y + (2. / 9.) * dt * k1.y};
-            nx = V2{nx.x + (3. / 9.) * dt * k2.x, nx.y + (3. / 9.) * dt * k2.y};
-            nx = V2{nx.x + (4. / 9.) * dt * k3.x, nx.y + (4. / 9.) * dt * k3.y};
+            V2 nx;
+            if (rk2) {
+                // midpoint rule: the velocity at the half step, for the whole step
+                nx = V2{ps.x + dt * k2.x, ps.y + dt * k2.y};
+            } else {
+                V2 k3 = velInterp(V2{ps.x + 0.75 * dt * k2.x, ps.y + 0.75 * dt * k2.y});
+                nx = V2{ps.x + (2. / 9.) * dt * k1.x, ps.y + (2. / 9.) * dt * k1.y};
+                nx = V2{nx.x + (3. / 9.) * dt * k2.x, nx.y + (3. / 9.) * dt * k2.y};
+                nx = V2{nx.x + (4. / 9.) * dt * k3.x, nx.y + (4. / 9.) * dt * k3.y};
+            }
             nx = clampPos(nx);
             if (std::isnan(ps.x) || std::isnan(ps.y)) nan = 1;
             pos[e] = nx;
@@ -1052,10 +1071,10 @@ struct orc_stats {
 };
 
 void* orc_create(int nx, int ny, int k, double dt, double dx, double rho, double gx, double gy,
-                 int mode, double alpha, const uint8_t* init, unsigned seed, int level_set) {
+                 int mode, double alpha, const uint8_t* init, unsigned seed, int level_set, int uniform_seeding) {
     Sim* s = new Sim();
     s->levelSet = level_set;
-    s->create(nx, ny, k, dt, dx, rho, gx, gy, mode, alpha, init, seed);
+    s->create(nx, ny, k, dt, dx, rho, gx, gy, mode, alpha, init, seed, uniform_seeding);
     return s;
 }
 
@@ -1069,10 +1088,11 @@ void orc_set_options(void* h, double tau, double sigma, double tol, int max_iter
 }
 
 // The reference's compile-time variants (src/FluidSim2D.cpp:15-16).
-void orc_set_variants(void* h, int particle_cells, int ghost) {
+void orc_set_variants(void* h, int particle_cells, int ghost, int rk2) {
     Sim* s = (Sim*)h;
     s->particleCells = particle_cells;
     s->ghost = ghost;
+    s->rk2 = rk2;
 }
 
 void orc_set_params(void* h, double dt, double gx, double gy, double alpha) {

@@ -48,7 +48,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
         L.orc_create.restype = vp
-        L.orc_create.argtypes = [i, i, i, d, d, d, d, d, i, d, vp, ctypes.c_uint, i]
+        L.orc_create.argtypes = [i, i, i, d, d, d, d, d, i, d, vp, ctypes.c_uint, i, i]
         L.orc_destroy.argtypes = [vp]
         L.orc_set_options.argtypes = [vp, d, d, d, i, i, i]
         L.orc_set_params.argtypes = [vp, d, d, d, d]
@@ -66,7 +66,7 @@ def lib():
         L.orc_tricubic.argtypes = [vp, i, i, d, d]
         L.orc_extrapolate.argtypes = [vp, vp, i, i]
         L.orc_rand_stream.argtypes = [ctypes.c_uint, vp, i]
-        L.orc_set_variants.argtypes = [vp, i, i]
+        L.orc_set_variants.argtypes = [vp, i, i, i]
         L.orc_adjoint_step.restype = i
         L.orc_adjoint_step.argtypes = [vp] * 7
         L.orc_set_threads.argtypes = [i]
@@ -84,12 +84,16 @@ class Oracle:
 
     def __init__(self, cells, k=2, dt=0.005, dx=0.01, rho=997.0, gravity=(0.0, -9.81),
                  mode=MODE_PIC, alpha=1.0, seed=1, level_set=True, tau=0.999,
-                 sigma=0.25, tol=1e-12, max_iters=200, sl_inplace=True, density=False):
+                 sigma=0.25, tol=1e-12, max_iters=200, sl_inplace=True, density=False,
+                 uniform_seeding=False, rk2=False):
         cells = np.ascontiguousarray(cells, dtype=np.uint8)
         self.ny, self.nx = cells.shape
         self.L = lib()
         self.h = self.L.orc_create(self.nx, self.ny, k, dt, dx, rho, gravity[0], gravity[1],
-                                   mode, alpha, _ptr(cells), seed, int(level_set))
+                                   mode, alpha, _ptr(cells), seed, int(level_set), int(uniform_seeding))
+        self._variants = dict(particle_cells=False, ghost=True, rk2=rk2)
+        if rk2:
+            self.set_variants()
         self.set_options(tau=tau, sigma=sigma, tol=tol, max_iters=max_iters,
                          sl_inplace=sl_inplace, density=density)
 
@@ -97,11 +101,16 @@ class Oracle:
                     density=False):
         self.L.orc_set_options(self.h, tau, sigma, tol, max_iters, int(sl_inplace), int(density))
 
-    def set_variants(self, particle_cells=False, ghost=True):
+    def set_variants(self, particle_cells=None, ghost=None, rk2=None):
         """The reference's compile-time variants: cells from particles instead
         of the level set (USE_LEVEL_SET off), no ghost-fluid pressure
-        (USE_GHOST_PRESSURE off)."""
-        self.L.orc_set_variants(self.h, int(particle_cells), int(ghost))
+        (USE_GHOST_PRESSURE off); and RK2 particle advection (no reference
+        counterpart).  Arguments left None keep their current setting."""
+        for k, v in (("particle_cells", particle_cells), ("ghost", ghost), ("rk2", rk2)):
+            if v is not None:
+                self._variants[k] = v
+        vv = self._variants
+        self.L.orc_set_variants(self.h, int(vv["particle_cells"]), int(vv["ghost"]), int(vv["rk2"]))
 
     def set_params(self, dt, gravity, alpha):
         self.L.orc_set_params(self.h, dt, gravity[0], gravity[1], alpha)

@@ -423,3 +423,54 @@ def test_sl_zero_velocity_is_identity_outofplace():
     o.set(O.F_V, v)
     o.run_stage(O.ST_SL)
     assert not o.get(O.F_U).any() and not o.get(O.F_V).any()
+
+
+# ---------------------------------------------------------------------------
+# Options of the 512^2 PIC/FLIP dam-break workload: uniform seeding and RK2
+# particle advection.
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("seed", [0, 5])
+def test_uniform_seeding_is_libc_rand_over_whole_cells(seed):
+    """Each of the k^2 particles of a FLUID cell (row-major cell order) at
+    ((i + U) dx, (j + U) dx), U = rand()/RAND_MAX from the C library's own
+    generator, x before y."""
+    n, dx, k = 12, 0.05, 2
+    cells = O.dam_break(n, n)
+    o = O.Oracle(cells, k=k, dx=dx, seed=seed, uniform_seeding=True)
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    libc.srand(ctypes.c_uint(seed))
+    want = []
+    for j in range(n):
+        for i in range(n):
+            if cells[j, i] == O.FLUID:
+                for _ in range(k * k):
+                    ux = libc.rand() / 2147483647.0
+                    uy = libc.rand() / 2147483647.0
+                    want.append(((i + ux) * dx, (j + uy) * dx))
+    got = o.get(O.F_POS)
+    np.testing.assert_array_equal(got, np.array(want))
+    # every particle inside its own cell, not confined to a sub-cell
+    cell_ij = np.floor(got / dx).astype(int)
+    frac = got / dx - cell_ij
+    assert (frac.min(axis=0) < 0.25).all() and (frac.max(axis=0) > 0.75).all()
+
+
+@pytest.mark.parametrize("rk2", [False, True])
+def test_particle_advection_order_on_linear_field(rk2):
+    """u = c x, v = 0 (Catmull-Rom reproduces a linear field exactly away
+    from clamped taps): the midpoint rule moves x to x (1 + h + h^2/2),
+    Ralston's RK3 to x (1 + h + h^2/2 + h^3/6), h = c dt."""
+    n, dx, dt, c = 32, 1.0 / 32, 0.05, 2.0
+    cells = O.all_fluid(n, n)
+    o = O.Oracle(cells, k=0, dx=dx, dt=dt, rk2=rk2)
+    i = np.arange(n + 1, dtype=np.float64)
+    o.set(O.F_U, np.tile(c * i * dx, (n, 1)))
+    o.set(O.F_V, np.zeros((n + 1, n)))
+    x0 = np.array([[10.3 * dx, 12.7 * dx], [15.6 * dx, 9.2 * dx], [8.05 * dx, 20.5 * dx]])
+    o.set(O.F_POS, x0)
+    o.run_stage(O.ST_ADVECT)
+    h = c * dt
+    factor = 1 + h + h * h / 2 + (0.0 if rk2 else h ** 3 / 6)
+    got = o.get(O.F_POS)
+    np.testing.assert_allclose(got[:, 0], x0[:, 0] * factor, rtol=1e-13)
+    np.testing.assert_allclose(got[:, 1], x0[:, 1], rtol=1e-13)
