@@ -100,7 +100,9 @@ template <int WHICH, bool SCATTER, bool GRAD>
 __device__ __forceinline__ void interp_adjoint(const float* __restrict__ a, unsigned long long* acc, int nx, int ny,
                                                double dx, double px, double py, float g, double S, float& bx,
                                                float& by) {
-    double gx = px / dx, gy = py / dx;
+    // grid units as vel_interp_u / vel_interp_v / the SL density sample form them
+    const double inv_dx = 1.0 / dx;
+    double gx = px * inv_dx, gy = py * inv_dx;
     if (WHICH == 0) gy -= 0.5;
     else if (WHICH == 1) gx -= 0.5;
     else { gx -= 0.5; gy -= 0.5; }

@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kBlock) void b_sl_advect_kernel(const InstBufs* __r
     const bool dens = (I.flags & 2) != 0;  // FS2D_FLAG_DENSITY
     const float gu = with_gravity ? (float)(I.dt * I.gx) : 0.0f;
     const float gv = with_gravity ? (float)(I.dt * I.gy) : 0.0f;
-    sl_advect_body(MacView{I.u, I.v, I.nx, I.ny, I.dx}, I.u2, I.v2, dens ? I.dens : nullptr, I.dens2, I.dt,
+    sl_advect_body(make_mac(I.u, I.v, I.nx, I.ny, I.dx), I.u2, I.v2, dens ? I.dens : nullptr, I.dens2, I.dt,
                    clamp_box(I.nx, I.ny, I.dx), gu, gv, vb, nvb);
 }
 
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void b_particle_advect_kernel(const InstBuf
     int b, vb, nvb;
     if (!bmap(bm, b, vb, nvb)) return;
     const InstBufs& I = ins[b];
-    particle_advect_body(I.px, I.py, I.pvx, I.pvy, I.np, MacView{I.u, I.v, I.nx, I.ny, I.dx}, I.dt,
+    particle_advect_body(I.px, I.py, I.pvx, I.pvy, I.np, make_mac(I.u, I.v, I.nx, I.ny, I.dx), I.dt,
                          clamp_box(I.nx, I.ny, I.dx), I.dg, I.partials, (I.flags & 32) ? 1 : 0 /* FS2D_FLAG_RK2 */,
                          vb, nvb);
 }
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(kBlock) void b_diagnostics_kernel(const InstBufs* _
     int b, vb, nvb;
     if (!bmap(bm, b, vb, nvb)) return;
     const InstBufs& I = ins[b];
-    diagnostics_body(I.cell, MacView{I.u, I.v, I.nx, I.ny, I.dx}, I.px, I.py, I.pvx, I.pvy, I.np, I.rho, I.gx, I.gy,
+    diagnostics_body(I.cell, make_mac(I.u, I.v, I.nx, I.ny, I.dx), I.px, I.py, I.pvx, I.pvy, I.np, I.rho, I.gx, I.gy,
                      I.pmass, I.partials, vb, nvb);
 }
 __global__ __launch_bounds__(kBlock) void b_diag_finalize_kernel(const InstBufs* __restrict__ ins, int n,

@@ -161,7 +161,7 @@ fs2d_status build_maps(fs2d_batch* B) {
         const long nc = n_c(s), nf = n_u(s) + n_v(s);
         c_cells[b] = blocks_for(nc);
         c_faces[b] = blocks_for(nf);
-        c_sl[b] = blocks_for(nf + ((s->flags & FS2D_FLAG_DENSITY) ? nc : 0));
+        c_sl[b] = sl_tiles(s->nx, s->ny);
         c_part[b] = s->np > 0 ? blocks_for(s->np) : 0;
         c_diag[b] = blocks_for(nc + s->np);
         c_sk[b] = sk_blocks(s);

@@ -266,7 +266,7 @@ long n_u(const fs2d_sim* s) { return (long)(s->nx + 1) * s->ny; }
 long n_v(const fs2d_sim* s) { return (long)s->nx * (s->ny + 1); }
 long n_c(const fs2d_sim* s) { return (long)s->nx * s->ny; }
 
-MacView mac_view(const fs2d_sim* s) { return MacView{s->u, s->v, s->nx, s->ny, s->dx}; }
+MacView mac_view(const fs2d_sim* s) { return make_mac(s->u, s->v, s->nx, s->ny, s->dx); }
 
 fs2d_status ensure_particle_capacity(fs2d_sim* s, long n) {
     if (n <= s->np_cap) return FS2D_OK;
@@ -453,11 +453,10 @@ void stage_level_set(fs2d_sim* s) {
 void stage_sl(fs2d_sim* s, bool with_gravity) {
     Timed t(s, K_SL_ADVECT);
     const bool dens = (s->flags & FS2D_FLAG_DENSITY) != 0;
-    const long n = n_u(s) + n_v(s) + (dens ? n_c(s) : 0);
     const float gu = with_gravity ? (float)(s->dt * s->gx) : 0.0f;
     const float gv = with_gravity ? (float)(s->dt * s->gy) : 0.0f;
-    hipLaunchKernelGGL(sl_advect_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s->stream, mac_view(s), s->u2, s->v2,
-                       dens ? s->dens : nullptr, s->dens2, s->dt, clamp_box(s->nx, s->ny, s->dx), gu, gv);
+    hipLaunchKernelGGL(sl_advect_kernel, dim3(sl_tiles(s->nx, s->ny)), dim3(kBlock), 0, s->stream, mac_view(s), s->u2,
+                       s->v2, dens ? s->dens : nullptr, s->dens2, s->dt, clamp_box(s->nx, s->ny, s->dx), gu, gv);
     std::swap(s->u, s->u2);
     std::swap(s->v, s->v2);
     if (dens) std::swap(s->dens, s->dens2);

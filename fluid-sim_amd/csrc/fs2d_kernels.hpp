@@ -107,33 +107,73 @@ __global__ void copy_kernel(T* dst, const T* src, size_t n) {
 // applyGravity (:237-250) optionally fused into the store, and the passive
 // density (no reference counterpart) traced back from cell centres.
 // ---------------------------------------------------------------------------
+// The stage runs over tiles of kSlTX x kSlTY cells, one at a time per
+// workgroup: a tile's outputs are the u faces, v faces and density cells whose
+// index (i, j) falls in it.  The velocities (and density) of the tile and a
+// halo of kSlHalo cells are copied to LDS first, coalesced; a sample whose 16
+// taps all lie in that window reads LDS, any other (a departure point more
+// than about kSlHalo - 2.5 cells away, or the window's clipped border) reads
+// the arrays: the same bits either way.  Departure points within 1.5 cells
+// (CFL <= 1.5) read LDS only.
+constexpr int kSlTX = 64, kSlTY = 16, kSlHalo = 4;
+constexpr int kSlUW = kSlTX + 1 + 2 * kSlHalo, kSlUH = kSlTY + 2 * kSlHalo;
+constexpr int kSlVW = kSlTX + 2 * kSlHalo, kSlVH = kSlTY + 1 + 2 * kSlHalo;
+constexpr int kSlDW = kSlTX + 2 * kSlHalo, kSlDH = kSlTY + 2 * kSlHalo;
+__host__ __device__ inline int sl_tiles(int nx, int ny) {
+    return ((nx + 1 + kSlTX - 1) / kSlTX) * ((ny + 1 + kSlTY - 1) / kSlTY);
+}
+struct SlLds {
+    float u[kSlUW * kSlUH];
+    float v[kSlVW * kSlVH];
+    float d[kSlDW * kSlDH];
+};
+// Copies columns x0 .. x0 + ww - 1 and rows y0 .. y0 + wh - 1 of the w x h
+// array a, clipped to the array, into lds (row stride = the clipped width).
+__device__ __forceinline__ LdsWin load_win(float* lds, const float* __restrict__ a, int w, int h, int x0, int y0,
+                                           int ww, int wh) {
+    const int xa = max(x0, 0), ya = max(y0, 0);
+    const int cw = min(x0 + ww, w) - xa, ch = min(y0 + wh, h) - ya;
+    for (int k = threadIdx.x; k < cw * ch; k += blockDim.x) {
+        const int r = k / cw, c = k - r * cw;
+        lds[k] = a[(size_t)(ya + r) * w + xa + c];
+    }
+    return LdsWin{lds, xa, ya, cw, ch};
+}
+
 __device__ __forceinline__ void sl_advect_body(MacView m, float* __restrict__ u_out,
                                                            float* __restrict__ v_out,
                                                            const float* __restrict__ d_in,
                                                            float* __restrict__ d_out, double dt,
                                                            ClampBox cb, float gu, float gv, int vb, int nvb) {
+    __shared__ SlLds L;
     const int nx = m.nx, ny = m.ny;
-    const long nu = (long)(nx + 1) * ny, nv = (long)nx * (ny + 1);
-    const long nd = d_in ? (long)nx * ny : 0;
-    const long n = nu + nv + nd;
-    for (long e = vb * (long)blockDim.x + threadIdx.x; e < n; e += (long)nvb * blockDim.x) {
-        double ox, oy;
-        if (e < nu) {
-            const int i = (int)(e % (nx + 1)), j = (int)(e / (nx + 1));
-            backtrace(m, dt, cb, (double)i * m.dx, ((double)j + 0.5) * m.dx, ox, oy);
-            u_out[e] = vel_interp_u(m, ox, oy) + gu;
-        } else if (e < nu + nv) {
-            const long f = e - nu;
-            const int i = (int)(f % nx), j = (int)(f / nx);
-            backtrace(m, dt, cb, ((double)i + 0.5) * m.dx, (double)j * m.dx, ox, oy);
-            v_out[f] = vel_interp_v(m, ox, oy) + gv;
-        } else {
-            const long f = e - nu - nv;
-            const int i = (int)(f % nx), j = (int)(f / nx);
-            backtrace(m, dt, cb, ((double)i + 0.5) * m.dx, ((double)j + 0.5) * m.dx, ox, oy);
-            double gx = clampd(ox / m.dx - 0.5, 1e-6, (double)(nx - 1) - 1e-6);
-            double gy = clampd(oy / m.dx - 0.5, 1e-6, (double)(ny - 1) - 1e-6);
-            d_out[f] = catmull_rom(d_in, nx, ny, gx, gy);
+    const int ntx = (nx + 1 + kSlTX - 1) / kSlTX, nt = sl_tiles(nx, ny);
+    for (int t = vb; t < nt; t += nvb) {
+        const int X0 = (t % ntx) * kSlTX, Y0 = (t / ntx) * kSlTY;
+        __syncthreads();  // the previous tile's samples are done with L
+        const LdsWin U = load_win(L.u, m.u, nx + 1, ny, X0 - kSlHalo, Y0 - kSlHalo, kSlUW, kSlUH);
+        const LdsWin V = load_win(L.v, m.v, nx, ny + 1, X0 - kSlHalo, Y0 - kSlHalo, kSlVW, kSlVH);
+        LdsWin D{};
+        if (d_in) D = load_win(L.d, d_in, nx, ny, X0 - kSlHalo, Y0 - kSlHalo, kSlDW, kSlDH);
+        __syncthreads();
+        const WinSampler S{m, U, V};
+        for (int k = threadIdx.x; k < kSlTX * kSlTY; k += blockDim.x) {
+            const int i = X0 + k % kSlTX, j = Y0 + k / kSlTX;
+            double ox, oy;
+            if (i <= nx && j < ny) {  // u face (i, j + 1/2)
+                backtrace_with(S, dt, cb, (double)i * m.dx, ((double)j + 0.5) * m.dx, ox, oy);
+                u_out[(size_t)j * (nx + 1) + i] = S.u(ox, oy) + gu;
+            }
+            if (i < nx && j <= ny) {  // v face (i + 1/2, j)
+                backtrace_with(S, dt, cb, ((double)i + 0.5) * m.dx, (double)j * m.dx, ox, oy);
+                v_out[(size_t)j * nx + i] = S.v(ox, oy) + gv;
+            }
+            if (d_in && i < nx && j < ny) {  // density at the cell centre
+                backtrace_with(S, dt, cb, ((double)i + 0.5) * m.dx, ((double)j + 0.5) * m.dx, ox, oy);
+                const double gx = clampd(ox * m.inv_dx - 0.5, 1e-6, (double)(nx - 1) - 1e-6);
+                const double gy = clampd(oy * m.inv_dx - 0.5, 1e-6, (double)(ny - 1) - 1e-6);
+                d_out[(size_t)j * nx + i] = catmull_rom_win(D, d_in, nx, ny, gx, gy);
+            }
         }
     }
 }

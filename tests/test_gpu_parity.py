@@ -121,6 +121,33 @@ def test_sl_advection_stage():
     assert eu < REL and ev < REL
 
 
+@pytest.mark.parametrize("nx,ny,cfl", [(100, 70, 0.9), (130, 45, 3.5), (64, 200, 6.0)])
+def test_sl_advection_tiles_and_fast_flow(nx, ny, cfl):
+    """The tiled stage on grids that are not multiples of its 64 x 16 tiles,
+    at Courant numbers whose departure points stay in a tile's LDS window
+    (0.9) and leave it (3.5, 6: those samples read the arrays), with density
+    and dx not a power of two."""
+    cells = O.all_fluid(nx, ny)
+    dx = 0.013
+    rng = np.random.default_rng(nx + ny)
+    u0 = smooth_field((ny, nx + 1), rng).astype(np.float64)
+    v0 = smooth_field((ny + 1, nx), rng).astype(np.float64)
+    vmax = max(np.abs(u0).max(), np.abs(v0).max())
+    dt = cfl * dx / vmax
+    g, o = pair(cells, k=0, dt=dt, dx=dx, mode=fs2d.SEMILAGRANGIAN, static_cells=True, density=True,
+                gravity=(0.0, 0.0))
+    g.set(fs2d.F_U, u0.astype(np.float32))
+    g.set(fs2d.F_V, v0.astype(np.float32))
+    g.set(fs2d.F_DENSITY, np.abs(smooth_field((ny, nx), rng)))
+    push(g, o, [fs2d.F_U, fs2d.F_V, fs2d.F_DENSITY])
+    g.run_stage(fs2d.STAGE_SL)
+    o.run_stage(O.ST_SL)
+    e = [rel_l2(g.get(f), o.get(of)) for f, of in ((fs2d.F_U, O.F_U), (fs2d.F_V, O.F_V),
+                                                    (fs2d.F_DENSITY, O.F_DENS))]
+    print("%dx%d CFL %.1f: SL rel-L2 u %.2e v %.2e density %.2e" % (nx, ny, cfl, *e))
+    assert max(e) < REL
+
+
 def test_density_advection_stage():
     n = 48
     cells = O.all_fluid(n, n)
