@@ -222,7 +222,7 @@ def cpu_baseline(state, cells, n, dt, tol, tau, threads, iters_gpu, host, caps=(
                                                caps[1], iters_gpu)}
 
 
-def ensemble_leg(count, steps, warmup, loop_steps, tune_cache=None):
+def ensemble_leg(count, steps, warmup, loop_steps, tune_cache=None, graph=True):
     """The batched ensemble: stepped by replaying captured graphs (the
     value) and by uncaptured batch steps, two batches in lockstep from the
     same state (their members stay bit-identical); and the loop over the same
@@ -232,24 +232,30 @@ def ensemble_leg(count, steps, warmup, loop_steps, tune_cache=None):
     members = scenes.ensemble_members(count)
     cells = int(sum(c.shape[0] * c.shape[1] for c, _ in members))
     batch = fs2d.Batch(members)
-    gbatch = fs2d.Batch(members)
-    gbatch.set_graph(True)
-    batch.step()
-    gbatch.step()
+    gbatch = fs2d.Batch(members) if graph else None
+    twins = [batch] + ([gbatch] if graph else [])
+    if graph:
+        gbatch.set_graph(True)
+    for b in twins:
+        b.step()
     tune = batch.autotune(iters=24, cache=tune_cache)
-    gbatch.set_tuning(tune["pcg_grid"], tune["sweep_prefetch"])
+    if graph:
+        gbatch.set_tuning(tune["pcg_grid"], tune["sweep_prefetch"])
     for _ in range(warmup):
-        batch.step()
-        gbatch.step()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        gbatch.step()  # returns after the step's work (fs2d_batch_step synchronizes)
-    tg = (time.perf_counter() - t0) / steps
+        for b in twins:
+            b.step()
+    tg = None
+    if graph:
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            gbatch.step()  # returns after the step's work (fs2d_batch_step synchronizes)
+        tg = (time.perf_counter() - t0) / steps
+        gbatch.close()
     t0 = time.perf_counter()
     for _ in range(steps):
         batch.step()
     tb = (time.perf_counter() - t0) / steps
-    gbatch.close()
+    tv = tg if graph else tb
     # per-class breakdown from further steps (events around every launch cost time)
     batch.reset_kernel_times()
     batch.set_timing(True)
@@ -273,12 +279,12 @@ def ensemble_leg(count, steps, warmup, loop_steps, tune_cache=None):
         for s in singles:
             s.close()
     return {
-        "metric": "cell-steps/s of the batched ensemble", "value": cells / tg, "unit": "cell-steps/s",
-        "ms_per_step": tg * 1e3, "steps": steps, "warmup": warmup, "graph": True,
+        "metric": "cell-steps/s of the batched ensemble", "value": cells / tv, "unit": "cell-steps/s",
+        "ms_per_step": tv * 1e3, "steps": steps, "warmup": warmup, "graph": graph,
         "no_graph_value": cells / tb, "no_graph_ms_per_step": tb * 1e3,
         "tuning": tune,
         "loop_value": cells / tl if tl else None, "loop_ms_per_step": tl * 1e3 if tl else None,
-        "loop_steps": loop_steps, "speedup_vs_loop": tl / tg if tl else None,
+        "loop_steps": loop_steps, "speedup_vs_loop": tl / tv if tl else None,
         "config": {"workload": "%d PIC/FLIP dam breaks, sides 64..512 (seed 2), block fractions U[0.2,0.7], "
                                "k=2, FLIP 0.95, dt 0.002, PCG 1e-6 masked per member" % count,
                    "members": count, "cells": cells, "pcg_iters_max": int(max(its)),
@@ -420,6 +426,8 @@ def main():
     ap.add_argument("--loop-steps", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-grid3d", action="store_true", help="skip the 3D grid leg")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the captured-graph twins of the 2048^2 and ensemble legs (profiling runs)")
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "fluid-sim_amd", "tune_cache.txt"),
                     help="autotuner record (read, or appended to after timing the candidates)")
     ap.add_argument("--only", choices=["ensemble", "adjoint", "grid3d"],
@@ -439,7 +447,7 @@ def main():
     import fs2d
     if args.only == "ensemble":
         print(json.dumps(ensemble_leg(args.ensemble_members, args.ensemble_steps, args.warmup, args.loop_steps,
-                                      args.tune_cache)))
+                                      args.tune_cache, not args.no_graph)))
         return
     if args.only == "adjoint":
         print(json.dumps(adjoint_leg()))
@@ -451,22 +459,25 @@ def main():
     sim, cells, dt = build_gpu_sim(n, args.tol, args.tau, args.max_iters)
     state = [sim.get(f) for f in (fs2d.F_U, fs2d.F_V, fs2d.F_DENSITY)]
     # the same workload stepped by replaying captured graphs, in lockstep
-    gsim, _, _ = build_gpu_sim(n, args.tol, args.tau, args.max_iters)
-    gsim.set_graph(True)
-    sim.step()
-    gsim.step()
+    gsim = None if args.no_graph else build_gpu_sim(n, args.tol, args.tau, args.max_iters)[0]
+    twins = [sim] + ([gsim] if gsim else [])
+    if gsim:
+        gsim.set_graph(True)
+    for s_ in twins:
+        s_.step()
     tune = sim.autotune(iters=24, cache=args.tune_cache)
-    gsim.set_tuning(pcg_grid=tune["pcg_grid"])
+    if gsim:
+        gsim.set_tuning(pcg_grid=tune["pcg_grid"])
     for _ in range(args.warmup):
-        sim.step()
-        gsim.step()
+        for s_ in twins:
+            s_.step()
     # per-class breakdown and the dominant class, with every class timed
     sim.synchronize()
     sim.reset_kernel_times()
     sim.set_timing(True)
     for _ in range(PROFILE_STEPS):
-        sim.step()
-        gsim.step()
+        for s_ in twins:
+            s_.step()
     kt_all = sim.kernel_times()
     sim.set_timing(False)
     dom = max((k for k in kt_all if k in BYTES_PER_CELL), key=lambda k: kt_all[k][0])
@@ -486,15 +497,20 @@ def main():
     elapsed = time.perf_counter() - t0
     kt = sim.kernel_times()
     sim.set_timing(False)
-    gsim.synchronize()
-    t0 = time.perf_counter()
-    giters = []
-    for _ in range(args.steps):
-        gsim.step()
-        giters.append(gsim.stats()["pcg_iters"])
-    gsim.synchronize()
-    g_elapsed = time.perf_counter() - t0
-    gsim.close()
+    graph_out = None
+    if gsim:
+        gsim.synchronize()
+        t0 = time.perf_counter()
+        giters = []
+        for _ in range(args.steps):
+            gsim.step()
+            giters.append(gsim.stats()["pcg_iters"])
+        gsim.synchronize()
+        g_elapsed = time.perf_counter() - t0
+        gsim.close()
+        graph_out = {"value": n * n * args.steps / g_elapsed, "ms_per_step": g_elapsed / args.steps * 1e3,
+                     "pcg_iters_mean": float(np.mean(giters)),
+                     "note": "the same steps replayed from captured HIP graphs (fs2d_set_graph), no events"}
 
     value = n * n * args.steps / elapsed
     tot_ms, launches = kt[dom]
@@ -524,9 +540,7 @@ def main():
             "pcg_max_iters": args.max_iters, "pcg_capped_steps": capped,
             "tuning": tune,
         },
-        "graph": {"value": n * n * args.steps / g_elapsed, "ms_per_step": g_elapsed / args.steps * 1e3,
-                  "pcg_iters_mean": float(np.mean(giters)),
-                  "note": "the same steps replayed from captured HIP graphs (fs2d_set_graph), no events"},
+        "graph": graph_out,
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, n),
@@ -538,7 +552,7 @@ def main():
     sim.close()
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ensemble_members, args.ensemble_steps, 2, args.loop_steps,
-                                       args.tune_cache)
+                                       args.tune_cache, not args.no_graph)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()
     if not args.no_grid3d:
