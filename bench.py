@@ -25,9 +25,13 @@ the step, so the timed region times the dominant class only; the dominant
 class and `kernel_ms_per_step` come from 2 steps with every class timed,
 run after the warmup and before the timed region.
 
-`graph`: the same 2048^2 steps taken by a second simulation in lockstep
-with the first (same state, bit-identical results) whose fs2d_step replays
-captured HIP graphs (fs2d_set_graph), timed without events.
+Value = cells x steps / wall time of the 2048^2 steps taken by a simulation
+whose fs2d_step replays captured HIP graphs (fs2d_set_graph), with no events;
+a second simulation, in lockstep with it up to the timed regions (same state,
+bit-identical results, the same iteration counts), takes the same steps
+uncaptured with events on the dominant class (`no_graph_*`, and the source of
+`roofline` and `kernel_ms_per_step`).  `--no-graph` leaves the captured twin
+out; the value is then the uncaptured one.
 
 Both simulations, and both ensemble batches, run the launch shapes the
 autotuner picks (fs2d_autotune / fs2d_batch_autotune, which change no
@@ -512,7 +516,10 @@ def main():
                      "pcg_iters_mean": float(np.mean(giters)),
                      "note": "the same steps replayed from captured HIP graphs (fs2d_set_graph), no events"}
 
-    value = n * n * args.steps / elapsed
+    value, ms_per_step = n * n * args.steps / elapsed, elapsed / args.steps * 1e3
+    no_graph = {"no_graph_value": value, "no_graph_ms_per_step": ms_per_step}
+    if graph_out:
+        value, ms_per_step = graph_out["value"], graph_out["ms_per_step"]
     tot_ms, launches = kt[dom]
     if dom in EFFECTIVE:
         launches = int(sum(iters)) + EFFECTIVE[dom] * args.steps
@@ -526,7 +533,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -541,6 +548,7 @@ def main():
             "tuning": tune,
         },
         "graph": graph_out,
+        **no_graph,
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, n),

@@ -217,6 +217,21 @@ __global__ __launch_bounds__(kBlock) void b_pcg_plan_kernel(const InstBufs* __re
         const int f = my_first, cnt = count[b];
         for (int k = threadIdx.x; k < v; k += kBlock) P.map[f + k] = k < cnt ? b : -1;
         if (threadIdx.x == 0) P.first[b] = f;
+        // sigma_h and beta of the member, once, in the order and arithmetic
+        // of spmv_scalars (one wave's fixed-order sums), for b_spmv_kernel
+        if (threadIdx.x < 64) {
+            const InstBufs& I = ins[b];
+            const int ns = I.geom.S * kLoaders;
+            const double* qq0 = I.qq;
+            const double* qq1 = I.qq + (size_t)ns;
+            const double sig = wave_sum_of((h & 1) ? qq1 : qq0, ns);
+            double beta_d = 0;
+            if (h != 0) beta_d = sig / wave_sum_of((h & 1) ? qq0 : qq1, ns);
+            if (threadIdx.x == 0) {
+                I.sc->sigma = sig;
+                I.sc->beta = (float)beta_d;
+            }
+        }
     }
     if (i == 0 && threadIdx.x == 0) {
         *P.total = total;
@@ -226,15 +241,13 @@ __global__ __launch_bounds__(kBlock) void b_pcg_plan_kernel(const InstBufs* __re
 
 // Iteration h of the PCG loop over the members still iterating (one
 // persistent launch of any size: workgroup g runs virtual workgroups g, g +
-// gridDim.x, ... of the iteration's plan; s double-buffered by parity,
-// sigma_h and sigma_{h-1} in the two halves of qq).  Each member's virtual
-// workgroups compute what its single launch computes, so members step
-// bit-identically to single simulations.
+// gridDim.x, ... of the iteration's plan; s double-buffered by parity; beta
+// formed once per member by the plan kernel from sigma_h and sigma_{h-1}, the
+// two halves of qq).  Each member's virtual workgroups compute what its single
+// launch computes, so members step bit-identically to single simulations.
 __global__ __launch_bounds__(kBlock) void b_spmv_kernel(const InstBufs* __restrict__ ins,
                                                         const int* __restrict__ count, PcgPlanBufs P, int h) {
     const int total = *P.total;
-    int cached = -1;
-    SpmvScalars S{};
     int b_next = (int)blockIdx.x < total ? P.map[blockIdx.x] : -1;
     for (int k = blockIdx.x; k < total; k += gridDim.x) {
         const int b = b_next;
@@ -245,14 +258,8 @@ __global__ __launch_bounds__(kBlock) void b_spmv_kernel(const InstBufs* __restri
         const float* s_old = (h & 1) ? I.s1 : I.s0;
         float* s_new = (h & 1) ? I.s0 : I.s1;
         const SpmvPre pre = spmv_pre(I.geom, I.Adiag, I.cpl, I.z, s_old, vb, nvb);
-        if (b != cached) {
-            const double* qq0 = I.qq;
-            const double* qq1 = I.qq + (size_t)I.geom.S * kLoaders;
-            S = spmv_scalars(I.geom, (h & 1) ? qq1 : qq0, (h & 1) ? qq0 : qq1, h == 0 ? 1 : 0);
-            cached = b;
-        }
-        if (vb == 0 && threadIdx.x == 0) I.sc->sigma = S.sig;
-        spmv_core(pre, I.geom, I.Adiag, I.cpl, I.z, s_old, s_new, I.w, S.beta, I.ws_part, vb);
+        const float beta = I.sc->beta;  // b_pcg_plan_kernel's, as spmv_scalars forms it
+        spmv_core(pre, I.geom, I.Adiag, I.cpl, I.z, s_old, s_new, I.w, beta, I.ws_part, vb);
     }
 }
 // w.s of every member still iterating, summed once per member (one wave per

@@ -547,7 +547,9 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_kernel(Op op, SkGeom g, S
 struct PcgScalars {
     double rhs_norm, r_norm, sigma, tol;
     double ws;  // w.s of the current iteration when a separate reduction formed it
-    int iter, done, pad0, pad1;
+    int iter, done;
+    float beta;  // beta of the current iteration when a separate reduction formed it
+    int pad1;
 };
 
 __device__ __forceinline__ double wave_max_of(const double* a, int n) {
