@@ -122,6 +122,7 @@ struct fs2d_sim {
     int* h_err = nullptr;
 
     std::vector<void*> allocs;
+    int* render_cnt = nullptr;  // fs2d_render_buffers: block counts and totals, allocated on first use
     double current_time = 0, orig_volume = 0;
     long long steps = 0;
     int last_iters_hint = 8;
@@ -1631,4 +1632,5 @@ fs2d_status fs2d_project_noalloc(const fs2d_projection_args* a, int* iters, doub
 #include "fs2d_adjoint_api.hpp"
 #include "fs2d_batch_api.hpp"
 #include "fs2d_tune_api.hpp"
+#include "fs2d_render.hpp"
 #include "fs3d_api.hpp"

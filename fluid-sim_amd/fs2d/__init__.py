@@ -55,6 +55,19 @@ class Tuning(ctypes.Structure):
     _fields_ = [("pcg_grid", ctypes.c_int), ("sweep_prefetch", ctypes.c_int)]
 
 
+RENDER_BUFFERS = ("water_cells", "solid_cells", "cell_vel_lines", "pressure_cells", "pressure_values",
+                  "particle_lines", "phi_values")
+RENDER_COUNTS = ("water_cells", "solid_cells", "pressure_cells", "particles", "cell_vel_lines", "phi_values")
+
+
+class RenderTargets(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in RENDER_BUFFERS]
+
+
+class RenderCounts(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_long) for k in RENDER_COUNTS]
+
+
 class FS2DError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__("%s: %s" % (status_string(status), msg))
@@ -105,6 +118,7 @@ def load():
         "fs2d_batch_set_tuning": (i, [vp, ctypes.POINTER(Tuning)]),
         "fs2d_batch_autotune": (i, [vp, i, ctypes.c_char_p, ctypes.POINTER(Tuning), ctypes.POINTER(i)]),
         "fs2d_project_noalloc": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(d)]),
+        "fs2d_render_buffers": (i, [vp, ctypes.POINTER(RenderTargets), ctypes.POINTER(RenderCounts)]),
         "fs2d_set_timing": (i, [vp, i]),
         "fs2d_set_timing_classes": (i, [vp, ctypes.c_ulonglong]),
         "fs2d_kernel_count": (i, []),
@@ -325,6 +339,15 @@ class Sim:
     def set_graph(self, on):
         """Step by replaying captured HIP graphs (fs2d_set_graph)."""
         self._check(self.L.fs2d_set_graph(self.h, int(on)))
+
+    def render_buffers(self, **targets):
+        """The demo renderer's per-frame buffers written to device addresses
+        (fs2d_render_buffers); keyword arguments of RENDER_BUFFERS name the
+        buffers to fill (device pointers as ints).  Returns the counts."""
+        t = RenderTargets(*[targets.get(k) for k in RENDER_BUFFERS])
+        c = RenderCounts()
+        self._check(self.L.fs2d_render_buffers(self.h, ctypes.byref(t), ctypes.byref(c)))
+        return {k: getattr(c, k) for k in RENDER_COUNTS}
 
     def tuning(self):
         t = Tuning()

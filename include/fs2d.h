@@ -298,6 +298,36 @@ typedef struct {
  * they are not NULL; FS2D_WARN_MAXITER when pcg_max_iters was reached. */
 fs2d_status fs2d_project_noalloc(const fs2d_projection_args* args, int* iters, double* residual);
 
+/* Per-frame render buffers of the demo renderer, built on the device from the
+ * handle's state (FluidRenderer2D::updateBuffers, demo/FluidRenderer2D.cpp:
+ * 435-486), for a renderer that maps its vertex buffers into device memory
+ * (GL/Vulkan interop): the state never goes to the host.  Every pointer is a
+ * device address and may be NULL to skip that buffer; all are FP32, 2 floats
+ * per point, in the reference's push order (row-major cells, j outer).
+ *   water_cells, solid_cells  (i dx, j dx) of each FLUID / SOLID cell
+ *                             (capacity: size_x * size_y points each)
+ *   cell_vel_lines            per cell (i, j) with i < size_x - 1, j < size_y - 1:
+ *                             c = ((i + 1/2) dx, (j + 1/2) dx) and c + dt velInterp(c)
+ *   pressure_cells, pressure_values  (i dx, j dx) and sigmoid(0.01 p) of each
+ *                             cell with p != 0 (capacity: size_x * size_y)
+ *   particle_lines            per particle: x and x + dt velInterp(x)
+ *   phi_values                per cell: sigmoid(100 phi)
+ * sigmoid(x) = 1 / (1 + e^-x).  The counts go to *out (synchronises the
+ * handle's stream). */
+typedef struct {
+    float* water_cells;
+    float* solid_cells;
+    float* cell_vel_lines;
+    float* pressure_cells;
+    float* pressure_values;
+    float* particle_lines;
+    float* phi_values;
+} fs2d_render_targets;
+typedef struct {
+    long water_cells, solid_cells, pressure_cells, particles, cell_vel_lines, phi_values;
+} fs2d_render_counts;
+fs2d_status fs2d_render_buffers(fs2d_sim* sim, const fs2d_render_targets* targets, fs2d_render_counts* out);
+
 /*
  * Ensembles: many independent simulations of different sizes stepped
  * together.  No reference counterpart (the reference steps one FluidSim2D per

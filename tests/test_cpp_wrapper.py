@@ -4,7 +4,10 @@ tests/cpp/wrapper_demo.cpp drives the wrapper the way the reference's demo
 drives FluidSim2D (create, update() per frame, host writes to mac.u/v, gravity
 and picFlipAlpha between frames, saveStats, free).  The same sequence through
 the C ABI from Python must give bit-identical fields: the wrapper adds host
-mirrors and nothing else.
+mirrors and nothing else.  The same host writes applied through the C ABI and,
+independently, to the oracle must then agree with the oracle frame by frame,
+so the mutation path is checked against the reference's semantics and not
+only against itself.
 """
 import os
 import shutil
@@ -102,3 +105,49 @@ def test_wrapper_demo_matches_c_abi(tmp_path, mode, time_stages):
     assert [int(s[0]) for s in sc] == list(range(frames))
     assert abs(float(sc[-1][2]) - frames * 0.005) < 1e-12   # currentTime
     assert sc[-1][1] == "ApplyAdvection"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1])
+def test_mutation_path_against_oracle(mode):
+    """The demo's host writes between frames (FluidRenderer2D.cpp:305-308,
+    322, 383-388), applied through the C ABI on the GPU and independently to
+    the oracle, which restarts from the GPU's state before each frame's writes
+    (so a write the GPU applied wrongly shows up as a difference): flags
+    bit-exact, u, v, p and particle positions within REL every frame."""
+    import fs2d
+    from oracle import oracle as O
+    from test_gpu_configs import push
+    from test_gpu_parity import REL, rel_l2
+    n, frames = 48, 5
+    cells = O.dam_break(n, n)
+    kw = dict(k=2, dt=0.005, dx=1.0 / n, rho=997.0, gravity=(0.0, -9.81),
+              mode=fs2d.SEMILAGRANGIAN if mode == 0 else fs2d.PICFLIP, alpha=0.05)
+    g = fs2d.Sim(cells, tau=0.97, sigma=0.25, tol=1e-6, max_iters=500, **kw)
+    o = O.Oracle(cells, tau=0.97, sigma=0.25, tol=1e-12, max_iters=4000, sl_inplace=False, **kw)
+    for f in range(frames):
+        push(g, o)
+        if f == 2:
+            u, v = g.get(fs2d.F_U, np.float64), g.get(fs2d.F_V, np.float64)
+            ou, ov = o.get(O.F_U), o.get(O.F_V)
+            for a, b in ((u, v), (ou, ov)):
+                a[n // 4:n // 2, n // 5] += 0.5
+                b[n // 4:n // 2, n // 5] -= 0.25
+            g.set(fs2d.F_U, u)
+            g.set(fs2d.F_V, v)
+            o.set(O.F_U, ou)
+            o.set(O.F_V, ov)
+        if f == 3:
+            g.set_gravity(1.0, -12.0)
+            g.set_alpha(0.2)
+            o.set_params(0.005, (1.0, -12.0), 0.2)
+        g.step()
+        o.step()
+        np.testing.assert_array_equal(g.get(fs2d.F_CELL), o.get(O.F_CELL), err_msg="frame %d" % f)
+        e = dict(u=rel_l2(g.get(fs2d.F_U), o.get(O.F_U)), v=rel_l2(g.get(fs2d.F_V), o.get(O.F_V)),
+                 p=rel_l2(g.get(fs2d.F_P), o.get(O.F_P)),
+                 pos=rel_l2(g.get(fs2d.F_PARTICLE_POS), o.get(O.F_POS)))
+        print("mode %d frame %d: %s" % (mode, f, {k: "%.2e" % x for k, x in e.items()}))
+        assert all(x < REL for x in e.values()), (mode, f, e)
+    g.close()
+    o.close()
