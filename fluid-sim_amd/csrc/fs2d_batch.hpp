@@ -39,6 +39,7 @@ struct InstBufs {
     uint8_t* cell2;
     double *phi, *phi2;
     int *t0, *t1;
+    int* ls_chg;  // closest-particle pass change flags (16)
     float *p, *r, *z, *q, *w, *s0, *s1, *pk, *Adiag, *P, *fx, *fy, *gx_, *gy_;
     uint8_t* cpl;
     PcgScalars* sc;
@@ -549,23 +550,29 @@ __global__ __launch_bounds__(kBlock) void b_eik_sk_to_sk_kernel(const InstBufs* 
 // single path: candidates from t0 (first pass) or from the previous pass's
 // output (w, SK4 order of direction pdir); the last output back to t0.
 __global__ __launch_bounds__(kBlock) void b_ls_cand_sk_kernel(const InstBufs* __restrict__ ins, BlockMap bm, int dir,
-                                                              int first, int pdir) {
+                                                              int first, int pdir, int k) {
     int b, vb, nvb;
     if (!bmap(bm, b, vb, nvb)) return;
     const InstBufs& I = ins[b];
     ls_cand_sk_body(I.geom, first ? I.t0 : nullptr, I.w, pdir & 1, pdir >> 1, I.px, I.py, I.r, I.z, I.q, dir & 1,
-                    dir >> 1, I.dx, I.dr, vb, nvb);
+                    dir >> 1, I.dx, I.dr, I.ls_chg, k, vb, nvb);
 }
-__global__ __launch_bounds__(kBlock) void b_closest_t_kernel(const InstBufs* __restrict__ ins, BlockMap bm, int dir) {
+__global__ __launch_bounds__(kBlock) void b_closest_t_kernel(const InstBufs* __restrict__ ins, BlockMap bm) {
     int b, vb, nvb;
     if (!bmap(bm, b, vb, nvb)) return;
     const InstBufs& I = ins[b];
-    closest_t_to_rm_body(I.geom, I.w, I.t0, dir & 1, dir >> 1, vb, nvb);
+    closest_t_to_rm_body(I.geom, I.w, I.t0, I.ls_chg, vb, nvb);
+}
+// Every member's 16 pass change flags cleared (one thread per flag).
+__global__ __launch_bounds__(kBlock) void b_ls_chg_clear_kernel(const InstBufs* __restrict__ ins, int n) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < 16 * n) ins[e >> 4].ls_chg[e & 15] = 0;
 }
 template <int FX, int FY>
 struct ClosestSweepMaker {
+    int k;
     __device__ ClosestSweepOp<FX, FY> operator()(const InstBufs& I, int) const {
-        return ClosestSweepOp<FX, FY>{{I.r, I.z, I.q}, nullptr, {I.w}, I.nx, I.ny, I.dx, I.dr};
+        return ClosestSweepOp<FX, FY>{{I.r, I.z, I.q}, nullptr, {I.w}, I.nx, I.ny, I.dx, I.dr, I.ls_chg, k};
     }
 };
 struct EikSweepMaker {

@@ -110,7 +110,7 @@ InstBufs inst_bufs(const fs2d_sim* s, unsigned long long* sk_handoff) {
     I.rhs_zero = 1e-12 / solver_scale(s);
     I.geom = s->geom;
     I.u = s->u; I.v = s->v; I.u2 = s->u2; I.v2 = s->v2; I.nu = s->nu; I.nv = s->nv; I.dens = s->dens; I.dens2 = s->dens2;
-    I.cell = s->cell; I.cell2 = s->cell2; I.phi = s->phi; I.phi2 = s->phi2; I.t0 = s->t0; I.t1 = s->t1;
+    I.cell = s->cell; I.cell2 = s->cell2; I.phi = s->phi; I.phi2 = s->phi2; I.t0 = s->t0; I.t1 = s->t1; I.ls_chg = s->ls_chg;
     I.p = s->p; I.r = s->r; I.z = s->z; I.q = s->q; I.w = s->w; I.s0 = s->s[0]; I.s1 = s->s[1]; I.pk = s->pk;
     I.Adiag = s->Adiag; I.P = s->P; I.fx = s->fx; I.fy = s->fy; I.gx_ = s->gx_; I.gy_ = s->gy_; I.cpl = s->cpl;
     I.sc = s->sc; I.qq = s->qq; I.ws_part = s->ws_part; I.rmax_part = s->rmax_part;
@@ -301,35 +301,40 @@ void bstage_level_set(fs2d_batch* B) {
         const int S = B->strips.total;
         // closest-particle passes on the sweep engine, as the single path:
         // candidates in SK4 order of the pass's direction from the previous
-        // pass's output, the pass, and the last output back to t0
+        // pass's output, the pass, and the last output back to t0; a member's
+        // passes after its first one that changes nothing return at once
+        {
+            Timed t(B, K_LS_OTHER);
+            blaunch(B, b_ls_chg_clear_kernel, (16 * B->n + kBlock - 1) / kBlock, kBlock, B->d_ins, B->n);
+        }
         for (int k = 0; k < 16; k++) {
             const int d = k & 3;
             {
                 Timed t(B, K_LS_OTHER);
                 blaunch(B, b_ls_cand_sk_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells, d, k == 0 ? 1 : 0,
-                        (k + 3) & 3);
+                        (k + 3) & 3, k);
             }
             {
                 Timed t(B, K_LS_SWEEP);
                 switch (d) {
                     case 0: blaunch(B, b_sweep_kernel<ClosestSweepOp<0, 0>, false, ClosestSweepMaker<0, 0>>, S,
                                     kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
-                                    ClosestSweepMaker<0, 0>{}, (const int*)nullptr, 0); break;
+                                    ClosestSweepMaker<0, 0>{k}, (const int*)nullptr, 0); break;
                     case 1: blaunch(B, b_sweep_kernel<ClosestSweepOp<1, 0>, false, ClosestSweepMaker<1, 0>>, S,
                                     kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
-                                    ClosestSweepMaker<1, 0>{}, (const int*)nullptr, 0); break;
+                                    ClosestSweepMaker<1, 0>{k}, (const int*)nullptr, 0); break;
                     case 2: blaunch(B, b_sweep_kernel<ClosestSweepOp<0, 1>, false, ClosestSweepMaker<0, 1>>, S,
                                     kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
-                                    ClosestSweepMaker<0, 1>{}, (const int*)nullptr, 0); break;
+                                    ClosestSweepMaker<0, 1>{k}, (const int*)nullptr, 0); break;
                     default: blaunch(B, b_sweep_kernel<ClosestSweepOp<1, 1>, false, ClosestSweepMaker<1, 1>>, S,
                                      kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
-                                     ClosestSweepMaker<1, 1>{}, (const int*)nullptr, 0); break;
+                                     ClosestSweepMaker<1, 1>{k}, (const int*)nullptr, 0); break;
                 }
             }
         }
         {
             Timed t(B, K_LS_OTHER);
-            blaunch(B, b_closest_t_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells, 3);
+            blaunch(B, b_closest_t_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells);
         }
         {
             Timed t(B, K_LS_OTHER);

@@ -89,6 +89,7 @@ struct fs2d_sim {
     uint8_t* cell2 = nullptr;  // flags written by the bubble fill (FS2D_FLAG_PARTICLE_CELLS)
     double *phi = nullptr, *phi2 = nullptr;
     int *t0 = nullptr, *t1 = nullptr;
+    int* ls_chg = nullptr;  // 16 closest-particle pass change flags (ls_first_unchanged)
     // projection: row-major pressure + SK4 solver arrays (fs2d_skew.hpp)
     float* p = nullptr;
     SkGeom geom{};
@@ -348,9 +349,9 @@ void launch_sweep(fs2d_sim* s, const Op& op);
 int sk_blocks(const fs2d_sim* s);
 int pcg_blocks(const fs2d_sim* s);
 template <int FX, int FY>
-void launch_closest(fs2d_sim* s) {
+void launch_closest(fs2d_sim* s, int k) {
     launch_sweep<ClosestSweepOp<FX, FY>, false>(
-        s, ClosestSweepOp<FX, FY>{{s->r, s->z, s->q}, nullptr, {s->w}, s->nx, s->ny, s->dx, s->dr});
+        s, ClosestSweepOp<FX, FY>{{s->r, s->z, s->q}, nullptr, {s->w}, s->nx, s->ny, s->dx, s->dr, s->ls_chg, k});
 }
 
 // FluidSim2D::createWaterLevelSet (src/FluidSim2D.cpp:653-732).
@@ -378,28 +379,31 @@ void stage_level_set(fs2d_sim* s) {
         // >> 1) on the sweep engine: candidates in SK4 order of the pass's
         // flipped coordinates (the solver's scratch: r, z, q), read from t0
         // (row-major) for the first pass and from the previous pass's output
-        // (w, SK4 order of its direction) after; the last output back to t0
+        // (w, SK4 order of its direction) after; the passes after the first
+        // one that changes nothing return at once (ls_first_unchanged); the
+        // last output back to t0
+        fill(s, s->ls_chg, 0, 16);
         for (int k = 0; k < 16; k++) {
             const int d = k & 3, pd = (k + 3) & 3;
             {
                 Timed t(s, K_LS_OTHER);
                 hipLaunchKernelGGL(ls_cand_sk_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->geom,
                                    k == 0 ? s->t0 : nullptr, (const float*)s->w, pd & 1, pd >> 1, s->px, s->py, s->r,
-                                   s->z, s->q, d & 1, d >> 1, s->dx, s->dr);
+                                   s->z, s->q, d & 1, d >> 1, s->dx, s->dr, s->ls_chg, k);
             }
             Timed t(s, K_LS_SWEEP);
             switch (d) {
-                case 0: launch_closest<0, 0>(s); break;
-                case 1: launch_closest<1, 0>(s); break;
-                case 2: launch_closest<0, 1>(s); break;
-                default: launch_closest<1, 1>(s); break;
+                case 0: launch_closest<0, 0>(s, k); break;
+                case 1: launch_closest<1, 0>(s, k); break;
+                case 2: launch_closest<0, 1>(s, k); break;
+                default: launch_closest<1, 1>(s, k); break;
             }
         }
         int* const tin = s->t0;
         {
             Timed t(s, K_LS_OTHER);
             hipLaunchKernelGGL(closest_t_to_rm_kernel, dim3(blocks_for(nc)), dim3(kBlock), 0, s->stream, s->geom,
-                               s->w, tin, 1, 1);
+                               s->w, tin, (const int*)s->ls_chg);
         }
         double* a = s->phi;
         double* b = s->phi2;
@@ -980,7 +984,7 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
 #define A(p, n) \
     if ((st = dalloc(s, &s->p, (n))) != FS2D_OK) return bail(st)
     A(u, nU); A(v, nV); A(u2, nU); A(v2, nV); A(nu, nU); A(nv, nV);
-    A(dens, nC); A(dens2, nC); A(cell, nC); A(phi, nC); A(phi2, nC); A(t0, nC); A(t1, nC);
+    A(dens, nC); A(dens2, nC); A(cell, nC); A(phi, nC); A(phi2, nC); A(t0, nC); A(t1, nC); A(ls_chg, 16);
     if (s->flags & FS2D_FLAG_PARTICLE_CELLS) A(cell2, nC);
     s->geom = make_geom(s->nx, s->ny);
     const size_t nS = (size_t)s->geom.size();

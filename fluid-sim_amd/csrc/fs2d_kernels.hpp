@@ -1431,11 +1431,27 @@ __device__ __forceinline__ LsCand ls_cand_cell(TRead tr, const float* __restrict
 // SK4 arrays are the solver's scratch (r, z, q in; w out), free between
 // projections.
 constexpr int kCandPosShift = 29;  // particle indices below 2^29 - 1
+// The passes stop early.  A pass (candidates and sweep) that changes no
+// cell's adopted particle leaves every cell's particle at least as close as
+// each of its four neighbours' (every cell of a pass compares all four; the
+// direction only orders the visits, and ties never move a particle), so every
+// later pass, in any direction, changes nothing either: the reference's
+// remaining passes are the identity.  chg[k] != 0 records that pass k changed
+// a cell, or may have (a sweep adoption that keeps the same particle counts
+// too); the passes after the first unchanged one return at once, and the
+// state is read from that pass's output.  chg holds 16 ints, cleared before
+// pass 0.
+__device__ __forceinline__ int ls_first_unchanged(const int* __restrict__ chg, int k) {
+    for (int p = 0; p < k; p++)
+        if (chg[p] == 0) return p;
+    return -1;
+}
 __device__ __forceinline__ void ls_cand_sk_body(SkGeom g, const int* __restrict__ t_rm, const float* __restrict__ t_sk,
                                                 int pfx, int pfy, const float* __restrict__ px,
                                                 const float* __restrict__ py, float* __restrict__ w0,
                                                 float* __restrict__ w1, float* __restrict__ w2, int fx, int fy,
-                                                double dx, double dr, int vb, int nvb) {
+                                                double dx, double dr, int* chg, int k, int vb, int nvb) {
+    if (ls_first_unchanged(chg, k) >= 0) return;  // the same for every thread
     // SK4 entries of this pass, so the three stores coalesce; the sweep never
     // reads a padding entry's record (steps outside the grid reset its
     // state), so padding entries are skipped
@@ -1445,12 +1461,14 @@ __device__ __forceinline__ void ls_cand_sk_body(SkGeom g, const int* __restrict_
         return (int)__float_as_uint(t_sk[g.idx(pfx ? nx - 1 - i : i, pfy ? ny - 1 - j : j)]);
     };
     const long n = g.size();
+    bool changed = false;
     for (long e = vb * (long)blockDim.x + threadIdx.x; e < n; e += (long)nvb * blockDim.x) {
         int a, b;
         if (!sk_cell(g, e, a, b)) continue;
         const int i = fx ? nx - 1 - a : a, j = fy ? ny - 1 - b : b;
         const LsCand best = t_rm ? ls_cand_cell(tr_rm, px, py, nx, ny, i, j, fx, fy, dx, dr)
                                  : ls_cand_cell(tr_sk, px, py, nx, ny, i, j, fx, fy, dx, dr);
+        changed |= best.t != (t_rm ? tr_rm(i, j) : tr_sk(i, j));
         uint32_t r0 = 0;
         float rx = 0.0f, ry = 0.0f;
         if (best.t >= 0) {
@@ -1462,18 +1480,22 @@ __device__ __forceinline__ void ls_cand_sk_body(SkGeom g, const int* __restrict_
         w1[e] = rx;
         w2[e] = ry;
     }
+    if (__syncthreads_or(changed) && threadIdx.x == 0) atomicOr(chg + k, 1);
 }
 __global__ __launch_bounds__(kBlock) void ls_cand_sk_kernel(SkGeom g, const int* __restrict__ t_rm,
                                                             const float* __restrict__ t_sk, int pfx, int pfy,
                                                             const float* __restrict__ px, const float* __restrict__ py,
                                                             float* __restrict__ w0, float* __restrict__ w1,
                                                             float* __restrict__ w2, int fx, int fy, double dx,
-                                                            double dr) {
-    ls_cand_sk_body(g, t_rm, t_sk, pfx, pfy, px, py, w0, w1, w2, fx, fy, dx, dr, blockIdx.x, gridDim.x);
+                                                            double dr, int* chg, int k) {
+    ls_cand_sk_body(g, t_rm, t_sk, pfx, pfy, px, py, w0, w1, w2, fx, fy, dx, dr, chg, k, blockIdx.x, gridDim.x);
 }
-// The adopted indices of a pass (SK4 order of (fx, fy)) back to row-major.
+// The adopted indices back to row-major, from the output of the last pass
+// that ran (SK4 order of its direction; pass 15 when every pass changed).
 __device__ __forceinline__ void closest_t_to_rm_body(SkGeom g, const float* __restrict__ tk, int* __restrict__ t,
-                                                     int fx, int fy, int vb, int nvb) {
+                                                     const int* __restrict__ chg, int vb, int nvb) {
+    const int p = ls_first_unchanged(chg, 16);
+    const int d = (p >= 0 ? p : 15) & 3, fx = d & 1, fy = d >> 1;
     const long n = (long)g.nx * g.ny;
     for (long c = vb * (long)blockDim.x + threadIdx.x; c < n; c += (long)nvb * blockDim.x) {
         const int i = (int)(c % g.nx), j = (int)(c / g.nx);
@@ -1481,8 +1503,8 @@ __device__ __forceinline__ void closest_t_to_rm_body(SkGeom g, const float* __re
     }
 }
 __global__ __launch_bounds__(kBlock) void closest_t_to_rm_kernel(SkGeom g, const float* __restrict__ tk,
-                                                                 int* __restrict__ t, int fx, int fy) {
-    closest_t_to_rm_body(g, tk, t, fx, fy, blockIdx.x, gridDim.x);
+                                                                 int* __restrict__ t, const int* __restrict__ chg) {
+    closest_t_to_rm_body(g, tk, t, chg, blockIdx.x, gridDim.x);
 }
 
 // The sweep operator: carried state (t + 1, x, y) of the adopted particle (0:
@@ -1497,7 +1519,10 @@ struct ClosestSweepOp {
     float* out[NOUT];      // adopted particle index (bits; -1: none)
     int nx, ny;
     double dx, dr;
-    __device__ bool skip(bool) const { return false; }
+    int* chg;              // per-pass change flags (ls_first_unchanged)
+    int k;                 // this pass
+    bool changed = false;  // an adoption in this lane's cells
+    __device__ bool skip(bool) const { return ls_first_unchanged(chg, k) >= 0; }
     // The loaders also compute the candidate's distance (the ring's words 3
     // and 4: its low and high halves), which takes one of the compute wave's
     // three FP64 corner distances per step off the chain.
@@ -1527,10 +1552,11 @@ struct ClosestSweepOp {
     __device__ void loader_done(int, int, double) const {}
     template <bool B, bool C>
     __device__ void group(const float4 (&x)[NF], uint32_t, uint32_t (&st)[NW], const uint4 (&hb)[NW],
-                          float (&o)[NOUT][4], uint32_t (&prod)[NW][4], int t0, int s) const {
+                          float (&o)[NOUT][4], uint32_t (&prod)[NW][4], int t0, int s) {
         const int lane = threadIdx.x & 63;
         const int b = 64 * s + lane;
         const int j = FY ? ny - 1 - b : b;
+        bool took = false;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             uint32_t nb[NW];
@@ -1552,6 +1578,7 @@ struct ClosestSweepOp {
                 const float qx = u2f(sw[1]), qy = u2f(sw[2]);
                 const double d = corner_dist(i, j, qx, qy, dx, dr);
                 const bool take = (tn >= 0) & ((d < bd) | ((d == bd) & (pos < bp)));
+                took |= take & ing;
                 bd = take ? d : bd;
                 bt = take ? tn : bt;
                 bp = take ? pos : bp;
@@ -1567,8 +1594,12 @@ struct ClosestSweepOp {
             for (int w = 0; w < NW; w++) prod[w][q] = st[w];
             o[0][q] = __uint_as_float((uint32_t)bt);
         }
+        changed |= took;
     }
-    __device__ void strip_done(int) const {}
+    // wave 0 after the strip's last step: one flag write per strip that adopted
+    __device__ void strip_done(int) const {
+        if (__any(changed) && (threadIdx.x & 63) == 0) atomicOr(chg + k, 1);
+    }
 };
 
 // phi from the adopted particle; then the surface marks (:803-834) and the

@@ -106,6 +106,49 @@ def test_level_set_bit_exact(n, mode):
         g.step()  # move on to a generic state
 
 
+# Particle sets whose closest-particle passes first change nothing at pass 4
+# (direction 0), pass 2 (direction 2) and, for the seeded dam break, pass 3
+# (direction 3): the passes after it return at once and the adopted particles
+# are read back from that pass's output, in its direction's order.
+LS_EARLY_EXIT = {
+    "five particles": [[0.9742, 0.6448], [0.2478, 0.4382], [0.9505, 0.8778], [0.827, 0.3978], [0.4934, 0.6679]],
+    "one particle": [[0.8536, 0.0426]],
+    "dam break": None,
+}
+
+
+def test_level_set_early_exit_bit_exact():
+    """The closest-particle passes stop after the first pass that changes no
+    cell (a fixed point of every later pass): flags and phi stay bit-exact
+    against the oracle, which runs all 16 passes, whichever direction that
+    pass had; and a batch whose members stop at different passes matches."""
+    n = 48
+    kw = dict(k=2, dt=0.005, dx=1.0 / n, mode=fs2d.PICFLIP, alpha=0.05)
+    members = []
+    for name, pts in LS_EARLY_EXIT.items():
+        g, o = pair(O.dam_break(n, n), **kw)
+        if pts is not None:
+            g.set_particles(np.array(pts, np.float64))
+        a = g.get(fs2d.F_PARTICLE_POS).astype(np.float64)  # the GPU's FP32 positions
+        o.set(O.F_POS, a)
+        o.set(O.F_VEL, np.zeros_like(a))
+        g.run_stage(fs2d.STAGE_LEVEL_SET)
+        o.run_stage(O.ST_LEVELSET)
+        np.testing.assert_array_equal(g.get(fs2d.F_CELL), o.get(O.F_CELL), err_msg=name)
+        np.testing.assert_array_equal(g.get(fs2d.F_PHI, np.float64), o.get(O.F_PHI), err_msg=name)
+        members.append((name, pts, g.get(fs2d.F_PHI, np.float64), g.get(fs2d.F_CELL)))
+        g.close()
+    batch = fs2d.Batch([(O.dam_break(n, n), kw) for _ in members])
+    for m, (name, pts, _, _) in zip(batch.sims, members):
+        if pts is not None:
+            m.set_particles(np.array(pts, np.float64))
+    batch.run_stage(fs2d.STAGE_LEVEL_SET)
+    for m, (name, _, phi, cells) in zip(batch.sims, members):
+        np.testing.assert_array_equal(m.get(fs2d.F_PHI, np.float64), phi, err_msg="batch " + name)
+        np.testing.assert_array_equal(m.get(fs2d.F_CELL), cells, err_msg="batch " + name)
+    batch.close()
+
+
 def test_sl_advection_stage():
     n = 64
     cells = O.dam_break(n, n)
