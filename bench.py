@@ -438,8 +438,8 @@ def main():
                     help="run one leg alone and print its object (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(each GPU would run its own independent grids; see DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path "
+              "(each GPU would run its own independent grids)", file=sys.stderr)
         sys.exit(2)
 
     # read before OpenMP starts: with OMP_PROC_BIND its runtime pins this

@@ -1,7 +1,7 @@
 """GPU parity: every stage of the HIP path against the CPU oracle from an
 identical state, plus teacher-forced whole steps.
 
-Bars (north star): cell flags, particle cell indices and the level set's
+Bars: cell flags, particle cell indices and the level set's
 discrete decisions bit-exact; velocities, pressures and particle positions
 within 1e-5 relative L2 of the FP64 oracle.  The GPU stores FP32; the oracle
 is given the GPU's FP32 values (widened to FP64) so that both start from the
