@@ -17,7 +17,10 @@ every step runs into the iteration cap.
 
 `roofline` reports the dominant kernel class: algorithmic bytes per launch
 (BYTES_PER_CELL x cells) over its average launch time, measured with HIP
-events on the library's stream inside the timed region.  Launches that start
+events on the library's stream inside the timed region.  With the tuning's
+fused_spmv (the default) the backward-solve class is one launch that also
+runs the next iteration's SpMV, and it is reported as
+`pcg_backward_solve+spmv` with both kernels' arrays.  Launches that start
 after convergence exit at once; the average is taken over the launches that
 did work (PCG iterations + 1 per step for the two solves).  Each timed launch
 adds two events to the stream, and events around every kernel cost ~8% of
@@ -92,6 +95,9 @@ PROFILE_STEPS = 2  # steps with every kernel class timed, outside the timed regi
 BYTES_PER_CELL = {
     "pcg_forward_solve": 20,    # r, P, fx, fy in; q out
     "pcg_backward_solve": 20,   # q, P, gx, gy in; z out
+    # the backward solve with the next SpMV in one launch: q, P, gx, gy in, z
+    # out; s, Adiag, coupling byte in, s', w out (z and q counted once)
+    "pcg_backward_solve+spmv": 37,
     "pcg_spmv_dot": 21,         # z, s, Adiag, coupling byte in; s', w out
     "pcg_axpy_norm": 24,        # p, r, s, w in; p, r out
     "mic0_factor": 25,          # Adiag, coupling byte in; P, fx, fy, gx, gy out
@@ -471,7 +477,7 @@ def main():
         s_.step()
     tune = sim.autotune(iters=24, cache=args.tune_cache)
     if gsim:
-        gsim.set_tuning(pcg_grid=tune["pcg_grid"])
+        gsim.set_tuning(pcg_grid=tune["pcg_grid"], fused_spmv=tune["fused_spmv"])
     for _ in range(args.warmup):
         for s_ in twins:
             s_.step()
@@ -524,7 +530,8 @@ def main():
     if dom in EFFECTIVE:
         launches = int(sum(iters)) + EFFECTIVE[dom] * args.steps
     avg_s = tot_ms / 1e3 / max(launches, 1)
-    bytes_per_launch = BYTES_PER_CELL[dom] * n * n
+    label = dom + "+spmv" if dom == "pcg_backward_solve" and tune.get("fused_spmv") else dom
+    bytes_per_launch = BYTES_PER_CELL[label] * n * n
     achieved = bytes_per_launch / avg_s / 1e9
     out = {
         "metric": METRIC,
@@ -550,8 +557,8 @@ def main():
         "graph": graph_out,
         **no_graph,
         "roofline": {
-            "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, n),
+            "bound": "hbm", "kernel": label, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(label, n),
             "avg_launch_us": avg_s * 1e6, "launches": launches, "launches_all": kt[dom][1],
             "bytes_per_launch": bytes_per_launch,
         },

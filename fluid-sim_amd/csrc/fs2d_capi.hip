@@ -101,6 +101,15 @@ struct fs2d_sim {
     double *qq = nullptr, *ws_part = nullptr, *rmax_part = nullptr;
     unsigned long long* sk_ticket = nullptr;
     unsigned long long* sk_handoff = nullptr;
+    // the backward solve with the next SpMV in one launch (bwd_spmv_kernel):
+    // its own ticket, hand-off rows, per-strip completion flags and tagged
+    // sigma words
+    bool fused_spmv = true;
+    int ncu = 0;  // compute units of the device
+    unsigned long long* fb_ticket = nullptr;
+    unsigned long long* fb_handoff = nullptr;
+    unsigned* fb_flags = nullptr;
+    unsigned long long* sig_words = nullptr;
     // extrapolation
     uint32_t *mu = nullptr, *mv = nullptr;
     int *fu0 = nullptr, *fv0 = nullptr;    // entries in layer order
@@ -540,9 +549,33 @@ int pcg_launch(const fs2d_sim* s) {
     return s->pcg_grid > 0 ? std::min(nvb, s->pcg_grid) : nvb;
 }
 
+// The backward solve of iteration h (next_iter = h + 1; 0 for the initial
+// solve) and the SpMV of iteration h + 1 in one launch (bwd_spmv_kernel):
+// sigma_{h+1} goes to sig_out, beta = sigma_{h+1} / sigma_h with sigma_h in
+// sig_prev (unused when first), s_new = z + beta s_old, w = A s_new.
+// Workgroups: K for sigma (one (strip, loader) pair per wave), the S strips,
+// and M for the SpMV, which together fill the device once (one workgroup per
+// CU fits).  More SpMV workgroups than that measured far slower: a workgroup
+// waits for a CU of its own XCD, and on XCDs whose CUs all hold strips or
+// other SpMV workgroups its statically assigned runs start only at the end.
+void launch_bwd_spmv(fs2d_sim* s, int next_iter, double* sig_out, const double* sig_prev, int first,
+                     const float* s_old, float* s_new) {
+    const SkGeom& g = s->geom;
+    const int waves = kSweepThreads / 64;
+    const int K = (g.S * kLoaders + waves - 1) / waves;
+    const int nbp = pcg_blocks(s);
+    const int M = std::max(8, std::min(nbp, (s->ncu > 0 ? s->ncu : 256) - g.S - K));
+    BwdFusedOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, nullptr, next_iter, s->fb_flags, 0u};
+    FusedSpmvArgs f{s->Adiag, s->cpl, s->z,       s->q,          s_old, s_new, s->w, s->sc, sig_out, s->sig_words,
+                    sig_prev, first,  s->ws_part, nbp, s->fb_flags, K,     M};
+    hipLaunchKernelGGL(bwd_spmv_kernel, dim3(K + g.S + M), dim3(kSweepThreads), 0, s->stream, op, g,
+                       SkCtl{s->fb_ticket, s->fb_handoff, s->werr}, f);
+}
+
 // MIC(0) factor and the first preconditioner application of the PCG
 // (src/FluidSim2D.cpp:364-430) on the assembled matrix and the rhs in r (with
-// p = s = 0 and the |rhs|_inf partials in rmax_part).
+// p = s = 0 and the |rhs|_inf partials in rmax_part).  With fused_spmv the
+// backward solve's launch also runs iteration 0's SpMV.
 void pcg_begin(fs2d_sim* s) {
     const int nb = sk_blocks(s);
     hipLaunchKernelGGL(pcg_init_kernel, dim3(1), dim3(64), 0, s->stream, s->sc, s->tol);
@@ -560,15 +593,21 @@ void pcg_begin(fs2d_sim* s) {
     }
     {
         Timed t(s, K_PCG_BWD);
-        BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, s->qq, 0};
-        launch_sweep<BwdOp, true>(s, op);
+        if (s->fused_spmv) {
+            launch_bwd_spmv(s, 0, s->qq, nullptr, 1, s->s[0], s->s[1]);
+        } else {
+            BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, s->qq, 0};
+            launch_sweep<BwdOp, true>(s, op);
+        }
     }
 }
 
 // Iterations h0 .. h1-1 of the PCG loop (:431-463): SpMV with the s update,
 // the p/r update, forward and backward solves.  Every kernel returns at once
 // once the stopping test has passed, so a fixed number of iterations can be
-// queued (or captured) without host round trips.
+// queued (or captured) without host round trips.  With fused_spmv iteration
+// h's SpMV ran in the previous backward solve's launch, and this iteration's
+// backward launch runs iteration h + 1's.
 void pcg_iterations(fs2d_sim* s, int h0, int h1) {
     const SkGeom& g = s->geom;
     double* qq0 = s->qq;
@@ -579,7 +618,7 @@ void pcg_iterations(fs2d_sim* s, int h0, int h1) {
         float* s_new = s->s[(h + 1) & 1];
         double* sig_cur = (h & 1) ? qq1 : qq0;   // sigma_h
         double* sig_prev = (h & 1) ? qq0 : qq1;  // sigma_{h-1}; receives sigma_{h+1}
-        {
+        if (!s->fused_spmv) {
             Timed t(s, K_PCG_SPMV);
             hipLaunchKernelGGL(pcg_spmv_sk_kernel, dim3(grid), dim3(kBlock), 0, s->stream, g, s->Adiag, s->cpl, s->z,
                                s_old, s_new, s->w, s->sc, sig_cur, sig_prev, h == 0 ? 1 : 0, s->ws_part,
@@ -597,8 +636,12 @@ void pcg_iterations(fs2d_sim* s, int h0, int h1) {
         }
         {
             Timed t(s, K_PCG_BWD);
-            BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, sig_prev, h + 1};
-            launch_sweep<BwdOp, true>(s, op);
+            if (s->fused_spmv) {
+                launch_bwd_spmv(s, h + 1, sig_prev, sig_cur, 0, s_new, s_old);
+            } else {
+                BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, sig_prev, h + 1};
+                launch_sweep<BwdOp, true>(s, op);
+            }
         }
     }
 }
@@ -1000,7 +1043,10 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
     A(werr, 1);
     const int nstrips = (s->ny + 63) / 64;
     A(sk_handoff, (size_t)nstrips * sk_handoff_stride(s->nx, 3));
+    A(fb_ticket, 1); A(fb_flags, nstrips); A(sig_words, 2 * (size_t)nstrips * kLoaders);
+    A(fb_handoff, (size_t)nstrips * sk_handoff_stride(s->nx, BwdFusedOp::NW));
 #undef A
+    if (hipDeviceGetAttribute(&s->ncu, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess) s->ncu = 0;
     e = hipHostMalloc((void**)&s->h_sc, sizeof(PcgScalars), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_dg, sizeof(Diag), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_err, sizeof(int), hipHostMallocDefault);
@@ -1608,6 +1654,7 @@ fs2d_status fs2d_project_noalloc(const fs2d_projection_args* a, int* iters, doub
     t.sk_ticket = (unsigned long long*)q;        q += kNaWord;
     t.werr = (int*)q;                            q += kNaWord;
     t.sk_handoff = (unsigned long long*)q;
+    t.fused_spmv = false;  // the separate SpMV launch: no completion flags or tagged sigma in the scratch
     PcgScalars h_sc{};
     t.h_sc = &h_sc;  // pageable: read back with a synchronising copy
 

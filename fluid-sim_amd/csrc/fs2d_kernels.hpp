@@ -48,12 +48,14 @@ __device__ __forceinline__ T wave_max(T v) {
     }
     return v;
 }
-// Sum over a 256-thread block in a fixed order; result valid in thread 0.
+// Sum over a 256-thread block in a fixed order; result valid in thread 0.  In
+// a larger block only the first 256 threads contribute (the others still take
+// part in the barriers).
 __device__ __forceinline__ double block_sum(double v) {
     __shared__ double sh[kBlock / 64];
     v = wave_sum(v);
     __syncthreads();
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    if ((threadIdx.x & 63) == 0 && threadIdx.x < kBlock) sh[threadIdx.x >> 6] = v;
     __syncthreads();
     double t = 0;
     if (threadIdx.x == 0)
@@ -213,13 +215,19 @@ __global__ void gravity_kernel(float* __restrict__ u, float* __restrict__ v, lon
 struct XcdLoop {
     long e, end, stride;
 };
-__device__ __forceinline__ XcdLoop xcd_loop(long n, int vb, int nvb) {
+// xcd_loop_t: the same for a virtual workgroup of `bd` threads run by
+// threads 0 .. bd-1 of a larger block (the others get no entries).
+__device__ __forceinline__ XcdLoop xcd_loop_t(long n, int vb, int nvb, int bd, int tid) {
+    if (tid >= bd) return {n, n, 1};
     const int nb = nvb;
-    if (nb & 7) return {vb * (long)blockDim.x + threadIdx.x, n, (long)nb * blockDim.x};
+    if (nb & 7) return {vb * (long)bd + tid, n, (long)nb * bd};
     const int x = vb & 7, k = vb >> 3;
     const long chunk = (n + 7) / 8;
     const long b0 = x * chunk, b1 = b0 + chunk < n ? b0 + chunk : n;
-    return {b0 + k * (long)blockDim.x + threadIdx.x, b1, (long)(nb >> 3) * blockDim.x};
+    return {b0 + k * (long)bd + tid, b1, (long)(nb >> 3) * bd};
+}
+__device__ __forceinline__ XcdLoop xcd_loop(long n, int vb, int nvb) {
+    return xcd_loop_t(n, vb, nvb, (int)blockDim.x, (int)threadIdx.x);
 }
 
 // Decode an SK4 element index into its cell; false for padding entries.
@@ -340,9 +348,22 @@ __device__ __forceinline__ long sk_above_edge(const SkGeom& g, long e) {
     const long e4 = e & 3;
     return ((e >> 8) + g.T4 + ((e4 - 63) >> 2)) * 256 + ((e4 + 1) & 3);
 }
-// One virtual workgroup (vb of nvb) of the SpMV: the partial w.s of the
-// entries xcd_loop gives it goes to partials[vb].  The entries' own values do
-// not depend on beta, so the first entry's are loaded (spmv_pre) before the
+// Entries of SpMV virtual workgroup vb of nvb (256 threads): a contiguous run
+// of whole 256-entry groups, the n / 256 groups split as evenly as possible;
+// thread tid takes entry tid of each group (threads past 256 of a larger block
+// take none).  A strip's entries are contiguous, so a run touches one or two
+// strips (and reads one more on either side): the fused backward launch starts
+// a run as soon as those strips have finished.  The neighbours (i +- 1, j) at
+// +-253 lie in the run's own groups except at its two ends.
+__device__ __forceinline__ XcdLoop spmv_range(long n, int vb, int nvb, int tid) {
+    if (tid >= kBlock) return {n, n, 1};
+    const long ng = n >> 8;
+    const long g0 = ng * vb / nvb, g1 = ng * (vb + 1) / nvb;
+    return {g0 * 256 + tid, g1 * 256, 256};
+}
+// One virtual workgroup (vb of nvb) of the SpMV: the partial w.s of its
+// entries (spmv_range) goes to partials[vb].  The entries' own values do not
+// depend on beta, so the first entry's are loaded (spmv_pre) before the
 // reductions that give beta, and each later entry's at the end of the
 // previous round.
 struct SpmvPre {
@@ -354,7 +375,7 @@ __device__ __forceinline__ SpmvPre spmv_pre(const SkGeom& g, const float* __rest
                                             const uint8_t* __restrict__ flags, const float* __restrict__ z,
                                             const float* __restrict__ s_old, int vb, int nvb) {
     SpmvPre r;
-    r.it = xcd_loop(g.size(), vb, nvb);
+    r.it = spmv_range(g.size(), vb, nvb, (int)threadIdx.x);
     r.so = r.zc = r.ad = 0.0f;
     r.f = 0;
     const long e = r.it.e;
@@ -422,12 +443,67 @@ __device__ __forceinline__ void spmv_core(const SpmvPre& pre, const SkGeom& g, c
     double bs = block_sum(acc);
     if (threadIdx.x == 0) partials[vb] = bs;
 }
+// spmv_core for a launch of few workgroups per CU: U entries of the thread
+// at a time, every load of them (the entry, and both operands of all four
+// neighbours, indices clamped into the array) issued together, so a virtual
+// workgroup pays two memory round trips per U entries instead of two per
+// entry.  The same entries, arithmetic and summation order as spmv_core.
+template <int U>
+__device__ __forceinline__ void spmv_core_wide(const XcdLoop it, const SkGeom& g, const float* __restrict__ Adiag,
+                                               const uint8_t* __restrict__ flags, const float* __restrict__ z,
+                                               const float* __restrict__ s_old, float* __restrict__ s_new,
+                                               float* __restrict__ w, float beta, double* partials, int vb) {
+    const long last = g.size() - 1;
+    double acc = 0;
+    for (long e0 = it.e; e0 < it.end; e0 += U * it.stride) {
+        float so[U], zc[U], ad[U], ns[U][4], nz[U][4];
+        uint32_t f[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const long e = e0 + u * it.stride;
+            const long ec = e < it.end ? e : e0;
+            const int l = (int)((ec >> 2) & 63);
+            long nb[4] = {sk_step_back(ec), sk_step_fwd(ec), l > 0 ? sk_step_back(ec) - 4 : sk_below_edge(g, ec),
+                          l < 63 ? sk_step_fwd(ec) + 4 : sk_above_edge(g, ec)};
+            so[u] = s_old[ec];
+            zc[u] = z[ec];
+            ad[u] = Adiag[ec];
+            f[u] = flags[ec];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const long m = nb[k] < 0 ? 0 : (nb[k] > last ? last : nb[k]);
+                ns[u][k] = s_old[m];
+                nz[u][k] = z[m];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const long e = e0 + u * it.stride;
+            if (e < it.end) {
+                const float sc_ = fmaf(beta, so[u], zc[u]);
+                s_new[e] = sc_;
+                float wv = 0.0f;
+                if (ad[u] != 0.0f) {
+                    float nsum = 0.0f;
+                    if (f[u] & CPL_L) nsum += fmaf(beta, ns[u][0], nz[u][0]);
+                    if (f[u] & CPL_R) nsum += fmaf(beta, ns[u][1], nz[u][1]);
+                    if (f[u] & CPL_D) nsum += fmaf(beta, ns[u][2], nz[u][2]);
+                    if (f[u] & CPL_U) nsum += fmaf(beta, ns[u][3], nz[u][3]);
+                    wv = fmaf(ad[u], sc_, -nsum);
+                }
+                w[e] = wv;
+                acc += (double)wv * (double)sc_;
+            }
+        }
+    }
+    double bs = block_sum(acc);
+    if (threadIdx.x == 0) partials[vb] = bs;
+}
 // s' = z + beta s (:459, FMA as Array2D.h:185), w = A s' (:433-444), partial
 // w.s' (:450), over nvb virtual workgroups: workgroup k of the launch runs
 // vb = k, k + gridDim.x, ...  Each vb computes what a launch of nvb
 // workgroups computes in its workgroup vb, so the partials, and every result,
-// do not depend on the launch's size.  A launch size that is a multiple of 8
-// keeps vb % 8 the XCD (xcd_loop).  sigma goes to the scalars (vb 0).
+// do not depend on the launch's size.  sigma goes to the scalars (vb 0).
 __global__ __launch_bounds__(kBlock) void pcg_spmv_sk_kernel(
     SkGeom g, const float* __restrict__ Adiag, const uint8_t* __restrict__ flags, const float* __restrict__ z,
     const float* __restrict__ s_old, float* __restrict__ s_new, float* __restrict__ w,
@@ -523,6 +599,207 @@ __global__ __launch_bounds__(kBlock) void pcg_axpy_sk_kernel(
         if (vb >= nvb) break;
         it = xcd_loop(g.size() / 4, vb, nvb);
         B.load(it, it.e, p4, r4, s4, w4);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The backward solve and the next iteration's SpMV in one launch
+// (src/FluidSim2D.cpp:410-421 then :457-459, :433-444, :450).  The backward
+// strips finish top-down about 3 us apart and the last one runs long after
+// most of the chip has gone idle; the SpMV of a region needs only z of its
+// own and the neighbouring strips, so it runs beneath the solve's tail.
+// Workgroups take tickets of the launch (tickets / launch = K + S + M):
+//   0 .. K-1        sigma: q.q per (strip, loader) in the order the solve's
+//                   loaders would sum it (bwd_sigma), published as tagged
+//                   words {half, epoch} (no fences) and as plain doubles for
+//                   later launches
+//   K .. K+S-1      the backward solve's strips (BwdFusedOp), each publishing
+//                   its completion (release, agent scope)
+//   K+S .. K+S+M-1  SpMV: sigma from the tagged words, beta as spmv_scalars
+//                   forms it, then the virtual workgroups of pcg_spmv_sk_kernel
+//                   (the same entries, arithmetic and partials) in the order
+//                   their strips finish, each after its strips' flags
+// Every wait is on a workgroup with a smaller ticket, which started earlier,
+// so the launch completes at any residency; every wait is bounded and raises
+// the error flag on expiry.  Results are bit-identical to the separate
+// launches.  No workgroup on another XCD reads a strip's z before its flag
+// (z lines do not straddle strips), so none can hold a stale copy.
+// ---------------------------------------------------------------------------
+struct FusedSpmvArgs {
+    const float* Adiag;
+    const uint8_t* flags;
+    const float* z;
+    const float* q;
+    const float* s_old;
+    float* s_new;
+    float* w;
+    PcgScalars* sc;
+    double* sig_out;                // sigma_{k+1} partials [S][kLoaders], plain
+    unsigned long long* sig_words;  // the same as {low half, epoch}, {high half, epoch}
+    const double* sig_prev;         // sigma_k partials (an earlier launch's), plain
+    int first;                      // no beta (the initial solve)
+    double* ws_partials;            // [nvb]
+    int nvb;                        // SpMV virtual workgroups, a multiple of 8
+    const unsigned* strip_flags;
+    int K, M;
+};
+
+// sigma workgroup k of K: one (strip, loader) pair per wave; the lane sums its
+// q.q over the loader's supergroups (sg = me, me + kLoaders, ... in the
+// backward solve's order), then the wave's tree, as BwdOp's loaders do.
+__device__ __forceinline__ void bwd_sigma(const SkGeom& g, const FusedSpmvArgs& f, int k, unsigned epoch) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int kWaves = kSweepThreads / 64;
+    const int NSG = g.T4 / kSub, T = g.T4 * 4;
+    for (int pr = k * kWaves + wave; pr < g.S * kLoaders; pr += f.K * kWaves) {
+        const int s = pr / kLoaders, me = pr % kLoaders;
+        const float* qs = f.q + (long)s * g.T4 * 256 + lane * 4;
+        double acc = 0;
+        constexpr int U = 4;  // supergroups with loads in flight together
+        for (int sg0 = me; sg0 < NSG; sg0 += U * kLoaders) {
+            float4 v[U][kSub];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int sg = sg0 + u * kLoaders;
+                const int t0 = T - 16 * ((sg < NSG ? sg : NSG - 1) + 1);
+#pragma unroll
+                for (int j = 0; j < kSub; j++) v[u][j] = *(const float4*)(qs + (long)(t0 + 4 * j) * 64);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (sg0 + u * kLoaders < NSG)
+#pragma unroll
+                    for (int j = 0; j < kSub; j++) acc += qq4(v[u][j]);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (lane == 0) {
+            f.sig_out[pr] = acc;
+            const unsigned long long b = (unsigned long long)__double_as_longlong(acc);
+            const unsigned long long tag = (unsigned long long)epoch << 32;
+            atomic_st64(f.sig_words + 2 * pr, (b & 0xffffffffull) | tag);
+            atomic_st64(f.sig_words + 2 * pr + 1, (b >> 32) | tag);
+        }
+    }
+}
+
+// SpMV workgroup m of M.
+__device__ __forceinline__ void bwd_spmv_worker(const SkGeom& g, const FusedSpmvArgs& f, int m, unsigned epoch,
+                                                int* err) {
+    __shared__ int sh_bad;
+    __shared__ double sh_sig;
+    const int lane = threadIdx.x & 63;
+    const int n_sig = g.S * kLoaders;
+    // sigma_{k+1}: wave 0 waits for this launch's tagged words and sums them in
+    // wave_sum_of's order
+    if (threadIdx.x < 64) {
+        double t = 0;
+        bool bad = false;
+        for (int spin = 0;; spin++) {
+            t = 0;
+            bool ok = true;
+            for (int i = lane; i < n_sig; i += 64) {
+                const unsigned long long lo = atomic_ld64(f.sig_words + 2 * i);
+                const unsigned long long hi = atomic_ld64(f.sig_words + 2 * i + 1);
+                ok = ok && (unsigned)(lo >> 32) == epoch && (unsigned)(hi >> 32) == epoch;
+                t += __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
+            }
+            if (__all(ok)) break;
+            if (spin >= kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                bad = true;
+                if (lane == 0) atomicExch(err, 1);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+        if (lane == 0) {
+            sh_bad = bad;
+            sh_sig = t;
+        }
+    }
+    __syncthreads();
+    if (sh_bad) return;
+    const double sig = sh_sig;
+    double beta_d = 0;
+    if (!f.first) beta_d = sig / block_sum_bcast(f.sig_prev, n_sig);
+    const float beta = (float)beta_d;
+    if (m == 0 && threadIdx.x == 0) f.sc->sigma = sig;
+
+    // Virtual workgroups in the order their strips finish: the top of the
+    // array (the last runs) first.  A run reads z of its strips and of one
+    // strip on either side; the strips above it finish earlier.
+    const long n = g.size(), strip_sz = (long)g.T4 * 256;
+    int lo_ready = g.S;  // strips lo_ready .. S-1 are known to be done
+    for (int it = m; it < f.nvb; it += f.M) {
+        const int vb = f.nvb - 1 - it;
+        const XcdLoop xl = spmv_range(n, vb, f.nvb, (int)threadIdx.x);
+        const long b0 = (n >> 8) * vb / f.nvb * 256, b1 = (n >> 8) * (vb + 1) / f.nvb * 256;
+        const int need = b0 < b1 ? max(0, (int)(b0 / strip_sz) - 1) : g.S;
+        if (need < lo_ready) {
+            if (threadIdx.x < 64) {
+                bool bad = false;
+                // strips finish top-down: one lane watches the lowest one
+                // needed (few polls through the fabric while the strips hand
+                // off), then the wave checks them all
+                for (int spin = 0;; spin++) {
+                    unsigned v = 0;
+                    if (lane == 0)
+                        v = __hip_atomic_load(f.strip_flags + need, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((unsigned)__shfl((int)v, 0, 64) == epoch) break;
+                    if (spin >= kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        bad = true;
+                        if (lane == 0) atomicExch(err, 1);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(16);
+                }
+                for (int s0 = need; s0 < lo_ready && !bad; s0 += 64) {
+                    const int s = s0 + lane;
+                    for (int spin = 0;; spin++) {
+                        const bool ok = s >= lo_ready ||
+                                        __hip_atomic_load(f.strip_flags + s, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT) == epoch;
+                        if (__all(ok)) break;
+                        if (spin >= kSpinLimit ||
+                            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                            bad = true;
+                            if (lane == 0) atomicExch(err, 1);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(8);
+                    }
+                }
+                if (lane == 0) sh_bad = bad;
+            }
+            __syncthreads();
+            if (sh_bad) return;
+            lo_ready = need;
+        }
+        spmv_core_wide<8>(xl, g, f.Adiag, f.flags, f.z, f.s_old, f.s_new, f.w, beta, f.ws_partials, vb);
+    }
+}
+
+__global__ __launch_bounds__(kSweepThreads) void bwd_spmv_kernel(BwdFusedOp op, SkGeom g, SkCtl c,
+                                                                 FusedSpmvArgs f) {
+    __shared__ __align__(16) SweepLds<BwdFusedOp, sweep_ring<BwdFusedOp>()> L;
+    const unsigned long long ticket = sweep_ticket(L, c.ticket);
+    const unsigned long long per = (unsigned long long)(f.K + g.S + f.M);
+    const int idx = (int)(ticket % per);
+    const unsigned epoch = (unsigned)(ticket / per) + 1u;
+    // sc->done is set only between launches, so every workgroup sees the same
+    __shared__ int sh_done;
+    if (threadIdx.x == 0) sh_done = f.sc->done;
+    __syncthreads();
+    if (sh_done) return;
+    if (idx < f.K) {
+        bwd_sigma(g, f, idx, epoch);
+    } else if (idx < f.K + g.S) {
+        op.epoch = epoch;
+        sweep_run<BwdFusedOp, true, true>(op, g, c, idx - f.K, epoch, L);
+    } else {
+        bwd_spmv_worker(g, f, idx - f.K - g.S, epoch, c.err);
     }
 }
 

@@ -141,6 +141,26 @@ __device__ __forceinline__ uint32_t nbr(uint32_t edge, uint32_t v) {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, ctl, 0xf, 0xf, true);
 }
 
+// Operators whose outputs other workgroups read within the same launch
+// declare kOutThrough: their stores then carry device scope (SC1), so each is
+// written through to memory, and a completion flag needs only those stores'
+// own completion (vmcnt) before it, not a write-back of the whole L2.
+template <class Op>
+constexpr auto out_through_(int) -> decltype(Op::kOutThrough) {
+    return Op::kOutThrough;
+}
+template <class Op>
+constexpr bool out_through_(long) {
+    return false;
+}
+template <class Op>
+constexpr bool out_through() {
+    return out_through_<Op>(0);
+}
+constexpr int kBufferCfg = 0x00020000;  // raw buffer resource word 3 of gfx9-family targets
+constexpr int kCpolSc1 = 16;            // cache policy operand: SC1 (device scope)
+constexpr unsigned kWaitVm0 = 0x0F70u;  // s_waitcnt vmcnt(0), gfx9 encoding
+
 template <class Op, int RING = kRing>
 struct SweepLds {
     static_assert(RING >= 2 && (RING & (RING - 1)) == 0, "ring slots are taken modulo RING");
@@ -179,6 +199,12 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
     const int T = g.T4 * 4;
     const int prod_lane = B ? 0 : 63;
     const long strip_base = (long)s * g.T4 * 256;
+    constexpr bool THROUGH = out_through<Op>();
+    __amdgpu_buffer_rsrc_t out_rs[NOUT];
+    if (THROUGH)
+#pragma unroll
+        for (int n = 0; n < NOUT; n++)
+            out_rs[n] = __builtin_amdgcn_make_buffer_rsrc(op.out[n], 0, (int)(g.size() * 4), kBufferCfg);
 
     struct Buf {
         float4 x[kSub][NF];
@@ -280,8 +306,15 @@ __device__ __forceinline__ void sweep_compute(Op& op, const SkGeom& g, SweepLds<
             op.template group<B, C>(cur.x[j], cur.b[j], st, cur.h[j], o, prod[j], t0 + 4 * j, s);
 #pragma unroll
             for (int n = 0; n < NOUT; n++) {
-                float* base = op.out[n] + strip_base + (long)t0 * 64;  // uniform
-                *(float4*)(base + j * 256 + lane * 4) = make_float4(o[n][0], o[n][1], o[n][2], o[n][3]);
+                if (THROUGH) {
+                    typedef float f4 __attribute__((ext_vector_type(4)));
+                    const f4 v = {o[n][0], o[n][1], o[n][2], o[n][3]};
+                    __builtin_amdgcn_raw_buffer_store_b128(v, out_rs[n], (j * 256 + lane * 4) * 4,
+                                                           (int)((strip_base + (long)t0 * 64) * 4), kCpolSc1);
+                } else {
+                    float* base = op.out[n] + strip_base + (long)t0 * 64;  // uniform
+                    *(float4*)(base + j * 256 + lane * 4) = make_float4(o[n][0], o[n][1], o[n][2], o[n][3]);
+                }
             }
             if (k == 0) {
                 // the previous supergroup's boundary values leave once the
@@ -705,17 +738,30 @@ struct FwdOp {
 };
 
 // Backward solve z = Lt^-T q (:410-419), z = P q + gx z_R + gy z_U; steps run
-// q = 3, 2, 1, 0 within a group; the loaders form P q.  The loaders also sum
-// q.q per strip: z.r = r.M^-1 r = |Lt^-1 r|^2 = q.q is sigma (:428, :457).
-// The first workgroup records the iteration count (:462).
-struct BwdOp {
+// q = 3, 2, 1, 0 within a group; the loaders form P q.  The first workgroup
+// records the iteration count (:462).
+//   FUSED false: the loaders also sum q.q per strip: z.r = r.M^-1 r =
+//     |Lt^-1 r|^2 = q.q is sigma (:428, :457), into qq_partials[S][kLoaders].
+//   FUSED true: the solve's strips inside bwd_spmv_kernel; sigma is summed by
+//     that launch's own workgroups (in the loaders' order), and each strip's
+//     compute wave publishes its completion for the SpMV workgroups of the
+//     same launch: z is stored with device scope (kOutThrough), so once those
+//     stores have completed (vmcnt 0) the flag may follow.
+__device__ __forceinline__ double qq4(const float4& q) {
+    return (double)fmaf(q.w, q.w, fmaf(q.z, q.z, fmaf(q.y, q.y, q.x * q.x)));
+}
+template <bool FUSED>
+struct BwdOpT {
     static constexpr int NW = 1, NIN = 4, NF = 3, NB = 0, NOUT = 1;
+    static constexpr bool kOutThrough = FUSED;
     const float* in[NIN];  // q, P, gx, gy
     const uint8_t* bytes;
     float* out[NOUT];      // z
     PcgScalars* sc;
-    double* qq_partials;   // [S][kLoaders]
+    double* qq_partials;   // [S][kLoaders] (FUSED false)
     int next_iter;         // iterations completed after this solve (0: initial solve)
+    unsigned* strip_flags; // [S] (FUSED true): the launch's epoch once the strip's z is stored
+    unsigned epoch;
     __device__ bool skip(bool first) const {
         if (sc->done) return true;
         if (first && (threadIdx.x & 63) == 0 && next_iter > 0) sc->iter = next_iter;
@@ -726,9 +772,10 @@ struct BwdOp {
         x[0] = make_float4(P.x * q.x, P.y * q.y, P.z * q.z, P.w * q.w);
         x[1] = in[2];
         x[2] = in[3];
-        acc += (double)fmaf(q.w, q.w, fmaf(q.z, q.z, fmaf(q.y, q.y, q.x * q.x)));
+        if (!FUSED) acc += qq4(q);
     }
     __device__ void loader_done(int s, int me, double acc) const {
+        if (FUSED) return;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
         if ((threadIdx.x & 63) == 0) qq_partials[s * kLoaders + me] = acc;
@@ -749,7 +796,16 @@ struct BwdOp {
         }
         st[0] = f2u(zp);
     }
-    __device__ void strip_done(int) const {}
+    __device__ void strip_done(int s) const {
+        if (!FUSED) return;
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);  // every z store of the strip is in memory
+        asm volatile("" ::: "memory");
+        if ((threadIdx.x & 63) == 0)
+            __hip_atomic_store(strip_flags + s, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 };
+using BwdOp = BwdOpT<false>;
+using BwdFusedOp = BwdOpT<true>;
 
 }  // namespace fs2d

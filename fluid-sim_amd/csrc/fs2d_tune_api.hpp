@@ -11,6 +11,9 @@
 //   sweep_prefetch  (ensembles) whether the batched solves' loaders issue
 //                   their first loads before the skip test: sooner data for
 //                   members still iterating, wasted loads for converged ones.
+//   fused_spmv      (single handles) the backward solve and the next SpMV in
+//                   one launch (bwd_spmv_kernel) or in two.  The SpMV's
+//                   virtual workgroups are the same either way, so the bits are.
 //
 // The tuner times each candidate on the handle's own state: `iters` PCG
 // iterations of the projection (tolerance 0, so every member runs them all),
@@ -42,21 +45,24 @@ std::string device_tag() {
     return buf;
 }
 
-// The cache file: one line per key, "key pcg_grid sweep_prefetch"; the last
-// line of a key wins.
+// The cache file: one line per key, "key pcg_grid sweep_prefetch fused_spmv"
+// (a line without the last value means fused_spmv 1); the last line of a key
+// wins.
 bool cache_lookup(const char* path, const std::string& key, fs2d_tuning* t) {
     if (!path) return false;
     FILE* f = fopen(path, "r");
     if (!f) return false;
-    char k[512];
-    int g, p;
+    char line[768], k[512];
     bool found = false;
-    while (fscanf(f, "%511s %d %d", k, &g, &p) == 3)
-        if (key == k) {
+    while (fgets(line, sizeof line, f)) {
+        int g = 0, p = 0, fu = 1;
+        if (sscanf(line, "%511s %d %d %d", k, &g, &p, &fu) >= 3 && key == k) {
             t->pcg_grid = g;
             t->sweep_prefetch = p;
+            t->fused_spmv = fu;
             found = true;
         }
+    }
     fclose(f);
     return found;
 }
@@ -64,7 +70,7 @@ void cache_store(const char* path, const std::string& key, const fs2d_tuning& t)
     if (!path) return;
     FILE* f = fopen(path, "a");
     if (!f) return;
-    fprintf(f, "%s %d %d\n", key.c_str(), t.pcg_grid, t.sweep_prefetch);
+    fprintf(f, "%s %d %d %d\n", key.c_str(), t.pcg_grid, t.sweep_prefetch, t.fused_spmv);
     fclose(f);
 }
 
@@ -105,13 +111,21 @@ fs2d_status fs2d_get_tuning(fs2d_sim* s, fs2d_tuning* t) {
     if (!s || !t) return FS2D_ERR_ARG;
     t->pcg_grid = s->pcg_grid;
     t->sweep_prefetch = 0;
+    t->fused_spmv = s->fused_spmv ? 1 : 0;
     return FS2D_OK;
 }
 
 fs2d_status fs2d_set_tuning(fs2d_sim* s, const fs2d_tuning* t) {
-    if (!s || !t || t->pcg_grid < 0 || (t->pcg_grid & 7)) return FS2D_ERR_ARG;
+    if (!s || !t || t->pcg_grid < 0 || (t->pcg_grid & 7) || (t->fused_spmv != 0 && t->fused_spmv != 1))
+        return FS2D_ERR_ARG;
     HIPCHK(s, hipStreamSynchronize(s->stream));
+    if ((t->fused_spmv != 0) != s->fused_spmv || t->pcg_grid != s->pcg_grid) {
+        // captured steps hold the old launches
+        for (auto& g : s->graphs) graph_free(g);
+        s->graphs.clear();
+    }
     s->pcg_grid = t->pcg_grid;
+    s->fused_spmv = t->fused_spmv != 0;
     return FS2D_OK;
 }
 
@@ -119,6 +133,7 @@ fs2d_status fs2d_batch_get_tuning(fs2d_batch* B, fs2d_tuning* t) {
     if (!B || !t) return FS2D_ERR_ARG;
     t->pcg_grid = B->pcg_grid;
     t->sweep_prefetch = B->sweep_prefetch ? 1 : 0;
+    t->fused_spmv = 0;
     return FS2D_OK;
 }
 
@@ -137,7 +152,7 @@ fs2d_status fs2d_autotune(fs2d_sim* s, int iters, const char* cache_path, fs2d_t
     snprintf(kb, sizeof kb, "%s|single|%dx%d|mode%d|flags%d", device_tag().c_str(), s->nx, s->ny, s->mode,
              s->flags);
     const std::string key = kb;
-    fs2d_tuning best{s->pcg_grid, 0};
+    fs2d_tuning best{s->pcg_grid, 0, s->fused_spmv ? 1 : 0};
     if (from_cache) *from_cache = 0;
     if (cache_lookup(cache_path, key, &best)) {
         if (from_cache) *from_cache = 1;
@@ -152,26 +167,30 @@ fs2d_status fs2d_autotune(fs2d_sim* s, int iters, const char* cache_path, fs2d_t
     const int resident = resident_groups(pcg_spmv_sk_kernel, kBlock, 0);
     double best_t = 1e30;
     fs2d_status st = FS2D_OK;
-    for (int g : grid_candidates(pcg_blocks(s), resident)) {
-        s->pcg_grid = g;
-        double t_min = 1e30;
-        for (int rep = 0; rep < 4 && st >= 0; rep++) {
-            HIPCHK(s, hipStreamSynchronize(s->stream));
-            const double t0 = now_s();
-            st = fs2d_run_stage(s, FS2D_STAGE_APPLY_PROJECTION);
-            const double t = now_s() - t0;
-            if (rep > 0) t_min = std::min(t_min, t);
+    for (int fu = 1; fu >= 0 && st >= 0; fu--)
+        for (int g : grid_candidates(pcg_blocks(s), resident)) {
+            s->pcg_grid = g;
+            s->fused_spmv = fu != 0;
+            double t_min = 1e30;
+            for (int rep = 0; rep < 4 && st >= 0; rep++) {
+                HIPCHK(s, hipStreamSynchronize(s->stream));
+                const double t0 = now_s();
+                st = fs2d_run_stage(s, FS2D_STAGE_APPLY_PROJECTION);
+                const double t = now_s() - t0;
+                if (rep > 0) t_min = std::min(t_min, t);
+            }
+            if (st < 0) break;
+            if (t_min < best_t) {
+                best_t = t_min;
+                best.pcg_grid = g;
+                best.fused_spmv = fu;
+            }
         }
-        if (st < 0) break;
-        if (t_min < best_t) {
-            best_t = t_min;
-            best.pcg_grid = g;
-        }
-    }
     s->tol = tol;
     s->max_iters = max_iters;
     s->last_iters_hint = hint;
     s->pcg_grid = best.pcg_grid;
+    s->fused_spmv = best.fused_spmv != 0;
     if (st < 0) return st;
     cache_store(cache_path, key, best);
     if (out) *out = best;
@@ -192,7 +211,7 @@ fs2d_status fs2d_batch_autotune(fs2d_batch* B, int iters, const char* cache_path
     snprintf(kb, sizeof kb, "%s|batch|n%d|sizes%016llx|mode%d|flags%d", device_tag().c_str(), B->n, h, B->mode,
              B->flags);
     const std::string key = kb;
-    fs2d_tuning best{B->pcg_grid, B->sweep_prefetch ? 1 : 0};
+    fs2d_tuning best{B->pcg_grid, B->sweep_prefetch ? 1 : 0, 0};
     if (from_cache) *from_cache = 0;
     if (cache_lookup(cache_path, key, &best)) {
         if (from_cache) *from_cache = 1;

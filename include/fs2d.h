@@ -243,6 +243,10 @@ const char* fs2d_status_string(int status);
  *     size, so every value gives the same bits.
  *   sweep_prefetch: (ensembles) the batched triangular solves' loaders issue
  *     their first loads before testing whether their member has converged.
+ *   fused_spmv: (single handles) 1, the default: the backward solve's launch
+ *     also runs the next iteration's SpMV, whose workgroups take each region
+ *     as soon as the solve's strips around it have finished; 0: separate
+ *     launches.  Ensembles report 0 and ignore it.
  * fs2d_autotune / fs2d_batch_autotune time the candidates on the handle's
  * current state (call them after a first step): `iters` PCG iterations of the
  * projection each (tolerance 0), best of three after a warm-up; the fastest is
@@ -254,6 +258,7 @@ const char* fs2d_status_string(int status);
 typedef struct {
     int pcg_grid;
     int sweep_prefetch;
+    int fused_spmv;
 } fs2d_tuning;
 fs2d_status fs2d_get_tuning(fs2d_sim* sim, fs2d_tuning* out);
 fs2d_status fs2d_set_tuning(fs2d_sim* sim, const fs2d_tuning* t);

@@ -4,8 +4,11 @@ fs2d_batch_set_tuning) and the autotuner that picks them.
 The PCG SpMV and axpy loop over a fixed list of virtual workgroups, each
 writing its own partial sums, so the number of physical workgroups cannot
 change a bit of the result; nor can whether the batched solves' loaders
-prefetch.  Steps under every tuning must match bit for bit, and the autotuner
-must record its choice and read it back.
+prefetch, nor whether a single handle runs the SpMV inside the backward
+solve's launch (fused_spmv: the same virtual workgroups, started as the
+solve's strips finish, with sigma summed in the solve's loaders' order).
+Steps under every tuning must match bit for bit, and the autotuner must
+record its choice and read it back.
 """
 import numpy as np
 import pytest
@@ -32,19 +35,74 @@ def smoke(n, seed):
 def test_single_tunings_bit_identical():
     n = 300  # pcg_blocks 256: tunings below it loop several virtual workgroups per workgroup
     ref = smoke(n, 7)
+    assert ref.tuning()["fused_spmv"] == 1  # the default
     for _ in range(3):
         ref.step()
     want = [ref.get(f) for f in FIELDS]
     it = ref.stats()["pcg_iters"]
-    for g in (8, 64, 248):
+    for g, fu in ((8, 1), (64, 0), (248, 1), (0, 0)):
         s = smoke(n, 7)
-        s.set_tuning(pcg_grid=g)
-        assert s.tuning()["pcg_grid"] == g
+        s.set_tuning(pcg_grid=g, fused_spmv=fu)
+        assert s.tuning()["pcg_grid"] == g and s.tuning()["fused_spmv"] == fu
         for _ in range(3):
             s.step()
         for f, w in zip(FIELDS, want):
-            np.testing.assert_array_equal(s.get(f), w, err_msg="pcg_grid %d field %d" % (g, f))
+            np.testing.assert_array_equal(s.get(f), w, err_msg="pcg_grid %d fused %d field %d" % (g, fu, f))
         assert s.stats()["pcg_iters"] == it
+        s.close()
+
+
+def box(nx, ny, seed):
+    """All-fluid nx x ny box with a smoothed random velocity (static cells)."""
+    from scipy.ndimage import gaussian_filter
+    c = scenes.walls(nx, ny)
+    c[1:-1, 1:-1] = scenes.FLUID
+    kw = dict(k=0, dt=0.01, dx=1.0 / max(nx, ny), mode=fs2d.SEMILAGRANGIAN, gravity=(0.0, 0.0), tau=0.97,
+              tol=1e-6, max_iters=150, static_cells=True, density=True)
+    s = fs2d.Sim(c, **kw)
+    rng = np.random.default_rng(seed)
+    s.set(fs2d.F_U, gaussian_filter(rng.standard_normal((ny, nx + 1)), 4.0, mode="wrap").astype(np.float32))
+    s.set(fs2d.F_V, gaussian_filter(rng.standard_normal((ny + 1, nx)), 4.0, mode="wrap").astype(np.float32))
+    s.set(fs2d.F_DENSITY, np.ones((ny, nx), np.float32))
+    return s
+
+
+@pytest.mark.parametrize("nx,ny", [(64, 64), (1000, 130), (300, 300), (200, 700), (64, 16000)])
+def test_fused_spmv_bit_identical_across_strip_counts(nx, ny):
+    """1, 3, 5, 11 and 250 strips; the last launches 125 + 250 + 8 workgroups
+    of one CU each, more than the device holds at once, so workgroups wait on
+    strips that started before them across rounds of residency."""
+    a, b = box(nx, ny, 5), box(nx, ny, 5)
+    b.set_tuning(pcg_grid=0, fused_spmv=0)
+    for step in range(2):
+        a.step()
+        b.step()
+        for f in FIELDS[:4]:
+            np.testing.assert_array_equal(a.get(f), b.get(f), err_msg="%dx%d step %d field %d" % (nx, ny, step, f))
+        assert a.stats()["pcg_iters"] == b.stats()["pcg_iters"]
+    print("%dx%d: %d strips, iterations %d" % (nx, ny, (ny + 63) // 64, a.stats()["pcg_iters"]))
+    a.close()
+    b.close()
+
+
+def test_fused_spmv_pic_and_captured_steps():
+    """A PIC/FLIP dam break (EMPTY cells, a changing matrix) stepped with the
+    fused launch, uncaptured and from captured graphs, against separate
+    launches."""
+    n = 160
+    kw = dict(k=2, dt=0.002, dx=1.0 / n, mode=fs2d.PICFLIP, alpha=0.05, tau=0.97, tol=1e-6, max_iters=500)
+    c = scenes.dam_break(n)
+    sims = [fs2d.Sim(c, **kw) for _ in range(3)]
+    sims[1].set_graph(True)
+    sims[2].set_tuning(pcg_grid=0, fused_spmv=0)
+    for step in range(4):
+        for s in sims:
+            s.step()
+        for f in FIELDS[:3] + (fs2d.F_PARTICLE_POS, fs2d.F_PARTICLE_VEL, fs2d.F_CELL):
+            want = sims[2].get(f)
+            for s in sims[:2]:
+                np.testing.assert_array_equal(s.get(f), want, err_msg="step %d field %d" % (step, f))
+    for s in sims:
         s.close()
 
 
