@@ -18,7 +18,7 @@ every step runs into the iteration cap.
 `roofline` reports the dominant kernel class: algorithmic bytes per launch
 (BYTES_PER_CELL x cells) over its average launch time, measured with HIP
 events on the library's stream inside the timed region.  With the tuning's
-fused_spmv (the default) the backward-solve class is one launch that also
+fused_pcg (the default) the backward-solve class is one launch that also
 runs the next iteration's SpMV, and it is reported as
 `pcg_backward_solve+spmv` with both kernels' arrays.  Launches that start
 after convergence exit at once; the average is taken over the launches that
@@ -477,7 +477,7 @@ def main():
         s_.step()
     tune = sim.autotune(iters=24, cache=args.tune_cache)
     if gsim:
-        gsim.set_tuning(pcg_grid=tune["pcg_grid"], fused_spmv=tune["fused_spmv"])
+        gsim.set_tuning(pcg_grid=tune["pcg_grid"], fused_pcg=tune["fused_pcg"])
     for _ in range(args.warmup):
         for s_ in twins:
             s_.step()
@@ -530,7 +530,7 @@ def main():
     if dom in EFFECTIVE:
         launches = int(sum(iters)) + EFFECTIVE[dom] * args.steps
     avg_s = tot_ms / 1e3 / max(launches, 1)
-    label = dom + "+spmv" if dom == "pcg_backward_solve" and tune.get("fused_spmv") else dom
+    label = dom + "+spmv" if dom == "pcg_backward_solve" and tune.get("fused_pcg") else dom
     bytes_per_launch = BYTES_PER_CELL[label] * n * n
     achieved = bytes_per_launch / avg_s / 1e9
     out = {

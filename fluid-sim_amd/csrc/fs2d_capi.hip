@@ -101,11 +101,15 @@ struct fs2d_sim {
     double *qq = nullptr, *ws_part = nullptr, *rmax_part = nullptr;
     unsigned long long* sk_ticket = nullptr;
     unsigned long long* sk_handoff = nullptr;
-    // the backward solve with the next SpMV in one launch (bwd_spmv_kernel):
-    // its own ticket, hand-off rows, per-strip completion flags and tagged
-    // sigma words
-    bool fused_spmv = true;
+    // a PCG iteration as two launches: the p/r update with the forward solve
+    // (fwd_axpy_kernel: its own ticket, hand-off rows and per-run flags) and
+    // the backward solve with the next SpMV (bwd_spmv_kernel: ticket,
+    // hand-off rows, per-strip completion flags, tagged sigma words)
+    bool fused_pcg = true;
     int ncu = 0;  // compute units of the device
+    unsigned long long* fa_ticket = nullptr;
+    unsigned long long* fa_handoff = nullptr;
+    unsigned* fa_flags = nullptr;
     unsigned long long* fb_ticket = nullptr;
     unsigned long long* fb_handoff = nullptr;
     unsigned* fb_flags = nullptr;
@@ -559,22 +563,41 @@ int pcg_launch(const fs2d_sim* s) {
 // waits for a CU of its own XCD, and on XCDs whose CUs all hold strips or
 // other SpMV workgroups its statically assigned runs start only at the end.
 void launch_bwd_spmv(fs2d_sim* s, int next_iter, double* sig_out, const double* sig_prev, int first,
-                     const float* s_old, float* s_new) {
+                     const float* s_old, float* s_new, int check) {
     const SkGeom& g = s->geom;
     const int waves = kSweepThreads / 64;
     const int K = (g.S * kLoaders + waves - 1) / waves;
     const int nbp = pcg_blocks(s);
     const int M = std::max(8, std::min(nbp, (s->ncu > 0 ? s->ncu : 256) - g.S - K));
-    BwdFusedOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, nullptr, next_iter, s->fb_flags, 0u};
-    FusedSpmvArgs f{s->Adiag, s->cpl, s->z,       s->q,          s_old, s_new, s->w, s->sc, sig_out, s->sig_words,
-                    sig_prev, first,  s->ws_part, nbp, s->fb_flags, K,     M};
+    BwdFusedOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, nullptr, next_iter, s->fb_flags, 0u,
+                  check, s->rmax_part, nbp};
+    FusedSpmvArgs f{s->Adiag, s->cpl,     s->z, s->q,         s_old, s_new, s->w, s->sc, sig_out, s->sig_words,
+                    sig_prev, first,      s->ws_part, nbp, s->fb_flags, K, M, check, s->rmax_part};
     hipLaunchKernelGGL(bwd_spmv_kernel, dim3(K + g.S + M), dim3(kSweepThreads), 0, s->stream, op, g,
                        SkCtl{s->fb_ticket, s->fb_handoff, s->werr}, f);
 }
 
+// The p/r update of iteration h (s' in s_new, alpha = sigma_h / w.s' with
+// sigma_h in sig_cur) and its forward solve in one launch (fwd_axpy_kernel).
+// The update's workgroups sit beside the S strips: enough for two strips'
+// runs at a time, which keeps it far ahead of the solve's wavefront; more
+// (every free CU) measured slower at 2048^2 (158.6 against 154.1 us), the
+// update's burst of traffic slowing the first strips.  The stopping test of
+// iteration h runs in the next backward launch.
+void launch_fwd_axpy(fs2d_sim* s, const float* s_new, const double* sig_cur) {
+    const SkGeom& g = s->geom;
+    const int nbp = pcg_blocks(s);
+    const int per_strip = (nbp + g.S - 1) / g.S;
+    const int M = std::max(8, std::min({nbp, 2 * per_strip, (s->ncu > 0 ? s->ncu : 256) - g.S}));
+    FwdFusedOp op{{s->r, s->P, s->fx, s->fy}, nullptr, {s->q}, s->sc, s->rmax_part, nbp, 0, s->fa_flags, g, 0u};
+    FusedAxpyArgs f{s->pk, s->r, s_new, s->w, s->sc, sig_cur, s->ws_part, s->rmax_part, s->fa_flags, nbp, M};
+    hipLaunchKernelGGL(fwd_axpy_kernel, dim3(M + g.S), dim3(kSweepThreads), 0, s->stream, op, g,
+                       SkCtl{s->fa_ticket, s->fa_handoff, s->werr}, f);
+}
+
 // MIC(0) factor and the first preconditioner application of the PCG
 // (src/FluidSim2D.cpp:364-430) on the assembled matrix and the rhs in r (with
-// p = s = 0 and the |rhs|_inf partials in rmax_part).  With fused_spmv the
+// p = s = 0 and the |rhs|_inf partials in rmax_part).  With fused_pcg the
 // backward solve's launch also runs iteration 0's SpMV.
 void pcg_begin(fs2d_sim* s) {
     const int nb = sk_blocks(s);
@@ -593,8 +616,8 @@ void pcg_begin(fs2d_sim* s) {
     }
     {
         Timed t(s, K_PCG_BWD);
-        if (s->fused_spmv) {
-            launch_bwd_spmv(s, 0, s->qq, nullptr, 1, s->s[0], s->s[1]);
+        if (s->fused_pcg) {
+            launch_bwd_spmv(s, 0, s->qq, nullptr, 1, s->s[0], s->s[1], 0);
         } else {
             BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, s->qq, 0};
             launch_sweep<BwdOp, true>(s, op);
@@ -605,9 +628,10 @@ void pcg_begin(fs2d_sim* s) {
 // Iterations h0 .. h1-1 of the PCG loop (:431-463): SpMV with the s update,
 // the p/r update, forward and backward solves.  Every kernel returns at once
 // once the stopping test has passed, so a fixed number of iterations can be
-// queued (or captured) without host round trips.  With fused_spmv iteration
-// h's SpMV ran in the previous backward solve's launch, and this iteration's
-// backward launch runs iteration h + 1's.
+// queued (or captured) without host round trips.  With fused_pcg iteration
+// h's SpMV ran in the previous backward solve's launch; this iteration is the
+// p/r update with the forward solve, then the backward solve (after the
+// stopping test) with iteration h + 1's SpMV.
 void pcg_iterations(fs2d_sim* s, int h0, int h1) {
     const SkGeom& g = s->geom;
     double* qq0 = s->qq;
@@ -618,7 +642,16 @@ void pcg_iterations(fs2d_sim* s, int h0, int h1) {
         float* s_new = s->s[(h + 1) & 1];
         double* sig_cur = (h & 1) ? qq1 : qq0;   // sigma_h
         double* sig_prev = (h & 1) ? qq0 : qq1;  // sigma_{h-1}; receives sigma_{h+1}
-        if (!s->fused_spmv) {
+        if (s->fused_pcg) {
+            {
+                Timed t(s, K_PCG_FWD);
+                launch_fwd_axpy(s, s_new, sig_cur);
+            }
+            Timed t(s, K_PCG_BWD);
+            launch_bwd_spmv(s, h + 1, sig_prev, sig_cur, 0, s_new, s_old, 1);
+            continue;
+        }
+        {
             Timed t(s, K_PCG_SPMV);
             hipLaunchKernelGGL(pcg_spmv_sk_kernel, dim3(grid), dim3(kBlock), 0, s->stream, g, s->Adiag, s->cpl, s->z,
                                s_old, s_new, s->w, s->sc, sig_cur, sig_prev, h == 0 ? 1 : 0, s->ws_part,
@@ -636,12 +669,8 @@ void pcg_iterations(fs2d_sim* s, int h0, int h1) {
         }
         {
             Timed t(s, K_PCG_BWD);
-            if (s->fused_spmv) {
-                launch_bwd_spmv(s, h + 1, sig_prev, sig_cur, 0, s_new, s_old);
-            } else {
-                BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, sig_prev, h + 1};
-                launch_sweep<BwdOp, true>(s, op);
-            }
+            BwdOp op{{s->q, s->P, s->gx_, s->gy_}, nullptr, {s->z}, s->sc, sig_prev, h + 1};
+            launch_sweep<BwdOp, true>(s, op);
         }
     }
 }
@@ -1045,6 +1074,8 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
     A(sk_handoff, (size_t)nstrips * sk_handoff_stride(s->nx, 3));
     A(fb_ticket, 1); A(fb_flags, nstrips); A(sig_words, 2 * (size_t)nstrips * kLoaders);
     A(fb_handoff, (size_t)nstrips * sk_handoff_stride(s->nx, BwdFusedOp::NW));
+    A(fa_ticket, 1); A(fa_flags, kMaxBlocks);
+    A(fa_handoff, (size_t)nstrips * sk_handoff_stride(s->nx, FwdFusedOp::NW));
 #undef A
     if (hipDeviceGetAttribute(&s->ncu, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess) s->ncu = 0;
     e = hipHostMalloc((void**)&s->h_sc, sizeof(PcgScalars), hipHostMallocDefault);
@@ -1654,7 +1685,7 @@ fs2d_status fs2d_project_noalloc(const fs2d_projection_args* a, int* iters, doub
     t.sk_ticket = (unsigned long long*)q;        q += kNaWord;
     t.werr = (int*)q;                            q += kNaWord;
     t.sk_handoff = (unsigned long long*)q;
-    t.fused_spmv = false;  // the separate SpMV launch: no completion flags or tagged sigma in the scratch
+    t.fused_pcg = false;  // the separate SpMV launch: no completion flags or tagged sigma in the scratch
     PcgScalars h_sc{};
     t.h_sc = &h_sc;  // pageable: read back with a synchronising copy
 

@@ -66,7 +66,7 @@ __device__ __forceinline__ double block_max(double v) {
     __shared__ double sh[kBlock / 64];
     v = wave_max(v);
     __syncthreads();
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    if ((threadIdx.x & 63) == 0 && threadIdx.x < kBlock) sh[threadIdx.x >> 6] = v;
     __syncthreads();
     double t = 0;
     if (threadIdx.x == 0)
@@ -642,6 +642,8 @@ struct FusedSpmvArgs {
     int nvb;                        // SpMV virtual workgroups, a multiple of 8
     const unsigned* strip_flags;
     int K, M;
+    int check;                      // the stopping test of the iteration (its p/r update's partials)
+    const double* rmax_partials;     // [nvb]
 };
 
 // sigma workgroup k of K: one (strip, loader) pair per wave; the lane sums its
@@ -788,9 +790,22 @@ __global__ __launch_bounds__(kSweepThreads) void bwd_spmv_kernel(BwdFusedOp op, 
     const unsigned long long per = (unsigned long long)(f.K + g.S + f.M);
     const int idx = (int)(ticket % per);
     const unsigned epoch = (unsigned)(ticket / per) + 1u;
-    // sc->done is set only between launches, so every workgroup sees the same
+    // sc->done is set before the launch or by this launch's stopping test
+    // (:453), which every workgroup evaluates for itself from the same
+    // partials: all agree whether the launch has work.  The strips evaluate
+    // it in BwdFusedOp::skip (the first one records it), beside their
+    // loaders' first loads.
+    const bool strip = idx >= f.K && idx < f.K + g.S;
     __shared__ int sh_done;
-    if (threadIdx.x == 0) sh_done = f.sc->done;
+    if (threadIdx.x < 64) {
+        int done = f.sc->done;
+        if (!done && f.check && !strip) {
+            const double rn = f.nvb <= 64 * 32 ? wave_max_of_wide<32>(f.rmax_partials, f.nvb)
+                                               : wave_max_of(f.rmax_partials, f.nvb);
+            done = rn <= f.sc->tol * f.sc->rhs_norm;
+        }
+        if (threadIdx.x == 0) sh_done = done;
+    }
     __syncthreads();
     if (sh_done) return;
     if (idx < f.K) {
@@ -800,6 +815,129 @@ __global__ __launch_bounds__(kSweepThreads) void bwd_spmv_kernel(BwdFusedOp op, 
         sweep_run<BwdFusedOp, true, true>(op, g, c, idx - f.K, epoch, L);
     } else {
         bwd_spmv_worker(g, f, idx - f.K - g.S, epoch, c.err);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The p/r update and the forward solve in one launch (src/FluidSim2D.cpp:450-452
+// then :400-409).  Forward strip s needs only r of strip s, so the update runs
+// over strip-ordered runs of whole 256-entry groups (pcg_run) and flags each
+// one; the solve's loaders start on a strip as soon as its runs are flagged.
+// Workgroups take tickets of the launch (tickets / launch = M + S):
+//   0 .. M-1      the update: alpha as pcg_axpy_sk_kernel forms it, then runs
+//                 m, m + M, ...; r is stored with device scope, so after the
+//                 stores' completion (vmcnt 0, every wave, then a barrier) the
+//                 run's flag may follow; the |r|_inf partial of each run
+//   M .. M+S-1    the forward solve's strips (FwdFusedOp)
+// The stopping test, which needs every run's partial, moves to the backward
+// solve's launch.  Each entry's update and the maximum over all partials are
+// those of the separate launches, so every result is bit-identical.
+// ---------------------------------------------------------------------------
+struct FusedAxpyArgs {
+    float* p;
+    float* r;
+    const float* s;                // s' of the iteration
+    const float* w;
+    const PcgScalars* sc;
+    const double* sig_cur;         // sigma partials [S][kLoaders]
+    const double* ws_partials;     // [nvb]
+    double* rmax_partials;         // [nvb], one per run
+    unsigned* run_flags;           // [nvb]
+    int nvb, M;
+};
+// Float4 range of p/r update run vb of nvb: whole 256-entry groups (64
+// float4s each), the n / 256 groups split evenly.
+__device__ __forceinline__ void pcg_run(long n, int vb, int nvb, long& q0, long& q1) {
+    const long ng = n >> 8;
+    q0 = ng * vb / nvb * 64;
+    q1 = ng * (vb + 1) / nvb * 64;
+}
+__device__ __forceinline__ void fwd_axpy_worker(const SkGeom& g, const FusedAxpyArgs& f, int m, unsigned epoch) {
+    float4* p4 = (float4*)f.p;
+    const float4* s4 = (const float4*)f.s;
+    const float4* w4 = (const float4*)f.w;
+    const float4* r4 = (const float4*)f.r;
+    // alpha = sigma / w.s as pcg_axpy_sk_kernel's block_sum_bcast forms it,
+    // with each lane's loads in flight together (both sums fit one round)
+    __shared__ float sh_a;
+    if (threadIdx.x < 64) {
+        const double sig = f.nvb <= 64 * 32 && g.S * kLoaders <= 64 * 32
+                               ? wave_sum_of_wide<32>(f.sig_cur, g.S * kLoaders)
+                               : wave_sum_of(f.sig_cur, g.S * kLoaders);
+        const double ws = f.nvb <= 64 * 32 ? wave_sum_of_wide<32>(f.ws_partials, f.nvb)
+                                           : wave_sum_of(f.ws_partials, f.nvb);
+        if (threadIdx.x == 0) sh_a = (float)(sig / ws);
+    }
+    __syncthreads();
+    const float a = sh_a;
+    const __amdgpu_buffer_rsrc_t r_rs = __builtin_amdgcn_make_buffer_rsrc(f.r, 0, (int)(g.size() * 4), kBufferCfg);
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    constexpr int U = 4;
+    for (int vb = m; vb < f.nvb; vb += f.M) {
+        long q0, q1;
+        pcg_run(g.size(), vb, f.nvb, q0, q1);
+        float mx = 0.0f;
+        if (threadIdx.x < kBlock) {
+            for (long e0 = q0 + threadIdx.x; e0 < q1; e0 += U * kBlock) {
+                float4 sv[U], wv[U], pv[U], rv[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const long e = e0 + u * kBlock;
+                    if (e < q1) {
+                        sv[u] = s4[e];
+                        wv[u] = w4[e];
+                        pv[u] = p4[e];
+                        rv[u] = r4[e];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const long e = e0 + u * kBlock;
+                    if (e < q1) {
+                        float4 pn = pv[u], rn = rv[u];
+                        pn.x = fmaf(a, sv[u].x, pn.x);
+                        pn.y = fmaf(a, sv[u].y, pn.y);
+                        pn.z = fmaf(a, sv[u].z, pn.z);
+                        pn.w = fmaf(a, sv[u].w, pn.w);
+                        rn.x = fmaf(-a, wv[u].x, rn.x);
+                        rn.y = fmaf(-a, wv[u].y, rn.y);
+                        rn.z = fmaf(-a, wv[u].z, rn.z);
+                        rn.w = fmaf(-a, wv[u].w, rn.w);
+                        p4[e] = pn;
+                        const f4 rv4 = {rn.x, rn.y, rn.z, rn.w};
+                        __builtin_amdgcn_raw_buffer_store_b128(rv4, r_rs, (int)(e * 16), 0, kCpolSc1);
+                        mx = fmaxf(mx, fmaxf(fmaxf(fabsf(rn.x), fabsf(rn.y)), fmaxf(fabsf(rn.z), fabsf(rn.w))));
+                    }
+                }
+            }
+        }
+        const double bm = block_max((double)mx);
+        if (threadIdx.x == 0) f.rmax_partials[vb] = bm;
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);  // this wave's r stores are in memory
+        asm volatile("" ::: "memory");
+        __syncthreads();                        // ... and every wave's
+        if (threadIdx.x == 0) __hip_atomic_store(f.run_flags + vb, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ __launch_bounds__(kSweepThreads) void fwd_axpy_kernel(FwdFusedOp op, SkGeom g, SkCtl c,
+                                                                 FusedAxpyArgs f) {
+    __shared__ __align__(16) SweepLds<FwdFusedOp, sweep_ring<FwdFusedOp>()> L;
+    const unsigned long long ticket = sweep_ticket(L, c.ticket);
+    const unsigned long long per = (unsigned long long)(f.M + g.S);
+    const int idx = (int)(ticket % per);
+    const unsigned epoch = (unsigned)(ticket / per) + 1u;
+    // sc->done is set only between launches of this kind
+    __shared__ int sh_done;
+    if (threadIdx.x == 0) sh_done = f.sc->done;
+    __syncthreads();
+    if (sh_done) return;
+    if (idx < f.M) {
+        fwd_axpy_worker(g, f, idx, epoch);
+    } else {
+        op.epoch = epoch;
+        sweep_run<FwdFusedOp, false, true>(op, g, c, idx - f.M, epoch, L);
     }
 }
 

@@ -157,6 +157,21 @@ template <class Op>
 constexpr bool out_through() {
     return out_through_<Op>(0);
 }
+// Operators whose inputs other workgroups of the same launch produce declare
+// kWaitInputs and wait_inputs(s, err): each loader wave calls it before its
+// first load of strip s.
+template <class Op>
+constexpr auto wait_inputs_(int) -> decltype(Op::kWaitInputs) {
+    return Op::kWaitInputs;
+}
+template <class Op>
+constexpr bool wait_inputs_(long) {
+    return false;
+}
+template <class Op>
+constexpr bool waits_inputs() {
+    return wait_inputs_<Op>(0);
+}
 constexpr int kBufferCfg = 0x00020000;  // raw buffer resource word 3 of gfx9-family targets
 constexpr int kCpolSc1 = 16;            // cache policy operand: SC1 (device scope)
 constexpr unsigned kWaitVm0 = 0x0F70u;  // s_waitcnt vmcnt(0), gfx9 encoding
@@ -424,7 +439,10 @@ __device__ __forceinline__ void sweep_run(Op& op, const SkGeom& g, const SkCtl& 
     };
     // With PREFETCH a loader's first batch goes out before the skip test,
     // whose reads it overlaps (loads only: a skipped pass discards them).
-    if (PREFETCH && wave >= 1 && wave <= kLoaders) load_batch((wave - 1) * kBatch);
+    if (wave >= 1 && wave <= kLoaders) {
+        if constexpr (waits_inputs<Op>()) op.wait_inputs(s, c.err);
+        if (PREFETCH) load_batch((wave - 1) * kBatch);
+    }
     if (wave == 0) {
         int sk = __hip_atomic_load(c.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
         if (!sk) sk = op.skip(order == 0);
@@ -603,6 +621,39 @@ __device__ __forceinline__ double wave_max_of(const double* a, int n) {
     }
     return m;
 }
+// wave_max_of / wave_sum_of with every load of a lane in flight at once
+// (n <= 64 W): one memory round trip instead of n / 512; the sum's additions
+// in the same order.
+template <int W>
+__device__ __forceinline__ double wave_max_of_wide(const double* a, int n) {
+    const int lane = threadIdx.x & 63;
+    double v[W];
+#pragma unroll
+    for (int u = 0; u < W; u++) v[u] = lane + 64 * u < n ? a[lane + 64 * u] : 0.0;
+    double m = 0;
+#pragma unroll
+    for (int u = 0; u < W; u++) m = v[u] > m ? v[u] : m;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        double x = __shfl_xor(m, o, 64);
+        m = x > m ? x : m;
+    }
+    return m;
+}
+template <int W>
+__device__ __forceinline__ double wave_sum_of_wide(const double* a, int n) {
+    const int lane = threadIdx.x & 63;
+    double v[W];
+#pragma unroll
+    for (int u = 0; u < W; u++) v[u] = lane + 64 * u < n ? a[lane + 64 * u] : 0.0;
+    double t = 0;
+#pragma unroll
+    for (int u = 0; u < W; u++)
+        if (lane + 64 * u < n) t += v[u];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    return t;
+}
 __device__ __forceinline__ double wave_sum_of(const double* a, int n) {
     double t = 0;
     for (int k0 = threadIdx.x & 63; k0 < n; k0 += 64 * 8) {
@@ -690,20 +741,32 @@ struct FactorOp {
 };
 
 // Forward solve q = Lt^-1 r (:400-409), q = P r + fx q_L + fy q_D; the loaders
-// form P r.  In iteration mode the first workgroup finishes |r|_inf and
-// applies the stopping test (:453) before any preconditioning, as the
-// reference does.
-struct FwdOp {
+// form P r.
+//   FUSED false: in iteration mode the first workgroup finishes |r|_inf and
+//     applies the stopping test (:453) before any preconditioning, as the
+//     reference does.
+//   FUSED true: the solve's strips inside fwd_axpy_kernel, after the same
+//     launch's p/r update.  Strip s's loaders first wait for the update's runs
+//     over strip s (run_flags[vb] = the launch's epoch, published after the
+//     runs' device-scope r stores); the stopping test moves to the backward
+//     solve's launch, so the last solve of a projection runs once more than
+//     needed, writing only q.
+template <bool FUSED>
+struct FwdOpT {
     static constexpr int NW = 1, NIN = 4, NF = 3, NB = 0, NOUT = 1;
+    static constexpr bool kWaitInputs = FUSED;
     const float* in[NIN];  // r, P, fx, fy
     const uint8_t* bytes;
     float* out[NOUT];      // q
     PcgScalars* sc;
     const double* rmax_partials;
     int n_partials, check;
+    const unsigned* run_flags;  // FUSED: [n_partials] p/r update runs
+    SkGeom geom;                // FUSED: the runs' strips
+    unsigned epoch;
     __device__ bool skip(bool first) {
         if (sc->done) return true;
-        if (!check) return false;
+        if (FUSED || !check) return false;
         const double rn = wave_max_of(rmax_partials, n_partials);
         const bool conv = rn <= sc->tol * sc->rhs_norm;
         if (first && (threadIdx.x & 63) == 0) {
@@ -711,6 +774,30 @@ struct FwdOp {
             if (conv) sc->done = 1;
         }
         return conv;
+    }
+    // Runs of the p/r update are whole 256-entry groups, the n / 256 groups
+    // split evenly over n_partials runs (pcg_run); strip s is groups
+    // s T4 .. (s+1) T4 - 1.
+    __device__ void wait_inputs(int s, int* err) const {
+        if (!FUSED || sc->done) return;
+        const long ng = geom.size() >> 8, nr = n_partials;
+        const long g0 = (long)s * geom.T4, g1 = g0 + geom.T4;  // the strip's groups
+        // runs vb with ng vb / nr < g1 and ng (vb + 1) / nr > g0
+        const int lo = (int)(g0 * nr / ng), hi = (int)min(nr - 1, (g1 * nr + ng - 1) / ng);
+        const int lane = threadIdx.x & 63;
+        for (int v0 = lo; v0 <= hi; v0 += 64) {
+            const int vb = v0 + lane;
+            for (int spin = 0;; spin++) {
+                const bool ok = vb > hi || __hip_atomic_load(run_flags + vb, __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT) == epoch;
+                if (__all(ok)) break;
+                if (spin >= kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    if (lane == 0) atomicExch(err, 1);
+                    return;
+                }
+                __builtin_amdgcn_s_sleep(4);
+            }
+        }
     }
     __device__ void xform(const float4 (&in)[NIN], float4 (&x)[NF], double&, int, int) const {
         x[0] = make_float4(in[1].x * in[0].x, in[1].y * in[0].y, in[1].z * in[0].z, in[1].w * in[0].w);
@@ -736,6 +823,8 @@ struct FwdOp {
     }
     __device__ void strip_done(int) const {}
 };
+using FwdOp = FwdOpT<false>;
+using FwdFusedOp = FwdOpT<true>;
 
 // Backward solve z = Lt^-T q (:410-419), z = P q + gx z_R + gy z_U; steps run
 // q = 3, 2, 1, 0 within a group; the loaders form P q.  The first workgroup
@@ -762,8 +851,21 @@ struct BwdOpT {
     int next_iter;         // iterations completed after this solve (0: initial solve)
     unsigned* strip_flags; // [S] (FUSED true): the launch's epoch once the strip's z is stored
     unsigned epoch;
+    int check;                    // FUSED: the iteration's stopping test (:453) first, as FwdOp's
+    const double* rmax_partials;  // FUSED: |r|_inf partials of the iteration's p/r update
+    int n_partials;
     __device__ bool skip(bool first) const {
         if (sc->done) return true;
+        if (FUSED && check) {
+            const double rn = n_partials <= 64 * 32 ? wave_max_of_wide<32>(rmax_partials, n_partials)
+                                                    : wave_max_of(rmax_partials, n_partials);
+            const bool conv = rn <= sc->tol * sc->rhs_norm;
+            if (first && (threadIdx.x & 63) == 0) {
+                sc->r_norm = rn;
+                if (conv) sc->done = 1;
+            }
+            if (conv) return true;
+        }
         if (first && (threadIdx.x & 63) == 0 && next_iter > 0) sc->iter = next_iter;
         return false;
     }

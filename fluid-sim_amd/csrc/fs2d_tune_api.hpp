@@ -11,9 +11,10 @@
 //   sweep_prefetch  (ensembles) whether the batched solves' loaders issue
 //                   their first loads before the skip test: sooner data for
 //                   members still iterating, wasted loads for converged ones.
-//   fused_spmv      (single handles) the backward solve and the next SpMV in
-//                   one launch (bwd_spmv_kernel) or in two.  The SpMV's
-//                   virtual workgroups are the same either way, so the bits are.
+//   fused_pcg       (single handles) a PCG iteration as two launches
+//                   (fwd_axpy_kernel, bwd_spmv_kernel) or as four.  The
+//                   SpMV's virtual workgroups, each entry's update and the
+//                   maximum of |r| are the same either way, so the bits are.
 //
 // The tuner times each candidate on the handle's own state: `iters` PCG
 // iterations of the projection (tolerance 0, so every member runs them all),
@@ -45,8 +46,8 @@ std::string device_tag() {
     return buf;
 }
 
-// The cache file: one line per key, "key pcg_grid sweep_prefetch fused_spmv"
-// (a line without the last value means fused_spmv 1); the last line of a key
+// The cache file: one line per key, "key pcg_grid sweep_prefetch fused_pcg"
+// (a line without the last value means fused_pcg 1); the last line of a key
 // wins.
 bool cache_lookup(const char* path, const std::string& key, fs2d_tuning* t) {
     if (!path) return false;
@@ -59,7 +60,7 @@ bool cache_lookup(const char* path, const std::string& key, fs2d_tuning* t) {
         if (sscanf(line, "%511s %d %d %d", k, &g, &p, &fu) >= 3 && key == k) {
             t->pcg_grid = g;
             t->sweep_prefetch = p;
-            t->fused_spmv = fu;
+            t->fused_pcg = fu;
             found = true;
         }
     }
@@ -70,7 +71,7 @@ void cache_store(const char* path, const std::string& key, const fs2d_tuning& t)
     if (!path) return;
     FILE* f = fopen(path, "a");
     if (!f) return;
-    fprintf(f, "%s %d %d %d\n", key.c_str(), t.pcg_grid, t.sweep_prefetch, t.fused_spmv);
+    fprintf(f, "%s %d %d %d\n", key.c_str(), t.pcg_grid, t.sweep_prefetch, t.fused_pcg);
     fclose(f);
 }
 
@@ -111,21 +112,21 @@ fs2d_status fs2d_get_tuning(fs2d_sim* s, fs2d_tuning* t) {
     if (!s || !t) return FS2D_ERR_ARG;
     t->pcg_grid = s->pcg_grid;
     t->sweep_prefetch = 0;
-    t->fused_spmv = s->fused_spmv ? 1 : 0;
+    t->fused_pcg = s->fused_pcg ? 1 : 0;
     return FS2D_OK;
 }
 
 fs2d_status fs2d_set_tuning(fs2d_sim* s, const fs2d_tuning* t) {
-    if (!s || !t || t->pcg_grid < 0 || (t->pcg_grid & 7) || (t->fused_spmv != 0 && t->fused_spmv != 1))
+    if (!s || !t || t->pcg_grid < 0 || (t->pcg_grid & 7) || (t->fused_pcg != 0 && t->fused_pcg != 1))
         return FS2D_ERR_ARG;
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    if ((t->fused_spmv != 0) != s->fused_spmv || t->pcg_grid != s->pcg_grid) {
+    if ((t->fused_pcg != 0) != s->fused_pcg || t->pcg_grid != s->pcg_grid) {
         // captured steps hold the old launches
         for (auto& g : s->graphs) graph_free(g);
         s->graphs.clear();
     }
     s->pcg_grid = t->pcg_grid;
-    s->fused_spmv = t->fused_spmv != 0;
+    s->fused_pcg = t->fused_pcg != 0;
     return FS2D_OK;
 }
 
@@ -133,7 +134,7 @@ fs2d_status fs2d_batch_get_tuning(fs2d_batch* B, fs2d_tuning* t) {
     if (!B || !t) return FS2D_ERR_ARG;
     t->pcg_grid = B->pcg_grid;
     t->sweep_prefetch = B->sweep_prefetch ? 1 : 0;
-    t->fused_spmv = 0;
+    t->fused_pcg = 0;
     return FS2D_OK;
 }
 
@@ -152,7 +153,7 @@ fs2d_status fs2d_autotune(fs2d_sim* s, int iters, const char* cache_path, fs2d_t
     snprintf(kb, sizeof kb, "%s|single|%dx%d|mode%d|flags%d", device_tag().c_str(), s->nx, s->ny, s->mode,
              s->flags);
     const std::string key = kb;
-    fs2d_tuning best{s->pcg_grid, 0, s->fused_spmv ? 1 : 0};
+    fs2d_tuning best{s->pcg_grid, 0, s->fused_pcg ? 1 : 0};
     if (from_cache) *from_cache = 0;
     if (cache_lookup(cache_path, key, &best)) {
         if (from_cache) *from_cache = 1;
@@ -168,9 +169,9 @@ fs2d_status fs2d_autotune(fs2d_sim* s, int iters, const char* cache_path, fs2d_t
     double best_t = 1e30;
     fs2d_status st = FS2D_OK;
     for (int fu = 1; fu >= 0 && st >= 0; fu--)
-        for (int g : grid_candidates(pcg_blocks(s), resident)) {
+        for (int g : fu ? std::vector<int>{0} : grid_candidates(pcg_blocks(s), resident)) {
             s->pcg_grid = g;
-            s->fused_spmv = fu != 0;
+            s->fused_pcg = fu != 0;
             double t_min = 1e30;
             for (int rep = 0; rep < 4 && st >= 0; rep++) {
                 HIPCHK(s, hipStreamSynchronize(s->stream));
@@ -183,14 +184,14 @@ fs2d_status fs2d_autotune(fs2d_sim* s, int iters, const char* cache_path, fs2d_t
             if (t_min < best_t) {
                 best_t = t_min;
                 best.pcg_grid = g;
-                best.fused_spmv = fu;
+                best.fused_pcg = fu;
             }
         }
     s->tol = tol;
     s->max_iters = max_iters;
     s->last_iters_hint = hint;
     s->pcg_grid = best.pcg_grid;
-    s->fused_spmv = best.fused_spmv != 0;
+    s->fused_pcg = best.fused_pcg != 0;
     if (st < 0) return st;
     cache_store(cache_path, key, best);
     if (out) *out = best;

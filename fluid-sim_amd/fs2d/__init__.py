@@ -52,7 +52,7 @@ class Stats(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("pcg_grid", ctypes.c_int), ("sweep_prefetch", ctypes.c_int), ("fused_spmv", ctypes.c_int)]
+    _fields_ = [("pcg_grid", ctypes.c_int), ("sweep_prefetch", ctypes.c_int), ("fused_pcg", ctypes.c_int)]
 
 
 RENDER_BUFFERS = ("water_cells", "solid_cells", "cell_vel_lines", "pressure_cells", "pressure_values",
@@ -352,10 +352,10 @@ class Sim:
     def tuning(self):
         t = Tuning()
         self._check(self.L.fs2d_get_tuning(self.h, ctypes.byref(t)))
-        return {"pcg_grid": t.pcg_grid, "sweep_prefetch": t.sweep_prefetch, "fused_spmv": t.fused_spmv}
+        return {"pcg_grid": t.pcg_grid, "sweep_prefetch": t.sweep_prefetch, "fused_pcg": t.fused_pcg}
 
-    def set_tuning(self, pcg_grid=0, sweep_prefetch=0, fused_spmv=1):
-        t = Tuning(pcg_grid, sweep_prefetch, fused_spmv)
+    def set_tuning(self, pcg_grid=0, sweep_prefetch=0, fused_pcg=1):
+        t = Tuning(pcg_grid, sweep_prefetch, fused_pcg)
         self._check(self.L.fs2d_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self, iters=24, cache=None):
@@ -364,7 +364,7 @@ class Sim:
         t, fc = Tuning(), ctypes.c_int()
         self._check(self.L.fs2d_autotune(self.h, iters, cache.encode() if cache else None, ctypes.byref(t),
                                          ctypes.byref(fc)))
-        return {"pcg_grid": t.pcg_grid, "sweep_prefetch": t.sweep_prefetch, "fused_spmv": t.fused_spmv,
+        return {"pcg_grid": t.pcg_grid, "sweep_prefetch": t.sweep_prefetch, "fused_pcg": t.fused_pcg,
                 "from_cache": bool(fc.value)}
 
     def set_timing(self, on, only=None):

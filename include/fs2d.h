@@ -240,12 +240,16 @@ const char* fs2d_status_string(int status);
  *   pcg_grid: physical workgroups of the PCG SpMV and axpy launches, a
  *     multiple of 8; 0 (single handles only) launches one per partial sum.
  *     The loops run a fixed list of virtual workgroups whatever the launch
- *     size, so every value gives the same bits.
+ *     size, so every value gives the same bits.  Unused by fused_pcg
+ *     iterations, whose launches size themselves to the device.
  *   sweep_prefetch: (ensembles) the batched triangular solves' loaders issue
  *     their first loads before testing whether their member has converged.
- *   fused_spmv: (single handles) 1, the default: the backward solve's launch
- *     also runs the next iteration's SpMV, whose workgroups take each region
- *     as soon as the solve's strips around it have finished; 0: separate
+ *   fused_pcg: (single handles) 1, the default: a PCG iteration is two
+ *     launches instead of four: the p/r update with the forward solve, whose
+ *     strips start as soon as the update has passed them, and the backward
+ *     solve with the next iteration's SpMV, whose workgroups take each region
+ *     as soon as the solve's strips around it have finished (the stopping
+ *     test moves from the forward to the backward launch); 0: separate
  *     launches.  Ensembles report 0 and ignore it.
  * fs2d_autotune / fs2d_batch_autotune time the candidates on the handle's
  * current state (call them after a first step): `iters` PCG iterations of the
@@ -258,7 +262,7 @@ const char* fs2d_status_string(int status);
 typedef struct {
     int pcg_grid;
     int sweep_prefetch;
-    int fused_spmv;
+    int fused_pcg;
 } fs2d_tuning;
 fs2d_status fs2d_get_tuning(fs2d_sim* sim, fs2d_tuning* out);
 fs2d_status fs2d_set_tuning(fs2d_sim* sim, const fs2d_tuning* t);

@@ -5,7 +5,7 @@ The PCG SpMV and axpy loop over a fixed list of virtual workgroups, each
 writing its own partial sums, so the number of physical workgroups cannot
 change a bit of the result; nor can whether the batched solves' loaders
 prefetch, nor whether a single handle runs the SpMV inside the backward
-solve's launch (fused_spmv: the same virtual workgroups, started as the
+solve's launch (fused_pcg: the same virtual workgroups, started as the
 solve's strips finish, with sigma summed in the solve's loaders' order).
 Steps under every tuning must match bit for bit, and the autotuner must
 record its choice and read it back.
@@ -35,15 +35,15 @@ def smoke(n, seed):
 def test_single_tunings_bit_identical():
     n = 300  # pcg_blocks 256: tunings below it loop several virtual workgroups per workgroup
     ref = smoke(n, 7)
-    assert ref.tuning()["fused_spmv"] == 1  # the default
+    assert ref.tuning()["fused_pcg"] == 1  # the default
     for _ in range(3):
         ref.step()
     want = [ref.get(f) for f in FIELDS]
     it = ref.stats()["pcg_iters"]
     for g, fu in ((8, 1), (64, 0), (248, 1), (0, 0)):
         s = smoke(n, 7)
-        s.set_tuning(pcg_grid=g, fused_spmv=fu)
-        assert s.tuning()["pcg_grid"] == g and s.tuning()["fused_spmv"] == fu
+        s.set_tuning(pcg_grid=g, fused_pcg=fu)
+        assert s.tuning()["pcg_grid"] == g and s.tuning()["fused_pcg"] == fu
         for _ in range(3):
             s.step()
         for f, w in zip(FIELDS, want):
@@ -68,12 +68,12 @@ def box(nx, ny, seed):
 
 
 @pytest.mark.parametrize("nx,ny", [(64, 64), (1000, 130), (300, 300), (200, 700), (64, 16000)])
-def test_fused_spmv_bit_identical_across_strip_counts(nx, ny):
+def test_fused_pcg_bit_identical_across_strip_counts(nx, ny):
     """1, 3, 5, 11 and 250 strips; the last launches 125 + 250 + 8 workgroups
     of one CU each, more than the device holds at once, so workgroups wait on
     strips that started before them across rounds of residency."""
     a, b = box(nx, ny, 5), box(nx, ny, 5)
-    b.set_tuning(pcg_grid=0, fused_spmv=0)
+    b.set_tuning(pcg_grid=0, fused_pcg=0)
     for step in range(2):
         a.step()
         b.step()
@@ -85,7 +85,7 @@ def test_fused_spmv_bit_identical_across_strip_counts(nx, ny):
     b.close()
 
 
-def test_fused_spmv_pic_and_captured_steps():
+def test_fused_pcg_pic_and_captured_steps():
     """A PIC/FLIP dam break (EMPTY cells, a changing matrix) stepped with the
     fused launch, uncaptured and from captured graphs, against separate
     launches."""
@@ -94,7 +94,7 @@ def test_fused_spmv_pic_and_captured_steps():
     c = scenes.dam_break(n)
     sims = [fs2d.Sim(c, **kw) for _ in range(3)]
     sims[1].set_graph(True)
-    sims[2].set_tuning(pcg_grid=0, fused_spmv=0)
+    sims[2].set_tuning(pcg_grid=0, fused_pcg=0)
     for step in range(4):
         for s in sims:
             s.step()
