@@ -18,9 +18,11 @@ every step runs into the iteration cap.
 `roofline` reports the dominant kernel class: algorithmic bytes per launch
 (BYTES_PER_CELL x cells) over its average launch time, measured with HIP
 events on the library's stream inside the timed region.  With the tuning's
-fused_pcg (the default) the backward-solve class is one launch that also
-runs the next iteration's SpMV, and it is reported as
-`pcg_backward_solve+spmv` with both kernels' arrays.  Launches that start
+fused_pcg (the default) a PCG iteration is two launches: the backward-solve
+class also runs the next iteration's SpMV (reported as
+`pcg_backward_solve+spmv`, with both kernels' arrays) and the forward-solve
+class the p/r update (`pcg_forward_solve+axpy`).  `--no-events` drops every
+HIP event (for profiler runs; no roofline).  Launches that start
 after convergence exit at once; the average is taken over the launches that
 did work (PCG iterations + 1 per step for the two solves).  Each timed launch
 adds two events to the stream, and events around every kernel cost ~8% of
@@ -98,6 +100,9 @@ BYTES_PER_CELL = {
     # the backward solve with the next SpMV in one launch: q, P, gx, gy in, z
     # out; s, Adiag, coupling byte in, s', w out (z and q counted once)
     "pcg_backward_solve+spmv": 37,
+    # the p/r update with the forward solve in one launch: p, r, s, w in, p, r
+    # out; P, fx, fy in, q out (r counted once each way)
+    "pcg_forward_solve+axpy": 40,
     "pcg_spmv_dot": 21,         # z, s, Adiag, coupling byte in; s', w out
     "pcg_axpy_norm": 24,        # p, r, s, w in; p, r out
     "mic0_factor": 25,          # Adiag, coupling byte in; P, fx, fy, gx, gy out
@@ -436,6 +441,8 @@ def main():
     ap.add_argument("--loop-steps", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-grid3d", action="store_true", help="skip the 3D grid leg")
+    ap.add_argument("--no-events", action="store_true",
+                    help="no HIP events anywhere (for profiler runs); no roofline or per-class breakdown")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the captured-graph twins of the 2048^2 and ensemble legs (profiling runs)")
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "fluid-sim_amd", "tune_cache.txt"),
@@ -482,18 +489,21 @@ def main():
         for s_ in twins:
             s_.step()
     # per-class breakdown and the dominant class, with every class timed
-    sim.synchronize()
-    sim.reset_kernel_times()
-    sim.set_timing(True)
-    for _ in range(PROFILE_STEPS):
-        for s_ in twins:
-            s_.step()
-    kt_all = sim.kernel_times()
-    sim.set_timing(False)
-    dom = max((k for k in kt_all if k in BYTES_PER_CELL), key=lambda k: kt_all[k][0])
+    kt_all, dom = {}, None
+    if not args.no_events:
+        sim.synchronize()
+        sim.reset_kernel_times()
+        sim.set_timing(True)
+        for _ in range(PROFILE_STEPS):
+            for s_ in twins:
+                s_.step()
+        kt_all = sim.kernel_times()
+        sim.set_timing(False)
+        dom = max((k for k in kt_all if k in BYTES_PER_CELL), key=lambda k: kt_all[k][0])
     # timed region: only the dominant class carries events
     sim.reset_kernel_times()
-    sim.set_timing(True, only=[dom])
+    if dom:
+        sim.set_timing(True, only=[dom])
     iters = []
     capped = 0
     sim.synchronize()
@@ -526,13 +536,25 @@ def main():
     no_graph = {"no_graph_value": value, "no_graph_ms_per_step": ms_per_step}
     if graph_out:
         value, ms_per_step = graph_out["value"], graph_out["ms_per_step"]
-    tot_ms, launches = kt[dom]
-    if dom in EFFECTIVE:
-        launches = int(sum(iters)) + EFFECTIVE[dom] * args.steps
-    avg_s = tot_ms / 1e3 / max(launches, 1)
-    label = dom + "+spmv" if dom == "pcg_backward_solve" and tune.get("fused_pcg") else dom
-    bytes_per_launch = BYTES_PER_CELL[label] * n * n
-    achieved = bytes_per_launch / avg_s / 1e9
+    roofline = None
+    if dom:
+        tot_ms, launches = kt[dom]
+        if dom in EFFECTIVE:
+            # fused_pcg: the forward class also runs the speculative solve of the converged iteration
+            extra = EFFECTIVE[dom] + (1 if dom == "pcg_forward_solve" and tune.get("fused_pcg") else 0)
+            launches = int(sum(iters)) + extra * args.steps
+        avg_s = tot_ms / 1e3 / max(launches, 1)
+        label = dom
+        if tune.get("fused_pcg") and dom in ("pcg_backward_solve", "pcg_forward_solve"):
+            label = dom + ("+spmv" if dom == "pcg_backward_solve" else "+axpy")
+        bytes_per_launch = BYTES_PER_CELL[label] * n * n
+        achieved = bytes_per_launch / avg_s / 1e9
+        roofline = {
+            "bound": "hbm", "kernel": label, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(label, n),
+            "avg_launch_us": avg_s * 1e6, "launches": launches, "launches_all": kt[dom][1],
+            "bytes_per_launch": bytes_per_launch,
+        }
     out = {
         "metric": METRIC,
         "value": value,
@@ -556,12 +578,7 @@ def main():
         },
         "graph": graph_out,
         **no_graph,
-        "roofline": {
-            "bound": "hbm", "kernel": label, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(label, n),
-            "avg_launch_us": avg_s * 1e6, "launches": launches, "launches_all": kt[dom][1],
-            "bytes_per_launch": bytes_per_launch,
-        },
+        "roofline": roofline,
         "kernel_ms_per_step": {k: v[0] / PROFILE_STEPS for k, v in kt_all.items() if v[1]},
     }
     sim.close()
