@@ -857,39 +857,49 @@ __device__ __forceinline__ void fwd_axpy_worker(const SkGeom& g, const FusedAxpy
     const float4* s4 = (const float4*)f.s;
     const float4* w4 = (const float4*)f.w;
     const float4* r4 = (const float4*)f.r;
+    constexpr int U = 4;
+    float4 sv[U], wv[U], pv[U], rv[U];
+    auto load = [&](long e0, long q1) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const long e = e0 + u * kBlock;
+            if (e < q1) {
+                sv[u] = s4[e];
+                wv[u] = w4[e];
+                pv[u] = p4[e];
+                rv[u] = r4[e];
+            }
+        }
+    };
+    // the first run's first entries do not depend on alpha: their loads go
+    // out before its reduction
+    long q0, q1;
+    pcg_run(g.size(), m, f.nvb, q0, q1);
+    if (threadIdx.x < kBlock) load(q0 + threadIdx.x, q1);
     // alpha = sigma / w.s as pcg_axpy_sk_kernel's block_sum_bcast forms it,
-    // with each lane's loads in flight together (both sums fit one round)
+    // with every load of both sums in flight together (one memory round
+    // trip: the update's first runs, and so the first strip, wait on it)
     __shared__ float sh_a;
     if (threadIdx.x < 64) {
-        const double sig = f.nvb <= 64 * 32 && g.S * kLoaders <= 64 * 32
-                               ? wave_sum_of_wide<32>(f.sig_cur, g.S * kLoaders)
-                               : wave_sum_of(f.sig_cur, g.S * kLoaders);
-        const double ws = f.nvb <= 64 * 32 ? wave_sum_of_wide<32>(f.ws_partials, f.nvb)
-                                           : wave_sum_of(f.ws_partials, f.nvb);
+        double sig, ws;
+        if (g.S * kLoaders <= 64 * 12 && f.nvb <= 64 * 32) {
+            wave_sum_of_wide2<12, 32>(f.sig_cur, g.S * kLoaders, f.ws_partials, f.nvb, sig, ws);
+        } else {
+            sig = wave_sum_of(f.sig_cur, g.S * kLoaders);
+            ws = wave_sum_of(f.ws_partials, f.nvb);
+        }
         if (threadIdx.x == 0) sh_a = (float)(sig / ws);
     }
     __syncthreads();
     const float a = sh_a;
     const __amdgpu_buffer_rsrc_t r_rs = __builtin_amdgcn_make_buffer_rsrc(f.r, 0, (int)(g.size() * 4), kBufferCfg);
     typedef float f4 __attribute__((ext_vector_type(4)));
-    constexpr int U = 4;
     for (int vb = m; vb < f.nvb; vb += f.M) {
-        long q0, q1;
         pcg_run(g.size(), vb, f.nvb, q0, q1);
         float mx = 0.0f;
         if (threadIdx.x < kBlock) {
             for (long e0 = q0 + threadIdx.x; e0 < q1; e0 += U * kBlock) {
-                float4 sv[U], wv[U], pv[U], rv[U];
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const long e = e0 + u * kBlock;
-                    if (e < q1) {
-                        sv[u] = s4[e];
-                        wv[u] = w4[e];
-                        pv[u] = p4[e];
-                        rv[u] = r4[e];
-                    }
-                }
+                if (vb != m || e0 != q0 + threadIdx.x) load(e0, q1);
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const long e = e0 + u * kBlock;

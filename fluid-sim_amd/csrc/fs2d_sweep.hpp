@@ -654,6 +654,32 @@ __device__ __forceinline__ double wave_sum_of_wide(const double* a, int n) {
     for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
     return t;
 }
+// Two such sums with all loads of both in flight together (n1 <= 64 W1,
+// n2 <= 64 W2): one memory round trip for both.
+template <int W1, int W2>
+__device__ __forceinline__ void wave_sum_of_wide2(const double* a, int n1, const double* b, int n2, double& sa,
+                                                  double& sb) {
+    const int lane = threadIdx.x & 63;
+    double va[W1], vb[W2];
+#pragma unroll
+    for (int u = 0; u < W1; u++) va[u] = lane + 64 * u < n1 ? a[lane + 64 * u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < W2; u++) vb[u] = lane + 64 * u < n2 ? b[lane + 64 * u] : 0.0;
+    double ta = 0, tb = 0;
+#pragma unroll
+    for (int u = 0; u < W1; u++)
+        if (lane + 64 * u < n1) ta += va[u];
+#pragma unroll
+    for (int u = 0; u < W2; u++)
+        if (lane + 64 * u < n2) tb += vb[u];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        ta += __shfl_xor(ta, o, 64);
+        tb += __shfl_xor(tb, o, 64);
+    }
+    sa = ta;
+    sb = tb;
+}
 __device__ __forceinline__ double wave_sum_of(const double* a, int n) {
     double t = 0;
     for (int k0 = threadIdx.x & 63; k0 < n; k0 += 64 * 8) {
