@@ -249,57 +249,71 @@ __device__ __forceinline__ bool sk_cell(const SkGeom& g, long e, int& i, int& j)
 // off-diagonals are -1 towards FLUID neighbours).  `scale` is the rhs factor
 // 1 / (dx scaleA).  The PCG vectors p and s start at zero (:424).  Partial
 // |rhs|_inf per block.
+// Tiles of 64 rows (a strip) x 32 steps, a tile per workgroup and round
+// (vb, vb + nvb, ...): each wave takes two rows of 32 consecutive steps at a
+// time, so its reads of the row-major cell, phi, u and v rows are coalesced,
+// and a tile's SK4 stores fill whole cache lines (a line holds 8 rows x 4
+// steps) before they leave the L2.  Padding entries (no cell) get zeros like
+// the cells without an equation.  partials[vb]: the largest |rhs| of the
+// workgroup's tiles.
 __device__ __forceinline__ void assemble_sk_body(
     SkGeom g, const uint8_t* __restrict__ cell, const double* __restrict__ phi, const float* __restrict__ u,
     const float* __restrict__ v, float* __restrict__ Adiag, uint8_t* __restrict__ flags, float* __restrict__ r,
     float* __restrict__ p, float* __restrict__ s0, double scale, double* partials, int ghost, int vb, int nvb) {
     constexpr double scaleA = 1.0;
     const int nx = g.nx;
+    const int tiles_per_strip = g.T4 / 8;
+    const long ntiles = (long)g.S * tiles_per_strip;
+    const int rr = threadIdx.x >> 5, tt = threadIdx.x & 31;
     double m = 0;
-    for (long e = vb * (long)blockDim.x + threadIdx.x; e < g.size(); e += (long)nvb * blockDim.x) {
-        int i, j;
-        double ad = 0, rhs = 0;
-        uint32_t f = 0;
-        if (sk_cell(g, e, i, j)) {
-            const long c = (long)j * nx + i;
-            if (cell[c] == FLUID) {
-                const uint8_t cl = cell[c - 1], cr = cell[c + 1], cb = cell[c - nx], ct = cell[c + nx];
-                if (ghost) {
-                    const double ph = phi[c];
-                    if (cl == FLUID) { ad += scaleA; f |= CPL_L; }
-                    else if (cl == EMPTY) ad -= scaleA * fmax(phi[c - 1] / ph, -1e3);
-                    if (cr == FLUID) { ad += scaleA; f |= CPL_R; }
-                    else if (cr == EMPTY) ad += scaleA * (1 - fmax(phi[c + 1] / ph, -1e3));
-                    if (cb == FLUID) { ad += scaleA; f |= CPL_D; }
-                    else if (cb == EMPTY) ad -= scaleA * fmax(phi[c - nx] / ph, -1e3);
-                    if (ct == FLUID) { ad += scaleA; f |= CPL_U; }
-                    else if (ct == EMPTY) ad += scaleA * (1 - fmax(phi[c + nx] / ph, -1e3));
-                } else {
-                    // without USE_GHOST_PRESSURE (:304-331): only the +x and +y EMPTY
-                    // neighbours add to the diagonal
-                    if (cl == FLUID) { ad += scaleA; f |= CPL_L; }
-                    if (cr == FLUID) { ad += scaleA; f |= CPL_R; }
-                    else if (cr == EMPTY) ad += scaleA;
-                    if (cb == FLUID) { ad += scaleA; f |= CPL_D; }
-                    if (ct == FLUID) { ad += scaleA; f |= CPL_U; }
-                    else if (ct == EMPTY) ad += scaleA;
+    for (long tile = vb; tile < ntiles; tile += nvb) {
+        const int s = (int)(tile / tiles_per_strip), t = (int)(tile % tiles_per_strip) * 32 + tt;
+        for (int l = rr; l < 64; l += 8) {
+            const int i = t - l, j = s * 64 + l;
+            const long e = (((long)s * g.T4 + (t >> 2)) * 64 + l) * 4 + (t & 3);
+            double ad = 0, rhs = 0;
+            uint32_t f = 0;
+            if (i >= 0 && i < nx && j < g.ny) {
+                const long c = (long)j * nx + i;
+                if (cell[c] == FLUID) {
+                    const uint8_t cl = cell[c - 1], cr = cell[c + 1], cb = cell[c - nx], ct = cell[c + nx];
+                    if (ghost) {
+                        const double ph = phi[c];
+                        if (cl == FLUID) { ad += scaleA; f |= CPL_L; }
+                        else if (cl == EMPTY) ad -= scaleA * fmax(phi[c - 1] / ph, -1e3);
+                        if (cr == FLUID) { ad += scaleA; f |= CPL_R; }
+                        else if (cr == EMPTY) ad += scaleA * (1 - fmax(phi[c + 1] / ph, -1e3));
+                        if (cb == FLUID) { ad += scaleA; f |= CPL_D; }
+                        else if (cb == EMPTY) ad -= scaleA * fmax(phi[c - nx] / ph, -1e3);
+                        if (ct == FLUID) { ad += scaleA; f |= CPL_U; }
+                        else if (ct == EMPTY) ad += scaleA * (1 - fmax(phi[c + nx] / ph, -1e3));
+                    } else {
+                        // without USE_GHOST_PRESSURE (:304-331): only the +x and +y EMPTY
+                        // neighbours add to the diagonal
+                        if (cl == FLUID) { ad += scaleA; f |= CPL_L; }
+                        if (cr == FLUID) { ad += scaleA; f |= CPL_R; }
+                        else if (cr == EMPTY) ad += scaleA;
+                        if (cb == FLUID) { ad += scaleA; f |= CPL_D; }
+                        if (ct == FLUID) { ad += scaleA; f |= CPL_U; }
+                        else if (ct == EMPTY) ad += scaleA;
+                    }
+                    const long iu = (long)j * (nx + 1) + i;
+                    const double u0 = u[iu], u1 = u[iu + 1], v0 = v[c], v1 = v[c + nx];
+                    rhs = -scale * (u1 - u0 + v1 - v0);
+                    if (cl == SOLID) rhs -= scale * u0;
+                    if (cr == SOLID) rhs += scale * u1;
+                    if (cb == SOLID) rhs -= scale * v0;
+                    if (ct == SOLID) rhs += scale * v1;
                 }
-                const long iu = (long)j * (nx + 1) + i;
-                const double u0 = u[iu], u1 = u[iu + 1], v0 = v[c], v1 = v[c + nx];
-                rhs = -scale * (u1 - u0 + v1 - v0);
-                if (cl == SOLID) rhs -= scale * u0;
-                if (cr == SOLID) rhs += scale * u1;
-                if (cb == SOLID) rhs -= scale * v0;
-                if (ct == SOLID) rhs += scale * v1;
             }
+            Adiag[e] = (float)ad;
+            flags[e] = (uint8_t)f;
+            r[e] = (float)rhs;
+            p[e] = 0.0f;
+            s0[e] = 0.0f;
+            const double a = fabs((double)(float)rhs);
+            m = a > m ? a : m;
         }
-        Adiag[e] = (float)ad;
-        flags[e] = (uint8_t)f;
-        r[e] = (float)rhs;
-        p[e] = 0.0f;
-        s0[e] = 0.0f;
-        const double a = fabs((double)(float)rhs);
-        m = a > m ? a : m;
     }
     double bm = block_max(m);
     if (threadIdx.x == 0) partials[vb] = bm;
