@@ -67,11 +67,14 @@ def box(nx, ny, seed):
     return s
 
 
-@pytest.mark.parametrize("nx,ny", [(64, 64), (1000, 130), (300, 300), (200, 700), (64, 16000)])
+@pytest.mark.parametrize("nx,ny", [(64, 64), (1000, 130), (300, 300), (200, 700), (64, 16000), (16384, 64),
+                                   (16000, 70), (5, 5), (129, 65)])
 def test_fused_pcg_bit_identical_across_strip_counts(nx, ny):
-    """1, 3, 5, 11 and 250 strips; the last launches 125 + 250 + 8 workgroups
-    of one CU each, more than the device holds at once, so workgroups wait on
-    strips that started before them across rounds of residency."""
+    """1, 3, 5, 11 and 250 strips; the 250-strip grid launches 125 + 250 + 8
+    workgroups of one CU each, more than the device holds at once, so
+    workgroups wait on strips that started before them across rounds of
+    residency.  A 16384-wide single strip puts every SpMV run and every p/r
+    update run on one strip; 5 x 5 is the smallest launch (8 runs)."""
     a, b = box(nx, ny, 5), box(nx, ny, 5)
     b.set_tuning(pcg_grid=0, fused_pcg=0)
     for step in range(2):
