@@ -793,7 +793,10 @@ __device__ __forceinline__ void bwd_spmv_worker(const SkGeom& g, const FusedSpmv
             if (sh_bad) return;
             lo_ready = need;
         }
-        spmv_core_wide<8>(xl, g, f.Adiag, f.flags, f.z, f.s_old, f.s_new, f.w, beta, f.ws_partials, vb);
+        // 9 entries at once: a run is n / 256 / nvb groups, about 8.4 at 2048^2
+        // (pcg_blocks aims at 8 entries per thread), so one batch takes it whole
+        // (8 at once: 1.2 us more per launch; 12: slower, register pressure)
+        spmv_core_wide<9>(xl, g, f.Adiag, f.flags, f.z, f.s_old, f.s_new, f.w, beta, f.ws_partials, vb);
     }
 }
 
