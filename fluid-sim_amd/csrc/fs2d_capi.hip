@@ -38,6 +38,12 @@ const char* kKernelNames[K_COUNT] = {
     "adjoint_other"};
 
 constexpr int kMaxBlocks = 2048;
+// Solver layouts of at least this many entries (a 512^2 grid has 294912) run
+// a PCG iteration as two launches by default: measured per iteration against
+// four launches, 0.94-0.98x from 512^2 up, 1.03-1.08x at 64^2-384^2, where
+// the separate SpMV and axpy are short and the fused launches' head and tail
+// cost more than they hide.
+constexpr long kFusedPcgMinEntries = 1L << 18;
 
 int blocks_for(long n) {
     long b = (n + kBlock - 1) / kBlock;
@@ -1060,6 +1066,7 @@ fs2d_status fs2d_create(const fs2d_config* cfg, fs2d_sim** out) {
     if (s->flags & FS2D_FLAG_PARTICLE_CELLS) A(cell2, nC);
     s->geom = make_geom(s->nx, s->ny);
     const size_t nS = (size_t)s->geom.size();
+    s->fused_pcg = (long)nS >= kFusedPcgMinEntries;
     A(p, nC);
     A(r, nS); A(z, nS); A(q, nS); A(w, nS); A(s[0], nS); A(s[1], nS); A(pk, nS);
     A(Adiag, nS); A(P, nS); A(fx, nS); A(fy, nS); A(gx_, nS); A(gy_, nS); A(cpl, nS);

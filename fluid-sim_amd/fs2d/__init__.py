@@ -354,7 +354,10 @@ class Sim:
         self._check(self.L.fs2d_get_tuning(self.h, ctypes.byref(t)))
         return {"pcg_grid": t.pcg_grid, "sweep_prefetch": t.sweep_prefetch, "fused_pcg": t.fused_pcg}
 
-    def set_tuning(self, pcg_grid=0, sweep_prefetch=0, fused_pcg=1):
+    def set_tuning(self, pcg_grid=0, sweep_prefetch=0, fused_pcg=None):
+        """fused_pcg None keeps the handle's current choice."""
+        if fused_pcg is None:
+            fused_pcg = self.tuning()["fused_pcg"]
         t = Tuning(pcg_grid, sweep_prefetch, fused_pcg)
         self._check(self.L.fs2d_set_tuning(self.h, ctypes.byref(t)))
 

@@ -244,8 +244,10 @@ const char* fs2d_status_string(int status);
  *     iterations, whose launches size themselves to the device.
  *   sweep_prefetch: (ensembles) the batched triangular solves' loaders issue
  *     their first loads before testing whether their member has converged.
- *   fused_pcg: (single handles) 1, the default: a PCG iteration is two
- *     launches instead of four: the p/r update with the forward solve, whose
+ *   fused_pcg: (single handles) 1, the default for solver layouts of at least
+ *     2^18 entries (ceil(size_y / 64) * 64 * ceil((size_x + 64) / 32) * 32;
+ *     512 x 512 has 294912; below it the separate launches measured 3-8%
+ *     faster): a PCG iteration is two launches instead of four: the p/r update with the forward solve, whose
  *     strips start as soon as the update has passed them, and the backward
  *     solve with the next iteration's SpMV, whose workgroups take each region
  *     as soon as the solve's strips around it have finished (the stopping

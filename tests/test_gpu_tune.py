@@ -35,12 +35,12 @@ def smoke(n, seed):
 def test_single_tunings_bit_identical():
     n = 300  # pcg_blocks 256: tunings below it loop several virtual workgroups per workgroup
     ref = smoke(n, 7)
-    assert ref.tuning()["fused_pcg"] == 1  # the default
+    assert ref.tuning()["fused_pcg"] == 0  # the default below 2^18 solver entries
     for _ in range(3):
         ref.step()
     want = [ref.get(f) for f in FIELDS]
     it = ref.stats()["pcg_iters"]
-    for g, fu in ((8, 1), (64, 0), (248, 1), (0, 0)):
+    for g, fu in ((8, 1), (64, 0), (248, 1), (0, 1)):
         s = smoke(n, 7)
         s.set_tuning(pcg_grid=g, fused_pcg=fu)
         assert s.tuning()["pcg_grid"] == g and s.tuning()["fused_pcg"] == fu
@@ -76,6 +76,7 @@ def test_fused_pcg_bit_identical_across_strip_counts(nx, ny):
     residency.  A 16384-wide single strip puts every SpMV run and every p/r
     update run on one strip; 5 x 5 is the smallest launch (8 runs)."""
     a, b = box(nx, ny, 5), box(nx, ny, 5)
+    a.set_tuning(pcg_grid=0, fused_pcg=1)
     b.set_tuning(pcg_grid=0, fused_pcg=0)
     for step in range(2):
         a.step()
@@ -96,6 +97,8 @@ def test_fused_pcg_pic_and_captured_steps():
     kw = dict(k=2, dt=0.002, dx=1.0 / n, mode=fs2d.PICFLIP, alpha=0.05, tau=0.97, tol=1e-6, max_iters=500)
     c = scenes.dam_break(n)
     sims = [fs2d.Sim(c, **kw) for _ in range(3)]
+    for s in sims[:2]:
+        s.set_tuning(pcg_grid=0, fused_pcg=1)
     sims[1].set_graph(True)
     sims[2].set_tuning(pcg_grid=0, fused_pcg=0)
     for step in range(4):
