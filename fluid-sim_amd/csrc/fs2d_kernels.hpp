@@ -215,19 +215,13 @@ __global__ void gravity_kernel(float* __restrict__ u, float* __restrict__ v, lon
 struct XcdLoop {
     long e, end, stride;
 };
-// xcd_loop_t: the same for a virtual workgroup of `bd` threads run by
-// threads 0 .. bd-1 of a larger block (the others get no entries).
-__device__ __forceinline__ XcdLoop xcd_loop_t(long n, int vb, int nvb, int bd, int tid) {
-    if (tid >= bd) return {n, n, 1};
+__device__ __forceinline__ XcdLoop xcd_loop(long n, int vb, int nvb) {
     const int nb = nvb;
-    if (nb & 7) return {vb * (long)bd + tid, n, (long)nb * bd};
+    if (nb & 7) return {vb * (long)blockDim.x + threadIdx.x, n, (long)nb * blockDim.x};
     const int x = vb & 7, k = vb >> 3;
     const long chunk = (n + 7) / 8;
     const long b0 = x * chunk, b1 = b0 + chunk < n ? b0 + chunk : n;
-    return {b0 + k * (long)bd + tid, b1, (long)(nb >> 3) * bd};
-}
-__device__ __forceinline__ XcdLoop xcd_loop(long n, int vb, int nvb) {
-    return xcd_loop_t(n, vb, nvb, (int)blockDim.x, (int)threadIdx.x);
+    return {b0 + k * (long)blockDim.x + threadIdx.x, b1, (long)(nb >> 3) * blockDim.x};
 }
 
 // Decode an SK4 element index into its cell; false for padding entries.
@@ -622,13 +616,17 @@ __global__ __launch_bounds__(kBlock) void pcg_axpy_sk_kernel(
 // strips finish top-down about 3 us apart and the last one runs long after
 // most of the chip has gone idle; the SpMV of a region needs only z of its
 // own and the neighbouring strips, so it runs beneath the solve's tail.
+// With `check`, the launch first applies the stopping test of the iteration
+// whose p/r update ran in the previous launch (:453): every workgroup
+// evaluates it from the same partials, so all agree whether there is work.
 // Workgroups take tickets of the launch (tickets / launch = K + S + M):
 //   0 .. K-1        sigma: q.q per (strip, loader) in the order the solve's
 //                   loaders would sum it (bwd_sigma), published as tagged
 //                   words {half, epoch} (no fences) and as plain doubles for
 //                   later launches
-//   K .. K+S-1      the backward solve's strips (BwdFusedOp), each publishing
-//                   its completion (release, agent scope)
+//   K .. K+S-1      the backward solve's strips (BwdFusedOp): z is stored
+//                   with device scope (written through to memory), and after
+//                   those stores have completed each strip flags itself
 //   K+S .. K+S+M-1  SpMV: sigma from the tagged words, beta as spmv_scalars
 //                   forms it, then the virtual workgroups of pcg_spmv_sk_kernel
 //                   (the same entries, arithmetic and partials) in the order
@@ -636,8 +634,9 @@ __global__ __launch_bounds__(kBlock) void pcg_axpy_sk_kernel(
 // Every wait is on a workgroup with a smaller ticket, which started earlier,
 // so the launch completes at any residency; every wait is bounded and raises
 // the error flag on expiry.  Results are bit-identical to the separate
-// launches.  No workgroup on another XCD reads a strip's z before its flag
-// (z lines do not straddle strips), so none can hold a stale copy.
+// launches.  Kernel starts invalidate the caches, and no workgroup reads a
+// strip's z before its flag (a strip's entries are whole 1 KB groups, so no
+// cache line holds two strips), so none can hold a stale copy.
 // ---------------------------------------------------------------------------
 struct FusedSpmvArgs {
     const float* Adiag;
@@ -653,7 +652,7 @@ struct FusedSpmvArgs {
     const double* sig_prev;         // sigma_k partials (an earlier launch's), plain
     int first;                      // no beta (the initial solve)
     double* ws_partials;            // [nvb]
-    int nvb;                        // SpMV virtual workgroups, a multiple of 8
+    int nvb;                        // SpMV virtual workgroups (pcg_blocks)
     const unsigned* strip_flags;
     int K, M;
     int check;                      // the stopping test of the iteration (its p/r update's partials)

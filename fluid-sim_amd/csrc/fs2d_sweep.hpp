@@ -621,9 +621,8 @@ __device__ __forceinline__ double wave_max_of(const double* a, int n) {
     }
     return m;
 }
-// wave_max_of / wave_sum_of with every load of a lane in flight at once
-// (n <= 64 W): one memory round trip instead of n / 512; the sum's additions
-// in the same order.
+// wave_max_of with every load of a lane in flight at once (n <= 64 W): one
+// memory round trip instead of n / 512.
 template <int W>
 __device__ __forceinline__ double wave_max_of_wide(const double* a, int n) {
     const int lane = threadIdx.x & 63;
@@ -640,22 +639,9 @@ __device__ __forceinline__ double wave_max_of_wide(const double* a, int n) {
     }
     return m;
 }
-template <int W>
-__device__ __forceinline__ double wave_sum_of_wide(const double* a, int n) {
-    const int lane = threadIdx.x & 63;
-    double v[W];
-#pragma unroll
-    for (int u = 0; u < W; u++) v[u] = lane + 64 * u < n ? a[lane + 64 * u] : 0.0;
-    double t = 0;
-#pragma unroll
-    for (int u = 0; u < W; u++)
-        if (lane + 64 * u < n) t += v[u];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-    return t;
-}
-// Two such sums with all loads of both in flight together (n1 <= 64 W1,
-// n2 <= 64 W2): one memory round trip for both.
+// Two wave_sum_of sums with every load of both in flight together (n1 <=
+// 64 W1, n2 <= 64 W2): one memory round trip instead of several; the same
+// additions in the same order.
 template <int W1, int W2>
 __device__ __forceinline__ void wave_sum_of_wide2(const double* a, int n1, const double* b, int n2, double& sa,
                                                   double& sb) {
