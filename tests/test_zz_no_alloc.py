@@ -77,9 +77,12 @@ def args_for(g, n, out_ptr):
 
 def test_noalloc_projection_matches_run_stage():
     L, H = lib(), hip()
-    for n, mode, steps in ((160, fs2d.PICFLIP, 4), (72, fs2d.SEMILAGRANGIAN, 3)):
-        print("case n=%d mode=%d" % (n, mode), flush=True)
+    # the entry point runs the PCG as four launches per iteration; the handle
+    # runs the first case as two (fused_pcg), the second as four
+    for n, mode, steps, fused in ((160, fs2d.PICFLIP, 4, 1), (72, fs2d.SEMILAGRANGIAN, 3, 0)):
+        print("case n=%d mode=%d fused_pcg=%d" % (n, mode, fused), flush=True)
         g = scene(n, mode, steps)
+        g.set_tuning(pcg_grid=0, fused_pcg=fused)
         out = ctypes.c_void_p()
         assert H.hipMalloc(ctypes.byref(out), n * n * 4) == 0
         try:
