@@ -255,7 +255,7 @@ def ensemble_leg(count, steps, warmup, loop_steps, tune_cache=None, graph=True):
         b.step()
     tune = batch.autotune(iters=24, cache=tune_cache)
     if graph:
-        gbatch.set_tuning(tune["pcg_grid"], tune["sweep_prefetch"])
+        gbatch.set_tuning(tune["pcg_grid"], tune["sweep_prefetch"], tune["fused_pcg"])
     for _ in range(warmup):
         for b in twins:
             b.step()

@@ -53,6 +53,8 @@ struct InstBufs {
     Diag* dg;
     double* partials;
     unsigned long long* sk_handoff;  // hand-off rows of the batched solver passes
+    unsigned* fa_flags;              // the batch's p/r update run flags (b_fwd_axpy_kernel)
+    unsigned long long* fa_handoff;  // the batch's hand-off rows of that launch's forward solve
 };
 
 // Workgroup -> (simulation, workgroup within it) of one batched launch.
@@ -589,10 +591,13 @@ struct FwdMaker {
 };
 struct BwdMaker {
     int h;
-    __device__ BwdOp operator()(const InstBufs& I, int) const {
+    const int* nparts;  // with check: the members' p/r update partial counts
+    int check;          // the stopping test of iteration h (after b_fwd_axpy_kernel)
+    __device__ BwdOp operator()(const InstBufs& I, int b) const {
         // sigma_{h+1} goes where sigma_{h-1} was: the half of parity h + 1
         double* qq = I.qq + (size_t)(((h + 1) & 1) ? I.geom.S * kLoaders : 0);
-        return BwdOp{{I.q, I.P, I.gx_, I.gy_}, nullptr, {I.z}, I.sc, qq, h + 1};
+        return BwdOp{{I.q, I.P, I.gx_, I.gy_}, nullptr, {I.z}, I.sc, qq, h + 1, nullptr, 0u, check, I.rmax_part,
+                     check ? nparts[b] : 0};
     }
 };
 // The batched passes use a 4-supergroup ring: 50 KB of LDS instead of 98 KB,
@@ -621,6 +626,60 @@ __global__ __launch_bounds__(kSweepThreads) void b_sweep_kernel(const InstBufs* 
     Op op = mk(I, b);
     const SkCtl c{ticket, I.sk_handoff, err};
     sweep_run<Op, BACKWARD, PREFETCH>(op, I.geom, c, gs - sm.first[b], epoch, L);
+}
+
+
+// Iteration h's p/r update and forward solves of every member in one launch,
+// as fwd_axpy_kernel does for one simulation: the waves of M workgroups run
+// the plan's virtual workgroups, one at a time each, as strip-ordered runs of
+// their member (pcg_run, axpy_run_wave; alpha as b_axpy_kernel forms it; r
+// stored with device scope, each run flagged in its member's fa_flags after
+// the wave's stores complete), beside the
+// strips of every member, whose loaders wait for their member's runs over
+// their strip.  The stopping test moves to the backward launch (BwdMaker
+// check).  Each entry's update and the maximum of |r| are the separate
+// launches', so members stay bit-identical to single simulations.
+template <bool PREFETCH>
+__global__ __launch_bounds__(kSweepThreads) void b_fwd_axpy_kernel(const InstBufs* __restrict__ ins,
+                                                                   const int* __restrict__ count, PcgPlanBufs P,
+                                                                   StripMap sm, unsigned long long* ticket, int* err,
+                                                                   int M, const int* all_done_since, int h) {
+    __shared__ __align__(16) SweepLds<FwdFusedOp, kBatchRing> L;
+    const unsigned long long per = (unsigned long long)(M + sm.total);
+    if (*all_done_since <= h) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(ticket, per);
+        return;
+    }
+    const unsigned long long t = sweep_ticket(L, ticket);
+    const int idx = (int)(t % per);
+    const unsigned epoch = (unsigned)(t / per) + 1u;
+    if (idx < M) {
+        // runner w M + idx (wave w) takes the plan's entries k = runner,
+        // runner + 6 M, ... one at a time (axpy_run_wave)
+        constexpr int kWaves = kSweepThreads / 64;
+        const int total = *P.total, runners = kWaves * M;
+        RunBuf Rb;
+        for (int k = (int)(threadIdx.x >> 6) * M + idx; k < total; k += runners) {
+            const int b = P.map[k];
+            if (b < 0) continue;  // padding to a multiple of 8
+            const int vb = k - P.first[b];
+            const InstBufs& I = ins[b];
+            const float a = (float)(I.sc->sigma / I.sc->ws);
+            const __amdgpu_buffer_rsrc_t r_rs =
+                __builtin_amdgcn_make_buffer_rsrc(I.r, 0, (int)(I.geom.size() * 4), kBufferCfg);
+            long q0, q1;
+            pcg_run(I.geom.size(), vb, count[b], q0, q1);
+            axpy_run_wave(Rb, (const float4*)((h & 1) ? I.s0 : I.s1), (const float4*)I.w, (float4*)I.pk,
+                          (const float4*)I.r, r_rs, a, q0, q1, I.rmax_part + vb, I.fa_flags + vb, epoch);
+        }
+        return;
+    }
+    const int gs = idx - M;
+    const int b = sm.inst[gs];
+    const InstBufs& I = ins[b];
+    FwdFusedOp op{{I.r, I.P, I.fx, I.fy}, nullptr, {I.q}, I.sc, I.rmax_part, count[b], 0, I.fa_flags, I.geom, epoch};
+    const SkCtl c{ticket, I.fa_handoff, err};
+    sweep_run<FwdFusedOp, false, PREFETCH>(op, I.geom, c, gs - sm.first[b], epoch, L);
 }
 
 }  // namespace fs2d

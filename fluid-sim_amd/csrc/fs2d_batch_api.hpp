@@ -39,6 +39,13 @@ struct fs2d_batch {
     size_t plan_lds = 0;
     PcgPlanBufs plan{};
     bool sweep_prefetch = true;
+    // the p/r update with the forward solves in one launch (b_fwd_axpy_kernel,
+    // its own ticket; the stopping test in the backward launch).  Off by
+    // default: on the 256-member ensemble the batched forward solves already
+    // fill the device (1233 strips), leaving no idle CUs for the update to
+    // hide in (107.4 against 103.6 ms per step); the autotuner tries both.
+    bool fused_pcg = false;
+    unsigned long long* fa_ticket = nullptr;
     int* all_done_since = nullptr;
     const int* pcg_order = nullptr;  // members by SpMV/axpy virtual-workgroup weight, heaviest first (in d_maps)
     int* h_err = nullptr;
@@ -52,7 +59,7 @@ struct fs2d_batch {
     int graph_budget = 0;
     struct Captured {
         const int* maps;
-        int budget, pcg_grid, prefetch;
+        int budget, pcg_grid, prefetch, fused;
         hipGraphExec_t pre = nullptr, post = nullptr;
     };
     std::vector<Captured> graphs;
@@ -98,8 +105,9 @@ fs2d_status balloc(fs2d_batch* B, T** p, size_t n) {
     return FS2D_OK;
 }
 
-// Descriptor of member s (the hand-off rows are the batch's).
-InstBufs inst_bufs(const fs2d_sim* s, unsigned long long* sk_handoff) {
+// Descriptor of member s (the hand-off rows and run flags are the batch's).
+InstBufs inst_bufs(const fs2d_sim* s, unsigned long long* sk_handoff, unsigned* fa_flags,
+                   unsigned long long* fa_handoff) {
     InstBufs I;
     memset(&I, 0, sizeof I);  // padding too: descriptors are compared bytewise
     I.nx = s->nx; I.ny = s->ny; I.mode = s->mode; I.flags = s->flags; I.np = s->np;
@@ -120,6 +128,8 @@ InstBufs inst_bufs(const fs2d_sim* s, unsigned long long* sk_handoff) {
     I.pkey = s->pkey; I.counts = s->counts; I.starts = s->starts; I.cursor = s->cursor; I.sorted = s->sorted;
     I.tile_sums = s->tile_sums; I.dg = s->dg; I.partials = s->partials;
     I.sk_handoff = sk_handoff;
+    I.fa_flags = fa_flags;
+    I.fa_handoff = fa_handoff;
     return I;
 }
 
@@ -241,7 +251,7 @@ fs2d_status build_maps(fs2d_batch* B) {
 fs2d_status refresh_inst(fs2d_batch* B) {
     bool changed = false, counts = false;
     for (int b = 0; b < B->n; b++) {
-        InstBufs I = inst_bufs(B->sims[b], B->h_ins[b].sk_handoff);
+        InstBufs I = inst_bufs(B->sims[b], B->h_ins[b].sk_handoff, B->h_ins[b].fa_flags, B->h_ins[b].fa_handoff);
         if (memcmp(&I, &B->h_ins[b], sizeof I) != 0) {
             counts = counts || I.np != B->h_ins[b].np;
             B->h_ins[b] = I;
@@ -425,6 +435,27 @@ void bpcg_iterations(fs2d_batch* B, int h0, int h1, Fwd fwd, Bwd bwd) {
                                B->m_pcg.count, B->pcg_order, B->plan, h, B->all_done_since);
             blaunch(B, b_spmv_kernel, B->pcg_grid, kBlock, B->d_ins, B->m_pcg.count, B->plan, h);
         }
+        if (B->fused_pcg) {
+            {
+                Timed t(B, K_PCG_AXPY);
+                blaunch(B, b_ws_reduce_kernel, B->n, 64, B->d_ins, B->n, B->m_pcg.count);
+            }
+            {
+                // the p/r update beside the forward solves; one update workgroup per CU
+                Timed t(B, K_PCG_FWD);
+                const int M = B->sims[0]->ncu > 0 ? B->sims[0]->ncu : 256;
+                if (B->sweep_prefetch)
+                    blaunch(B, b_fwd_axpy_kernel<true>, M + S, kSweepThreads, B->d_ins, B->m_pcg.count, B->plan,
+                            B->strips, B->fa_ticket, B->werr, M, (const int*)B->all_done_since, h);
+                else
+                    blaunch(B, b_fwd_axpy_kernel<false>, M + S, kSweepThreads, B->d_ins, B->m_pcg.count, B->plan,
+                            B->strips, B->fa_ticket, B->werr, M, (const int*)B->all_done_since, h);
+            }
+            Timed t(B, K_PCG_BWD);
+            blaunch(B, bwd, S, kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
+                    BwdMaker{h, B->m_pcg.count, 1}, (const int*)B->all_done_since, h);
+            continue;
+        }
         {
             Timed t(B, K_PCG_AXPY);
             blaunch(B, b_ws_reduce_kernel, B->n, 64, B->d_ins, B->n, B->m_pcg.count);
@@ -437,8 +468,8 @@ void bpcg_iterations(fs2d_batch* B, int h0, int h1, Fwd fwd, Bwd bwd) {
         }
         {
             Timed t(B, K_PCG_BWD);
-            blaunch(B, bwd, S, kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr, BwdMaker{h},
-                    (const int*)B->all_done_since, h);
+            blaunch(B, bwd, S, kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
+                    BwdMaker{h, nullptr, 0}, (const int*)B->all_done_since, h);
         }
     }
 }
@@ -473,7 +504,7 @@ void bpcg_begin(fs2d_batch* B) {
     {
         Timed t(B, K_PCG_BWD);
         blaunch(B, b_sweep_kernel<BwdOp, true, BwdMaker>, S, kSweepThreads, B->d_ins, B->strips, B->sk_ticket,
-                B->werr, BwdMaker{-1}, (const int*)nullptr, 0);
+                B->werr, BwdMaker{-1, nullptr, 0}, (const int*)nullptr, 0);
     }
 }
 
@@ -631,6 +662,7 @@ fs2d_status bcapture_step(fs2d_batch* B, fs2d_batch::Captured& G) {
     G.budget = B->graph_budget;
     G.pcg_grid = B->pcg_grid;
     G.prefetch = B->sweep_prefetch ? 1 : 0;
+    G.fused = B->fused_pcg ? 1 : 0;
     std::vector<int> st_before, st_after;
     bstep_stages(B, st_before, st_after);
     const bool fuse = B->mode == FS2D_SEMILAGRANGIAN;
@@ -672,7 +704,7 @@ fs2d_status bstep_graph(fs2d_batch* B) {
     fs2d_batch::Captured* G = nullptr;
     for (auto& g : B->graphs)
         if (g.maps == B->d_maps && g.budget == B->graph_budget && g.pcg_grid == B->pcg_grid &&
-            g.prefetch == (B->sweep_prefetch ? 1 : 0))
+            g.prefetch == (B->sweep_prefetch ? 1 : 0) && g.fused == (B->fused_pcg ? 1 : 0))
             G = &g;
     if (!G) {
         if (B->graphs.size() >= 2) {
@@ -755,14 +787,19 @@ fs2d_status fs2d_batch_create(const fs2d_config* cfgs, int n, fs2d_batch** out) 
     for (int b = 0; b < n; b++) {
         fs2d_sim* s = B->sims[b];
         const int ns = (s->ny + 63) / 64;
-        unsigned long long* skh = nullptr;
-        if ((st = balloc(B, &skh, (size_t)ns * sk_handoff_stride(s->nx, 3))) != FS2D_OK) return bail(st);
-        B->h_ins[b] = inst_bufs(s, skh);
+        unsigned long long *skh = nullptr, *fah = nullptr;
+        unsigned* faf = nullptr;
+        if ((st = balloc(B, &skh, (size_t)ns * sk_handoff_stride(s->nx, 3))) != FS2D_OK ||
+            (st = balloc(B, &fah, (size_t)ns * sk_handoff_stride(s->nx, FwdFusedOp::NW))) != FS2D_OK ||
+            (st = balloc(B, &faf, (size_t)kMaxBlocks)) != FS2D_OK)
+            return bail(st);
+        B->h_ins[b] = inst_bufs(s, skh, faf, fah);
     }
     if ((st = balloc(B, &B->d_ins, (size_t)n)) != FS2D_OK) return bail(st);
     if ((st = build_maps(B)) != FS2D_OK) return bail(st);
     if ((st = balloc(B, &B->werr, 1)) != FS2D_OK || (st = balloc(B, &B->all_done_since, 1)) != FS2D_OK ||
-        (st = balloc(B, &B->sk_ticket, 1)) != FS2D_OK || (st = balloc(B, &B->d_status, 4 * (size_t)n)) != FS2D_OK)
+        (st = balloc(B, &B->sk_ticket, 1)) != FS2D_OK || (st = balloc(B, &B->fa_ticket, 1)) != FS2D_OK ||
+        (st = balloc(B, &B->d_status, 4 * (size_t)n)) != FS2D_OK)
         return bail(st);
     B->plan_lds = sizeof(int) * (size_t)n;
     if (hipHostMalloc((void**)&B->h_status, 4 * n * sizeof(int), hipHostMallocDefault) != hipSuccess ||

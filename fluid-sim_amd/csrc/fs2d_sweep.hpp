@@ -863,12 +863,13 @@ struct BwdOpT {
     int next_iter;         // iterations completed after this solve (0: initial solve)
     unsigned* strip_flags; // [S] (FUSED true): the launch's epoch once the strip's z is stored
     unsigned epoch;
-    int check;                    // FUSED: the iteration's stopping test (:453) first, as FwdOp's
-    const double* rmax_partials;  // FUSED: |r|_inf partials of the iteration's p/r update
+    int check;                    // the iteration's stopping test (:453) first, as FwdOp's, when
+                                  // the p/r update ran with the forward solve (fwd_axpy_kernel)
+    const double* rmax_partials;  // |r|_inf partials of the iteration's p/r update
     int n_partials;
     __device__ bool skip(bool first) const {
         if (sc->done) return true;
-        if (FUSED && check) {
+        if (check) {
             const double rn = n_partials <= 64 * 32 ? wave_max_of_wide<32>(rmax_partials, n_partials)
                                                     : wave_max_of(rmax_partials, n_partials);
             const bool conv = rn <= sc->tol * sc->rhs_norm;

@@ -47,8 +47,8 @@ std::string device_tag() {
 }
 
 // The cache file: one line per key, "key pcg_grid sweep_prefetch fused_pcg"
-// (a line without the last value means fused_pcg 1); the last line of a key
-// wins.
+// (a line without the last value keeps t's fused_pcg, the handle's default);
+// the last line of a key wins.
 bool cache_lookup(const char* path, const std::string& key, fs2d_tuning* t) {
     if (!path) return false;
     FILE* f = fopen(path, "r");
@@ -56,7 +56,7 @@ bool cache_lookup(const char* path, const std::string& key, fs2d_tuning* t) {
     char line[768], k[512];
     bool found = false;
     while (fgets(line, sizeof line, f)) {
-        int g = 0, p = 0, fu = 1;
+        int g = 0, p = 0, fu = t->fused_pcg;
         if (sscanf(line, "%511s %d %d %d", k, &g, &p, &fu) >= 3 && key == k) {
             t->pcg_grid = g;
             t->sweep_prefetch = p;
@@ -134,15 +134,17 @@ fs2d_status fs2d_batch_get_tuning(fs2d_batch* B, fs2d_tuning* t) {
     if (!B || !t) return FS2D_ERR_ARG;
     t->pcg_grid = B->pcg_grid;
     t->sweep_prefetch = B->sweep_prefetch ? 1 : 0;
-    t->fused_pcg = 0;
+    t->fused_pcg = B->fused_pcg ? 1 : 0;
     return FS2D_OK;
 }
 
 fs2d_status fs2d_batch_set_tuning(fs2d_batch* B, const fs2d_tuning* t) {
-    if (!B || !t || t->pcg_grid < 8 || (t->pcg_grid & 7)) return FS2D_ERR_ARG;
+    if (!B || !t || t->pcg_grid < 8 || (t->pcg_grid & 7) || (t->fused_pcg != 0 && t->fused_pcg != 1))
+        return FS2D_ERR_ARG;
     BHIPCHK(B, hipStreamSynchronize(B->stream));
     B->pcg_grid = t->pcg_grid;
     B->sweep_prefetch = t->sweep_prefetch != 0;
+    B->fused_pcg = t->fused_pcg != 0;
     B->tuned = true;
     return FS2D_OK;
 }
@@ -212,7 +214,7 @@ fs2d_status fs2d_batch_autotune(fs2d_batch* B, int iters, const char* cache_path
     snprintf(kb, sizeof kb, "%s|batch|n%d|sizes%016llx|mode%d|flags%d", device_tag().c_str(), B->n, h, B->mode,
              B->flags);
     const std::string key = kb;
-    fs2d_tuning best{B->pcg_grid, B->sweep_prefetch ? 1 : 0, 0};
+    fs2d_tuning best{B->pcg_grid, B->sweep_prefetch ? 1 : 0, B->fused_pcg ? 1 : 0};
     if (from_cache) *from_cache = 0;
     if (cache_lookup(cache_path, key, &best)) {
         if (from_cache) *from_cache = 1;
@@ -231,10 +233,12 @@ fs2d_status fs2d_batch_autotune(fs2d_batch* B, int iters, const char* cache_path
     for (auto* s : B->sims) full += (pcg_blocks(s) + 7) / 8 * 8;
     double best_t = 1e30;
     fs2d_status st = FS2D_OK;
+    for (int fu = 0; fu < 2 && st >= 0; fu++)
     for (int pf = 0; pf < 2 && st >= 0; pf++)
         for (int g : batch_grid_candidates(full, resident)) {
             B->pcg_grid = g;
             B->sweep_prefetch = pf != 0;
+            B->fused_pcg = fu != 0;
             double t_min = 1e30;
             for (int rep = 0; rep < 4 && st >= 0; rep++) {
                 BHIPCHK(B, hipStreamSynchronize(B->stream));
@@ -248,6 +252,7 @@ fs2d_status fs2d_batch_autotune(fs2d_batch* B, int iters, const char* cache_path
                 best_t = t_min;
                 best.pcg_grid = g;
                 best.sweep_prefetch = pf;
+                best.fused_pcg = fu;
             }
         }
     B->tol = tol;
@@ -255,6 +260,7 @@ fs2d_status fs2d_batch_autotune(fs2d_batch* B, int iters, const char* cache_path
     B->last_iters_hint = hint;
     B->pcg_grid = best.pcg_grid;
     B->sweep_prefetch = best.sweep_prefetch != 0;
+    B->fused_pcg = best.fused_pcg != 0;
     B->tuned = true;
     if (st < 0) return st;
     cache_store(cache_path, key, best);

@@ -448,17 +448,21 @@ class Batch:
     def tuning(self):
         t = Tuning()
         self._check(self.L.fs2d_batch_get_tuning(self.h, ctypes.byref(t)))
-        return {"pcg_grid": t.pcg_grid, "sweep_prefetch": t.sweep_prefetch}
+        return {"pcg_grid": t.pcg_grid, "sweep_prefetch": t.sweep_prefetch, "fused_pcg": t.fused_pcg}
 
-    def set_tuning(self, pcg_grid, sweep_prefetch):
-        t = Tuning(pcg_grid, sweep_prefetch, 0)
+    def set_tuning(self, pcg_grid, sweep_prefetch, fused_pcg=None):
+        """fused_pcg None keeps the batch's current choice."""
+        if fused_pcg is None:
+            fused_pcg = self.tuning()["fused_pcg"]
+        t = Tuning(pcg_grid, sweep_prefetch, fused_pcg)
         self._check(self.L.fs2d_batch_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self, iters=24, cache=None):
         t, fc = Tuning(), ctypes.c_int()
         self._check(self.L.fs2d_batch_autotune(self.h, iters, cache.encode() if cache else None, ctypes.byref(t),
                                                ctypes.byref(fc)))
-        return {"pcg_grid": t.pcg_grid, "sweep_prefetch": t.sweep_prefetch, "from_cache": bool(fc.value)}
+        return {"pcg_grid": t.pcg_grid, "sweep_prefetch": t.sweep_prefetch, "fused_pcg": t.fused_pcg,
+                "from_cache": bool(fc.value)}
 
     def set_timing(self, on):
         self._check(self.L.fs2d_batch_set_timing(self.h, int(on)))

@@ -252,7 +252,9 @@ const char* fs2d_status_string(int status);
  *     solve with the next iteration's SpMV, whose workgroups take each region
  *     as soon as the solve's strips around it have finished (the stopping
  *     test moves from the forward to the backward launch); 0: separate
- *     launches.  Ensembles report 0 and ignore it.
+ *     launches.  Ensembles (default 0): 1 runs the p/r update of every
+ *     member beside the forward solves in one launch (the SpMV stays
+ *     separate); it pays only when those solves leave CUs idle.
  * fs2d_autotune / fs2d_batch_autotune time the candidates on the handle's
  * current state (call them after a first step): `iters` PCG iterations of the
  * projection each (tolerance 0), best of three after a warm-up; the fastest is

@@ -128,16 +128,17 @@ def test_batch_tunings_bit_identical():
     for _ in range(3):
         ref.step()
     want = [[s.get(f) for f in FIELDS] for s in ref.sims]
-    for g, pf in ((8, 0), (64, 1), (512, 0), (2048, 1)):
+    assert ref.tuning()["fused_pcg"] == 0  # the batch's default
+    for g, pf, fu in ((8, 0, 1), (64, 1, 0), (512, 0, 0), (2048, 1, 1)):
         b = fs2d.Batch(members)
-        b.set_tuning(g, pf)
-        assert b.tuning() == {"pcg_grid": g, "sweep_prefetch": pf}
+        b.set_tuning(g, pf, fu)
+        assert b.tuning() == {"pcg_grid": g, "sweep_prefetch": pf, "fused_pcg": fu}
         for _ in range(3):
             b.step()
         for k, s in enumerate(b.sims):
             for f, w in zip(FIELDS, want[k]):
-                np.testing.assert_array_equal(s.get(f), w, err_msg="grid %d prefetch %d member %d field %d"
-                                              % (g, pf, k, f))
+                np.testing.assert_array_equal(s.get(f), w, err_msg="grid %d prefetch %d fused %d member %d field %d"
+                                              % (g, pf, fu, k, f))
         b.close()
 
 
@@ -165,7 +166,8 @@ def test_autotune_records_and_reuses(tmp_path):
     b.step()
     bt = b.autotune(iters=8, cache=cache)
     assert not bt["from_cache"] and bt["pcg_grid"] % 8 == 0
-    assert b.tuning() == {"pcg_grid": bt["pcg_grid"], "sweep_prefetch": bt["sweep_prefetch"]}
+    assert b.tuning() == {"pcg_grid": bt["pcg_grid"], "sweep_prefetch": bt["sweep_prefetch"],
+                          "fused_pcg": bt["fused_pcg"]}
     b2 = fs2d.Batch(pic_members([64, 48], seed=4))
     assert b2.autotune(iters=8, cache=cache)["from_cache"]
     print(open(cache).read())
