@@ -275,10 +275,11 @@ fs2d_status fs2d_autotune(fs2d_sim* sim, int iters, const char* cache_path, fs2d
 /*
  * applyProjection without allocation.  No reference counterpart: it is for
  * callers that cannot allocate device memory (code under HIP graph capture,
- * tight serving loops, harnesses that forbid hipMalloc).  It runs the same
- * kernels as fs2d_run_stage(sim, FS2D_STAGE_APPLY_PROJECTION) (ghost-fluid
- * assembly, MIC(0) factor, PCG; src/FluidSim2D.cpp:252-467) on the caller's
- * device arrays and gives the same pressure bit for bit.  Every other array
+ * tight serving loops, harnesses that forbid hipMalloc).  It runs the
+ * ghost-fluid assembly, the MIC(0) factor and the PCG of
+ * fs2d_run_stage(sim, FS2D_STAGE_APPLY_PROJECTION) (src/FluidSim2D.cpp:252-467),
+ * with four launches per PCG iteration (fused_pcg 0, whatever the grid), on the
+ * caller's device arrays and gives the same pressure bit for bit.  Every other array
  * lives in a static device buffer of about 28 MB per GPU, held from the
  * library's load until the process ends; grids whose solver layout has at most
  * 2^19 entries fit: ceil(size_y / 64) * 64 * ceil((size_x + 64) / 32) * 32
