@@ -89,6 +89,28 @@ def test_fused_pcg_bit_identical_across_strip_counts(nx, ny):
     b.close()
 
 
+def test_fused_pcg_bit_identical_2048():
+    """The benchmark's grid size, 8 steps (about 4000 PCG iterations, each a
+    fused forward and a fused backward launch whose SpMV workgroups read z
+    written through by strips on other XCDs within the launch): every field
+    equals the separate launches' after every step."""
+    n = 2048
+    a, b = smoke(n, 11), smoke(n, 11)
+    assert a.tuning()["fused_pcg"] == 1  # the default at this size
+    b.set_tuning(pcg_grid=0, fused_pcg=0)
+    total = 0
+    for step in range(8):
+        a.step()
+        b.step()
+        for f in FIELDS[:4]:
+            np.testing.assert_array_equal(a.get(f), b.get(f), err_msg="step %d field %d" % (step, f))
+        assert a.stats()["pcg_iters"] == b.stats()["pcg_iters"]
+        total += a.stats()["pcg_iters"]
+    print("2048^2: 8 steps, %d PCG iterations, bit-identical" % total)
+    a.close()
+    b.close()
+
+
 def test_fused_pcg_pic_and_captured_steps():
     """A PIC/FLIP dam break (EMPTY cells, a changing matrix) stepped with the
     fused launch, uncaptured and from captured graphs, against separate
