@@ -806,11 +806,12 @@ __global__ __launch_bounds__(kSweepThreads) void bwd_spmv_kernel(BwdFusedOp op, 
     const unsigned long long per = (unsigned long long)(f.K + g.S + f.M);
     const int idx = (int)(ticket % per);
     const unsigned epoch = (unsigned)(ticket / per) + 1u;
-    // sc->done is set before the launch or by this launch's stopping test
-    // (:453), which every workgroup evaluates for itself from the same
-    // partials: all agree whether the launch has work.  The strips evaluate
-    // it in BwdFusedOp::skip (the first one records it), beside their
-    // loaders' first loads.
+    // sc->done is set before the launch, or by this launch's stopping test
+    // (:453), which the sigma and SpMV workgroups evaluate for themselves from
+    // the same partials (so they agree whether there is work) and SpMV
+    // workgroup 0 records (|r|_inf; done, or the iteration count).  The
+    // strips do not wait for it: when it passes, their solve is wasted but
+    // harmless (nothing reads z or sigma once done is set).
     const bool strip = idx >= f.K && idx < f.K + g.S;
     __shared__ int sh_done;
     if (threadIdx.x < 64) {
@@ -819,6 +820,11 @@ __global__ __launch_bounds__(kSweepThreads) void bwd_spmv_kernel(BwdFusedOp op, 
             const double rn = f.nvb <= 64 * 32 ? wave_max_of_wide<32>(f.rmax_partials, f.nvb)
                                                : wave_max_of(f.rmax_partials, f.nvb);
             done = rn <= f.sc->tol * f.sc->rhs_norm;
+            if (idx == f.K + g.S && threadIdx.x == 0) {  // SpMV workgroup 0
+                f.sc->r_norm = rn;
+                if (done) f.sc->done = 1;
+                else f.sc->iter = op.next_iter;
+            }
         }
         if (threadIdx.x == 0) sh_done = done;
     }

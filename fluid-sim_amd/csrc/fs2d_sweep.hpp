@@ -863,12 +863,18 @@ struct BwdOpT {
     int next_iter;         // iterations completed after this solve (0: initial solve)
     unsigned* strip_flags; // [S] (FUSED true): the launch's epoch once the strip's z is stored
     unsigned epoch;
-    int check;                    // the iteration's stopping test (:453) first, as FwdOp's, when
-                                  // the p/r update ran with the forward solve (fwd_axpy_kernel)
+    int check;                    // FUSED false: the iteration's stopping test (:453) first, as
+                                  // FwdOp's, when the p/r update ran with the forward solves
+                                  // (b_fwd_axpy_kernel)
     const double* rmax_partials;  // |r|_inf partials of the iteration's p/r update
     int n_partials;
     __device__ bool skip(bool first) const {
         if (sc->done) return true;
+        // fused: the launch's SpMV workgroups apply the stopping test and
+        // record the count; the strips run regardless (the solve of the
+        // iteration that converges writes only z and sigma, which no later
+        // launch reads)
+        if (FUSED) return false;
         if (check) {
             const double rn = n_partials <= 64 * 32 ? wave_max_of_wide<32>(rmax_partials, n_partials)
                                                     : wave_max_of(rmax_partials, n_partials);
