@@ -616,9 +616,10 @@ __global__ __launch_bounds__(kBlock) void pcg_axpy_sk_kernel(
 // strips finish top-down about 3 us apart and the last one runs long after
 // most of the chip has gone idle; the SpMV of a region needs only z of its
 // own and the neighbouring strips, so it runs beneath the solve's tail.
-// With `check`, the launch first applies the stopping test of the iteration
-// whose p/r update ran in the previous launch (:453): every workgroup
-// evaluates it from the same partials, so all agree whether there is work.
+// With `check`, the launch applies the stopping test of the iteration whose
+// p/r update ran in the previous launch (:453): the sigma and SpMV workgroups
+// evaluate it from the same partials, so they agree whether there is work;
+// the strips run regardless (see bwd_spmv_kernel).
 // Workgroups take tickets of the launch (tickets / launch = K + S + M):
 //   0 .. K-1        sigma: q.q per (strip, loader) in the order the solve's
 //                   loaders would sum it (bwd_sigma), published as tagged
