@@ -7,6 +7,8 @@
     seeded uniformly at random with seed 0, FLIP blend 0.95, RK2 particle
     advection, dt 0.002, PCG to 1e-6): seeding bit-exact, then teacher-forced
     steps.
+  - A 700 x 600 PIC/FLIP dam break (ragged, above the two-launch PCG
+    threshold): teacher-forced steps of the fused PCG path.
   - The 256-member ensemble (PIC/FLIP dam breaks, sides 64..512): batch steps
     replayed from captured graphs, every member bit-identical to the same
     simulation stepped alone, and a 64^2 and a 512^2 member teacher-forced
@@ -82,6 +84,23 @@ def test_pic_dam_break_512_uniform_seeding_rk2():
     np.testing.assert_array_equal(pg, po.astype(np.float32))
     g.step()  # leave the zero-velocity start
     teacher_forced(g, o, 4, 1.0 / n, "512^2 PIC/FLIP dam break, uniform seeding, RK2")
+
+
+def test_pic_dam_break_700x600_two_launch_iteration():
+    """A ragged grid above the two-launch threshold (10 strips, 700 columns:
+    no multiple of 64 either way), PIC/FLIP, teacher-forced against the
+    oracle: the fused PCG path checked directly, not only against four
+    launches."""
+    nx, ny = 700, 600
+    dx = 1.0 / nx
+    kw = dict(k=2, dt=0.002, dx=dx, mode=fs2d.PICFLIP, alpha=0.05, tau=0.97)
+    c = scenes.walls(nx, ny)
+    c[1:int(0.6 * ny), 1:int(0.4 * nx)] = scenes.FLUID
+    g = fs2d.Sim(c, tol=1e-6, max_iters=500, **kw)
+    assert g.tuning()["fused_pcg"] == 1
+    o = O.Oracle(c, tol=1e-6, max_iters=500, sl_inplace=False, **kw)
+    g.step()  # leave the zero-velocity start
+    teacher_forced(g, o, 3, dx, "700x600 PIC/FLIP dam break, two-launch PCG")
 
 
 def test_rk2_particle_advection_stage():
