@@ -10,7 +10,8 @@ rendered from 8 blobs (seed 4), checkpoint every 8 steps.  Bars: one adjoint
 step within 1e-4 relative L2 of the oracle's; the 64-step gradient within
 1e-3 of FP64 finite differences along 16 random directions and of the
 oracle's reverse pass; gradients bit-identical across checkpoint intervals
-and repeated runs (the fixed-point scatter is order-independent).
+and repeated runs (the fixed-point scatter is order-independent), and at
+512^2 bit-identical between the two-launch and four-launch PCG iterations.
 """
 import numpy as np
 import pytest
@@ -77,8 +78,10 @@ def test_adjoint_step_matches_oracle():
     adj.close()
 
 
-def _gpu_gradient(n, steps, every, target):
+def _gpu_gradient(n, steps, every, target, fused_pcg=None):
     g = gpu_sim(n)
+    if fused_pcg is not None:
+        g.set_tuning(fused_pcg=fused_pcg)
     x0 = gstate(g)
     adj = fs2d.Adjoint(g, checkpoint_every=every, max_steps=steps)
     adj.forward(steps)
@@ -146,6 +149,22 @@ def test_adjoint_checkpointing_and_repeats_are_bit_identical():
     again = _gpu_gradient(n, steps, 8, target)
     for a, b in zip(again[2], ref[2]):
         np.testing.assert_array_equal(a, b)
+
+
+def test_adjoint_two_launch_pcg_is_bit_identical():
+    """At 512^2 the forward and the adjoint Poisson solves take the two-launch
+    PCG iteration by default; the rollout, its final state and the gradient
+    are bit-identical to the four-launch iteration's."""
+    n, steps = 512, 4
+    target = scenes.target_density(n)
+    g = gpu_sim(n)
+    assert g.tuning()["fused_pcg"] == 1
+    g.close()
+    two = _gpu_gradient(n, steps, 2, target, fused_pcg=1)
+    four = _gpu_gradient(n, steps, 2, target, fused_pcg=0)
+    for a, b in zip([two[1]] + list(two[2]) + list(two[3]), [four[1]] + list(four[2]) + list(four[3])):
+        np.testing.assert_array_equal(a, b)
+    assert np.abs(two[2][2]).max() > 0
 
 
 def test_autograd_rollout_matches_the_c_abi():
