@@ -53,6 +53,12 @@ for comparison, by looping fs2d_step over the same 256 simulations.  Value =
 total cells x steps / wall time, with no events; `kernel_ms_per_step` comes
 from 2 further uncaptured steps with every class timed.
 
+`pic`: the 512 x 512 PIC/FLIP dam break (fluid block 40% x 60% of the
+domain, 4 particles per fluid cell seeded uniformly at random with seed 0,
+FLIP blend 0.95, RK2 particle advection, dt 0.002, PCG to 1e-6): captured
+steps timed without events, and each particle stage's particles/s from 2
+further steps with every class timed.
+
 `adjoint`: the differentiable rollout, 256 x 256 all-fluid smoke, 64 steps,
 dt 0.01, velocity seeded like the 2048^2 workload (seed 3), loss = L2 of the
 final density against 8 blobs (seed 4): forward rollout with a checkpoint
@@ -308,6 +314,53 @@ def ensemble_leg(count, steps, warmup, loop_steps, tune_cache=None, graph=True):
     }
 
 
+def pic_leg(n=512, steps=20, warmup=2):
+    """The 512^2 PIC/FLIP dam break: one step from rest, `warmup` steps, then
+    `steps` timed steps replayed from captured graphs (no events); the
+    per-class breakdown, and each particle stage's rate, from 2 further steps
+    with every class timed."""
+    import fs2d
+    from fs2d import scenes
+    sim = fs2d.Sim(scenes.dam_break(n), k=2, dt=0.002, dx=1.0 / n, mode=fs2d.PICFLIP, alpha=0.05, tau=0.97,
+                   tol=1e-6, max_iters=500, seed=0, uniform_seeding=True, rk2=True)
+    npart = int(sim.stats()["n_particles"])
+    sim.step()  # leave the zero-velocity start
+    sim.set_graph(True)
+    for _ in range(warmup):
+        sim.step()
+    iters = []
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+        iters.append(sim.stats()["pcg_iters"])
+    sim.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    sim.set_graph(False)
+    sim.reset_kernel_times()
+    sim.set_timing(True)
+    for _ in range(PROFILE_STEPS):
+        sim.step()
+    kt = sim.kernel_times()
+    sim.set_timing(False)
+    tune = sim.tuning()
+    sim.close()
+    kms = {k: v[0] / PROFILE_STEPS for k, v in kt.items() if v[1]}
+    rates = {k: npart / (kms[k] / 1e3) for k in ("particle_sort", "p2g", "g2p", "particle_advect") if k in kms}
+    return {
+        "metric": "cell-steps/s of the PIC/FLIP step", "value": n * n / t, "unit": "cell-steps/s",
+        "ms_per_step": t * 1e3, "steps": steps, "warmup": warmup, "graph": True,
+        "particles": npart, "particle_steps_per_s": npart / t,
+        "config": {"workload": "%dx%d PIC/FLIP dam break: fluid block 40%% x 60%%, 4 particles per fluid cell "
+                               "seeded uniformly at random (seed 0), FLIP blend 0.95, RK2 particle advection, "
+                               "dt 0.002, MIC(0)-PCG tau 0.97 to 1e-6" % (n, n),
+                   "grid": [n, n], "pcg_iters_mean": float(np.mean(iters)), "pcg_iters_max": int(max(iters)),
+                   "tuning": tune},
+        "kernel_ms_per_step": kms,
+        "particles_per_s_by_stage": rates,
+    }
+
+
 def adjoint_leg(n=256, steps=64, every=8):
     """Forward rollout and checkpointed reverse pass of the smoke workload."""
     import fs2d
@@ -439,6 +492,7 @@ def main():
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=5)
     ap.add_argument("--loop-steps", type=int, default=2)
+    ap.add_argument("--no-pic", action="store_true", help="skip the 512^2 PIC/FLIP leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-grid3d", action="store_true", help="skip the 3D grid leg")
     ap.add_argument("--no-events", action="store_true",
@@ -447,7 +501,7 @@ def main():
                     help="skip the captured-graph twins of the 2048^2 and ensemble legs (profiling runs)")
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "fluid-sim_amd", "tune_cache.txt"),
                     help="autotuner record (read, or appended to after timing the candidates)")
-    ap.add_argument("--only", choices=["ensemble", "adjoint", "grid3d"],
+    ap.add_argument("--only", choices=["ensemble", "pic", "adjoint", "grid3d"],
                     help="run one leg alone and print its object (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -465,6 +519,9 @@ def main():
     if args.only == "ensemble":
         print(json.dumps(ensemble_leg(args.ensemble_members, args.ensemble_steps, args.warmup, args.loop_steps,
                                       args.tune_cache, not args.no_graph)))
+        return
+    if args.only == "pic":
+        print(json.dumps(pic_leg()))
         return
     if args.only == "adjoint":
         print(json.dumps(adjoint_leg()))
@@ -585,6 +642,8 @@ def main():
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ensemble_members, args.ensemble_steps, 2, args.loop_steps,
                                        args.tune_cache, not args.no_graph)
+    if not args.no_pic:
+        out["pic"] = pic_leg()
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()
     if not args.no_grid3d:
