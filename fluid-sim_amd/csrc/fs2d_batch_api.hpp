@@ -320,8 +320,11 @@ void bstage_level_set(fs2d_batch* B) {
         for (int k = 0; k < 16; k++) {
             const int d = k & 3;
             {
+                // four cells per thread (the tile map): a member whose passes
+                // have stopped returns at once, and after the first few passes
+                // all do, so the launch is mostly its workgroups' start-up
                 Timed t(B, K_LS_OTHER);
-                blaunch(B, b_ls_cand_sk_kernel, B->g_cells, kBlock, B->d_ins, B->m_cells, d, k == 0 ? 1 : 0,
+                blaunch(B, b_ls_cand_sk_kernel, B->g_tiles, kBlock, B->d_ins, B->m_tiles, d, k == 0 ? 1 : 0,
                         (k + 3) & 3, k);
             }
             {
