@@ -162,9 +162,10 @@ __global__ __launch_bounds__(kBlock) void b_assemble_kernel(const InstBufs* __re
 // g + 2 G, ..., and each round of G consecutive entries then spans members of
 // similar weight.
 struct PcgPlanBufs {
-    int* map;    // [sum of aligned counts]
-    int* first;  // [n]
-    int* total;  // [1]
+    int* map;     // [sum of aligned counts]
+    int* first;   // [n]
+    int* total;   // [1]
+    int* active;  // [n] 1 while the member iterates (its done flag as the plan saw it)
 };
 __device__ __forceinline__ int block_excl_scan(int v, int& total) {
     __shared__ int wsum[kBlock / 64];
@@ -216,6 +217,7 @@ __global__ __launch_bounds__(kBlock) void b_pcg_plan_kernel(const InstBufs* __re
     const int i = blockIdx.x;
     const int b = order[i];
     const int v = vals[i];
+    if (threadIdx.x == 0) P.active[b] = v != 0;
     if (v) {
         const int f = my_first, cnt = count[b];
         for (int k = threadIdx.x; k < v; k += kBlock) P.map[f + k] = k < cnt ? b : -1;
@@ -610,9 +612,15 @@ constexpr int kBatchRing = 4;
 // by the launch's strip count and every workgroup returns, so the epochs of
 // later launches stay as if every strip had taken its ticket.
 template <class Op, bool BACKWARD, class Maker, bool PREFETCH = true>
+// With `active` (the PCG plan's flags of this iteration), a strip of a member
+// that had converged before it returns once it holds its ticket, without the
+// loads its skip test would discard: its done flag makes skip() true anyway.
+// The flag is read beside the member's descriptor, so an iterating member's
+// strips wait no longer.
 __global__ __launch_bounds__(kSweepThreads) void b_sweep_kernel(const InstBufs* __restrict__ ins, StripMap sm,
                                                                 unsigned long long* ticket, int* err, Maker mk,
-                                                                const int* all_done_since = nullptr, int h = 0) {
+                                                                const int* all_done_since = nullptr, int h = 0,
+                                                                const int* active = nullptr) {
     __shared__ __align__(16) SweepLds<Op, kBatchRing> L;
     if (all_done_since && *all_done_since <= h) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(ticket, (unsigned long long)sm.total);
@@ -622,9 +630,11 @@ __global__ __launch_bounds__(kSweepThreads) void b_sweep_kernel(const InstBufs* 
     const int gs = (int)(t % (unsigned long long)sm.total);
     const unsigned epoch = (unsigned)(t / (unsigned long long)sm.total) + 1u;
     const int b = sm.inst[gs];
+    const bool idle = active && active[b] == 0;
     const InstBufs& I = ins[b];
     Op op = mk(I, b);
     const SkCtl c{ticket, I.sk_handoff, err};
+    if (idle) return;
     sweep_run<Op, BACKWARD, PREFETCH>(op, I.geom, c, gs - sm.first[b], epoch, L);
 }
 

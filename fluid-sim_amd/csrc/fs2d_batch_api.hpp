@@ -238,8 +238,8 @@ fs2d_status build_maps(fs2d_batch* B) {
     int* plan = nullptr;
     int plan_len = 0;
     for (int b = 0; b < n; b++) plan_len += (c_pcg[b] + 7) & ~7;
-    TRY(balloc(B, &plan, (size_t)plan_len + n + 1));
-    B->plan = PcgPlanBufs{plan, plan + plan_len, plan + plan_len + n};
+    TRY(balloc(B, &plan, (size_t)plan_len + 2 * (size_t)n + 1));
+    B->plan = PcgPlanBufs{plan, plan + plan_len, plan + plan_len + n, plan + plan_len + n + 1};
     // untuned: one workgroup per virtual workgroup of a full iteration (the
     // hardware then balances members of different weights), at most 32768
     if (!B->tuned) B->pcg_grid = std::max(8, std::min(32768, plan_len));
@@ -332,16 +332,16 @@ void bstage_level_set(fs2d_batch* B) {
                 switch (d) {
                     case 0: blaunch(B, b_sweep_kernel<ClosestSweepOp<0, 0>, false, ClosestSweepMaker<0, 0>>, S,
                                     kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
-                                    ClosestSweepMaker<0, 0>{k}, (const int*)nullptr, 0); break;
+                                    ClosestSweepMaker<0, 0>{k}, (const int*)nullptr, 0, (const int*)nullptr); break;
                     case 1: blaunch(B, b_sweep_kernel<ClosestSweepOp<1, 0>, false, ClosestSweepMaker<1, 0>>, S,
                                     kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
-                                    ClosestSweepMaker<1, 0>{k}, (const int*)nullptr, 0); break;
+                                    ClosestSweepMaker<1, 0>{k}, (const int*)nullptr, 0, (const int*)nullptr); break;
                     case 2: blaunch(B, b_sweep_kernel<ClosestSweepOp<0, 1>, false, ClosestSweepMaker<0, 1>>, S,
                                     kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
-                                    ClosestSweepMaker<0, 1>{k}, (const int*)nullptr, 0); break;
+                                    ClosestSweepMaker<0, 1>{k}, (const int*)nullptr, 0, (const int*)nullptr); break;
                     default: blaunch(B, b_sweep_kernel<ClosestSweepOp<1, 1>, false, ClosestSweepMaker<1, 1>>, S,
                                      kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
-                                     ClosestSweepMaker<1, 1>{k}, (const int*)nullptr, 0); break;
+                                     ClosestSweepMaker<1, 1>{k}, (const int*)nullptr, 0, (const int*)nullptr); break;
                 }
             }
         }
@@ -367,7 +367,7 @@ void bstage_level_set(fs2d_batch* B) {
             {
                 Timed t(B, K_LS_SWEEP);
                 blaunch(B, b_sweep_kernel<EikSweepOp, false, EikSweepMaker>, S, kSweepThreads, B->d_ins, B->strips,
-                        B->sk_ticket, B->werr, EikSweepMaker{}, (const int*)nullptr, 0);
+                        B->sk_ticket, B->werr, EikSweepMaker{}, (const int*)nullptr, 0, (const int*)nullptr);
             }
             Timed t(B, K_LS_OTHER);
             if (k < 15)
@@ -456,7 +456,7 @@ void bpcg_iterations(fs2d_batch* B, int h0, int h1, Fwd fwd, Bwd bwd) {
             }
             Timed t(B, K_PCG_BWD);
             blaunch(B, bwd, S, kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
-                    BwdMaker{h, B->m_pcg.count, 1}, (const int*)B->all_done_since, h);
+                    BwdMaker{h, B->m_pcg.count, 1}, (const int*)B->all_done_since, h, (const int*)nullptr);
             continue;
         }
         {
@@ -467,12 +467,12 @@ void bpcg_iterations(fs2d_batch* B, int h0, int h1, Fwd fwd, Bwd bwd) {
         {
             Timed t(B, K_PCG_FWD);
             blaunch(B, fwd, S, kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
-                    FwdMaker{B->m_pcg.count, 1}, (const int*)B->all_done_since, h);
+                    FwdMaker{B->m_pcg.count, 1}, (const int*)B->all_done_since, h, (const int*)B->plan.active);
         }
         {
             Timed t(B, K_PCG_BWD);
             blaunch(B, bwd, S, kSweepThreads, B->d_ins, B->strips, B->sk_ticket, B->werr,
-                    BwdMaker{h, nullptr, 0}, (const int*)B->all_done_since, h);
+                    BwdMaker{h, nullptr, 0}, (const int*)B->all_done_since, h, (const int*)B->plan.active);
         }
     }
 }
@@ -497,17 +497,17 @@ void bpcg_begin(fs2d_batch* B) {
     {
         Timed t(B, K_FACTOR);
         blaunch(B, b_sweep_kernel<FactorOp, false, FactorMaker>, S, kSweepThreads, B->d_ins, B->strips,
-                B->sk_ticket, B->werr, FactorMaker{B->m_sk.count}, (const int*)nullptr, 0);
+                B->sk_ticket, B->werr, FactorMaker{B->m_sk.count}, (const int*)nullptr, 0, (const int*)nullptr);
     }
     {
         Timed t(B, K_PCG_FWD);
         blaunch(B, b_sweep_kernel<FwdOp, false, FwdMaker>, S, kSweepThreads, B->d_ins, B->strips, B->sk_ticket,
-                B->werr, FwdMaker{B->m_sk.count, 0}, (const int*)nullptr, 0);
+                B->werr, FwdMaker{B->m_sk.count, 0}, (const int*)nullptr, 0, (const int*)nullptr);
     }
     {
         Timed t(B, K_PCG_BWD);
         blaunch(B, b_sweep_kernel<BwdOp, true, BwdMaker>, S, kSweepThreads, B->d_ins, B->strips, B->sk_ticket,
-                B->werr, BwdMaker{-1, nullptr, 0}, (const int*)nullptr, 0);
+                B->werr, BwdMaker{-1, nullptr, 0}, (const int*)nullptr, 0, (const int*)nullptr);
     }
 }
 
