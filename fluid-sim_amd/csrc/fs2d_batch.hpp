@@ -602,10 +602,10 @@ struct BwdMaker {
                      check ? nparts[b] : 0};
     }
 };
-// The batched passes use a 4-supergroup ring: 50 KB of LDS instead of 98 KB,
-// so two workgroups fit a CU and a launch over a thousand strips needs half
-// the rounds of residency.  The ring depth changes no arithmetic.
-constexpr int kBatchRing = 4;
+// The batched passes use the single launch's ring depth (sweep_ring).  A
+// 4-supergroup ring would halve the LDS, but the sweep workgroup's 134 VGPRs
+// already keep a second one off the CU, and the shorter lookahead made the
+// ensemble's solves slower.  The ring depth changes no arithmetic.
 // PREFETCH: loaders issue their first loads before the skip test (which a
 // skipped strip then wastes).  all_done_since (PCG solves): when every member
 // had converged before iteration h, the first workgroup advances the ticket
@@ -621,7 +621,7 @@ __global__ __launch_bounds__(kSweepThreads) void b_sweep_kernel(const InstBufs* 
                                                                 unsigned long long* ticket, int* err, Maker mk,
                                                                 const int* all_done_since = nullptr, int h = 0,
                                                                 const int* active = nullptr) {
-    __shared__ __align__(16) SweepLds<Op, kBatchRing> L;
+    __shared__ __align__(16) SweepLds<Op, sweep_ring<Op>()> L;
     if (all_done_since && *all_done_since <= h) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(ticket, (unsigned long long)sm.total);
         return;
@@ -654,7 +654,7 @@ __global__ __launch_bounds__(kSweepThreads) void b_fwd_axpy_kernel(const InstBuf
                                                                    const int* __restrict__ count, PcgPlanBufs P,
                                                                    StripMap sm, unsigned long long* ticket, int* err,
                                                                    int M, const int* all_done_since, int h) {
-    __shared__ __align__(16) SweepLds<FwdFusedOp, kBatchRing> L;
+    __shared__ __align__(16) SweepLds<FwdFusedOp, sweep_ring<FwdFusedOp>()> L;
     const unsigned long long per = (unsigned long long)(M + sm.total);
     if (*all_done_since <= h) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(ticket, per);
